@@ -1,0 +1,172 @@
+"""ctypes binding of libclpt_hip.so (C ABI declared in include/clhip.h).
+
+The renderer has no CPU fallback: if the shared library is missing this
+module raises ImportError, and every call into it on a machine without a
+HIP device returns an error status that is raised as ClhipError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libclpt_hip.so")
+
+CLHIP_MAX_DEPTH = 32
+
+INTEGRATOR_DIRECT_LIGHTING = 1
+INTEGRATOR_PATH_TRACING = 2
+
+MTL_DIFFUSE, MTL_METAL, MTL_GLASS = 0, 1, 2
+EMITTER_DIRECTIONAL, EMITTER_HDRIDOME = 0, 1
+
+
+class ClhipError(RuntimeError):
+    """A clhip_* call returned a nonzero status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"clhip error {code}: {msg}")
+        self.code = code
+
+
+class clhip_mesh(C.Structure):
+    _fields_ = [
+        ("vertices", C.POINTER(C.c_float)),
+        ("indices", C.POINTER(C.c_uint32)),
+        ("normals", C.POINTER(C.c_float)),
+        ("tangents", C.POINTER(C.c_float)),
+        ("uvs", C.POINTER(C.c_float)),
+        ("num_vertices", C.c_uint32),
+        ("num_triangles", C.c_uint32),
+    ]
+
+
+class clhip_material(C.Structure):
+    _fields_ = [
+        ("type", C.c_uint32),
+        ("reflectance", C.c_float * 3),
+        ("roughness", C.c_float),
+        ("eta", C.c_float * 3),
+        ("k", C.c_float * 3),
+        ("interior_ior", C.c_float),
+    ]
+
+
+class clhip_emitter(C.Structure):
+    _fields_ = [
+        ("type", C.c_uint32),
+        ("intensity", C.c_float * 3),
+        ("direction", C.c_float * 3),
+        ("angular_radius", C.c_float),
+        ("texture", C.c_int32),
+    ]
+
+
+class clhip_params(C.Structure):
+    _fields_ = [
+        ("max_depth", C.c_uint32),
+        ("rr_depth", C.c_uint32),
+        ("view", C.c_uint32),
+        ("samples_per_batch", C.c_uint32),
+    ]
+
+
+_D = CLHIP_MAX_DEPTH + 1
+
+
+class clhip_stats(C.Structure):
+    _fields_ = [
+        ("rays", C.c_uint64 * _D),
+        ("hits", C.c_uint64 * _D),
+        ("shadow", C.c_uint64 * _D),
+        ("visible", C.c_uint64 * _D),
+        ("trace_nodes", C.c_uint64),
+        ("trace_tris", C.c_uint64),
+        ("shadow_nodes", C.c_uint64),
+        ("shadow_tris", C.c_uint64),
+        ("trace_launches", C.c_uint64),
+        ("shadow_launches", C.c_uint64),
+        ("trace_ms", C.c_double),
+        ("shade_ms", C.c_double),
+        ("shadow_ms", C.c_double),
+        ("render_ms", C.c_double),
+    ]
+
+
+P = C.c_void_p
+u32, i32, f32p, u32p = C.c_uint32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)
+
+# name -> (restype, argtypes); this list is also what the CPU test checks
+# against include/clhip.h.
+SIGNATURES = {
+    "clhip_create": (C.c_int, [u32, u32, u32, C.POINTER(P)]),
+    "clhip_destroy": (C.c_int, [P]),
+    "clhip_last_error": (C.c_char_p, [P]),
+    "clhip_create_texture": (C.c_int, [P, f32p, u32, u32, u32, C.POINTER(i32)]),
+    "clhip_build_blas": (C.c_int, [P, C.POINTER(clhip_mesh), u32]),
+    "clhip_build_tlas": (C.c_int, [P, u32p, u32p, f32p, u32]),
+    "clhip_build_materials": (C.c_int, [P, C.POINTER(clhip_material), u32]),
+    "clhip_build_entities": (C.c_int, [P, u32p, u32]),
+    "clhip_build_emitters": (C.c_int, [P, C.POINTER(clhip_emitter), u32, i32]),
+    "clhip_set_camera": (C.c_int, [P, f32p, f32p, f32p, f32p]),
+    "clhip_set_params": (C.c_int, [P, C.POINTER(clhip_params)]),
+    "clhip_render": (C.c_int, [P, u32]),
+    "clhip_sync": (C.c_int, [P]),
+    "clhip_resize": (C.c_int, [P, u32, u32]),
+    "clhip_reset": (C.c_int, [P]),
+    "clhip_get_film_size": (C.c_int, [P, u32p, u32p, u32p]),
+    "clhip_readback": (C.c_int, [P, f32p, C.c_size_t]),
+    "clhip_readback_sums": (C.c_int, [P, C.POINTER(C.c_double), C.c_size_t]),
+    "clhip_counters": (C.c_int, [P, C.POINTER(clhip_stats)]),
+    "clhip_set_timing": (C.c_int, [P, C.c_int]),
+    "clhip_rng_batch": (C.c_int, [P, u32p, u32, u32, u32p, f32p]),
+    "clhip_camera_rays": (C.c_int, [P, u32p, u32, f32p]),
+    "clhip_bsdf_batch": (C.c_int, [P, u32p, f32p, f32p, f32p, u32, f32p, f32p, f32p, f32p, f32p]),
+    "clhip_emitter_batch": (C.c_int, [P, f32p, u32, f32p, f32p, f32p, u32p]),
+    "clhip_dome_eval_batch": (C.c_int, [P, f32p, u32, f32p, f32p]),
+    "clhip_dome_tables": (C.c_int, [P, f32p, f32p, f32p, f32p, u32p, u32p]),
+    "clhip_trace_batch": (C.c_int, [P, f32p, u32, f32p, f32p, u32p, u32p]),
+    "clhip_occlusion_batch": (C.c_int, [P, f32p, u32, C.POINTER(C.c_uint8)]),
+    "clhip_instance_inverses": (C.c_int, [P, f32p, u32]),
+}
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build the HIP extension first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load_library()
+    return _LIB
+
+
+def last_error(ctx=None) -> str:
+    msg = lib().clhip_last_error(ctx)
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != 0:
+        raise ClhipError(rc, last_error(ctx))
+
+
+def fptr(a):
+    return a.ctypes.data_as(f32p) if a is not None else None
+
+
+def uptr(a):
+    return a.ctypes.data_as(u32p) if a is not None else None

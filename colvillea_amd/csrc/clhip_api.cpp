@@ -1,0 +1,1210 @@
+/*
+ * clhip_api.cpp -- the C ABI (include/clhip.h): context, scene upload,
+ * BVH build, the bounce loop of Integrator::render and the batch parity
+ * entry points.  Host code; the kernels are in kernels.hip.
+ */
+#include "../../include/clhip.h"
+#include "bvh_build.h"
+#include "clpt_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace
+{
+thread_local std::string g_last_error;
+
+struct DevBuf
+{
+    void*  p     = nullptr;
+    size_t bytes = 0;
+    DevBuf()     = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p     = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n)
+    {
+        if (n <= bytes && p)
+            return hipSuccess;
+        release();
+        if (n == 0)
+            n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess)
+            bytes = n;
+        else
+            p = nullptr;
+        return e;
+    }
+    template <typename T>
+    T* as() const
+    {
+        return static_cast<T*>(p);
+    }
+};
+
+struct TexHost
+{
+    std::unique_ptr<DevBuf> buf;
+    uint32_t                w = 0, h = 0;
+};
+
+struct MeshInfo
+{
+    uint32_t tri_base = 0, ntris = 0, nverts = 0, root = 0, depth = 0;
+    float    lo[3]    = {0, 0, 0}, hi[3] = {0, 0, 0};
+};
+
+struct TimedLaunch
+{
+    hipEvent_t a, b;
+    int        kind; /* 0 trace, 1 shade, 2 shadow */
+};
+} // namespace
+
+struct clhip_ctx
+{
+    int         device  = 0;
+    hipStream_t stream  = nullptr;
+    uint32_t    width   = 0, height = 0, itype = 0;
+    uint32_t    num_cus = 0;
+    std::string err;
+
+    std::vector<TexHost> textures;
+
+    /* geometry */
+    std::vector<Node4>       blas_nodes;
+    std::vector<MeshInfo>    meshes;
+    std::vector<MeshRec>     mesh_recs;
+    bool                     have_blas = false, have_tlas = false;
+    DevBuf                   d_nodes, d_tris, d_meshes, d_pos, d_nrm, d_tan, d_uv, d_idx, d_inst;
+    std::vector<InstanceRec> inst;
+    std::vector<uint32_t>    inst_entity;
+    uint32_t                 tlas_root = 0, tlas_depth = 0;
+    bool                     inst_dirty = true;
+
+    /* materials / entities */
+    std::vector<MaterialRec> mtl;
+    std::vector<uint32_t>    entity_material;
+    DevBuf                   d_mtl;
+
+    /* emitters */
+    std::vector<EmitterRec> emitters;
+    int32_t                 dome_index = -1, dome_tex = -1;
+    uint32_t                dome_w = 0, dome_h = 0;
+    DevBuf                  d_emitters, d_pcond, d_cdfcond, d_pv, d_cdfv, d_rowint;
+
+    /* camera / parameters */
+    FrameView    fr{};
+    bool         have_camera = false;
+    clhip_params params{5u, 0u, 0u, 0u};
+
+    /* render state */
+    DevBuf   d_accum, d_out;
+    DevBuf   d_q[2][3], d_hit, d_prim, d_sq[3], d_gstack, d_bc, d_tot, d_ctr;
+    size_t   qcap = 0;
+    uint32_t spp  = 0;
+    uint32_t blocks = 0;
+
+    /* timing */
+    bool                      timing = false;
+    std::vector<TimedLaunch>  timed;
+    std::vector<hipEvent_t>   event_pool;
+    hipEvent_t                ev_begin = nullptr, ev_end = nullptr;
+    bool                      have_render_events = false;
+    uint64_t                  trace_launches = 0, shadow_launches = 0;
+};
+
+namespace
+{
+int fail(clhip_ctx* ctx, int code, const char* fmt, ...)
+{
+    char    buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx)
+        ctx->err = buf;
+    g_last_error = buf;
+    return code;
+}
+
+#define CLHIP_CHECK(ctx, expr)                                                                           \
+    do                                                                                                   \
+    {                                                                                                    \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return fail((ctx), CLHIP_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                       \
+    } while (0)
+
+template <typename T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v)
+{
+    hipError_t e = b.alloc(std::max<size_t>(v.size() * sizeof(T), 16));
+    if (e != hipSuccess)
+        return e;
+    if (v.empty())
+        return hipSuccess;
+    return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+/* Inverse of an affine 3x4 by the adjugate in float64 (the oracle's
+ * orc_inverse3x4 uses the identical expression). */
+void inverse3x4(const float m[12], float out[12])
+{
+    const double a00 = m[0], a01 = m[1], a02 = m[2], t0 = m[3];
+    const double a10 = m[4], a11 = m[5], a12 = m[6], t1 = m[7];
+    const double a20 = m[8], a21 = m[9], a22 = m[10], t2 = m[11];
+    const double c00 = a11 * a22 - a12 * a21;
+    const double c01 = a12 * a20 - a10 * a22;
+    const double c02 = a10 * a21 - a11 * a20;
+    const double det = a00 * c00 + a01 * c01 + a02 * c02;
+    double       r[12];
+    r[0]  = c00 / det;
+    r[1]  = (a02 * a21 - a01 * a22) / det;
+    r[2]  = (a01 * a12 - a02 * a11) / det;
+    r[4]  = c01 / det;
+    r[5]  = (a00 * a22 - a02 * a20) / det;
+    r[6]  = (a02 * a10 - a00 * a12) / det;
+    r[8]  = c02 / det;
+    r[9]  = (a01 * a20 - a00 * a21) / det;
+    r[10] = (a00 * a11 - a01 * a10) / det;
+    r[3]  = -(r[0] * t0 + r[1] * t1 + r[2] * t2);
+    r[7]  = -(r[4] * t0 + r[5] * t1 + r[6] * t2);
+    r[11] = -(r[8] * t0 + r[9] * t1 + r[10] * t2);
+    for (int i = 0; i < 12; ++i)
+        out[i] = (float)r[i];
+}
+
+SceneView scene_view(const clhip_ctx* c)
+{
+    SceneView s{};
+    s.nodes      = c->d_nodes.as<const Node4>();
+    s.tris       = c->d_tris.as<const TriRec>();
+    s.inst       = c->d_inst.as<const InstanceRec>();
+    s.meshes     = c->d_meshes.as<const MeshRec>();
+    s.pos        = c->d_pos.as<const float>();
+    s.nrm        = c->d_nrm.as<const float>();
+    s.tan        = c->d_tan.as<const float>();
+    s.uv         = c->d_uv.as<const float>();
+    s.idx        = c->d_idx.as<const uint32_t>();
+    s.mtl        = c->d_mtl.as<const MaterialRec>();
+    s.emitters   = c->d_emitters.as<const EmitterRec>();
+    s.nemitters  = (uint32_t)c->emitters.size();
+    s.dome_index = c->dome_index;
+    if (c->dome_tex >= 0)
+    {
+        s.dome_tex = c->textures[c->dome_tex].buf->as<const float4>();
+        s.dome_w   = c->dome_w;
+        s.dome_h   = c->dome_h;
+    }
+    s.pcond     = c->d_pcond.as<const float>();
+    s.cdfcond   = c->d_cdfcond.as<const float>();
+    s.pv        = c->d_pv.as<const float>();
+    s.cdfv      = c->d_cdfv.as<const float>();
+    s.tlas_root = c->tlas_root;
+    s.ninst     = (uint32_t)c->inst.size();
+    return s;
+}
+
+/* Resolve entity -> material for every instance and upload. */
+int sync_instances(clhip_ctx* c)
+{
+    if (!c->have_tlas)
+        return fail(c, CLHIP_E_STATE, "no TLAS: call clhip_build_tlas first");
+    if (!c->inst_dirty)
+        return CLHIP_OK;
+    for (size_t i = 0; i < c->inst.size(); ++i)
+    {
+        const uint32_t e = c->inst_entity[i];
+        uint32_t       m = 0;
+        if (e < c->entity_material.size())
+            m = c->entity_material[e];
+        else if (c->entity_material.empty())
+            m = e; /* no entity table: entity index doubles as material index */
+        else
+            return fail(c, CLHIP_E_INVALID, "instance %zu references entity %u of %zu", i, e,
+                        c->entity_material.size());
+        if (m >= c->mtl.size())
+            return fail(c, CLHIP_E_STATE, "instance %zu needs material %u but only %zu are built", i, m, c->mtl.size());
+        c->inst[i].material = m;
+        c->inst[i].entity   = e;
+    }
+    CLHIP_CHECK(c, upload(c->d_inst, c->inst));
+    c->inst_dirty = false;
+    return CLHIP_OK;
+}
+
+int ensure_render_buffers(clhip_ctx* c, size_t paths)
+{
+    const size_t npix = (size_t)c->width * c->height;
+    CLHIP_CHECK(c, c->d_accum.alloc(3 * npix * sizeof(double)));
+    CLHIP_CHECK(c, c->d_out.alloc(npix * sizeof(float4)));
+    if (paths > c->qcap)
+    {
+        for (int q = 0; q < 2; ++q)
+            for (int k = 0; k < 3; ++k)
+                CLHIP_CHECK(c, c->d_q[q][k].alloc(paths * sizeof(float4)));
+        CLHIP_CHECK(c, c->d_hit.alloc(paths * sizeof(float4)));
+        CLHIP_CHECK(c, c->d_prim.alloc(paths * sizeof(uint32_t)));
+        CLHIP_CHECK(c, c->d_sq[0].alloc(paths * sizeof(float4)));
+        CLHIP_CHECK(c, c->d_sq[1].alloc(paths * sizeof(float4)));
+        CLHIP_CHECK(c, c->d_sq[2].alloc(paths * sizeof(float2)));
+        c->qcap = paths;
+    }
+    const size_t gthreads = (size_t)c->blocks * 256;
+    CLHIP_CHECK(c, c->d_gstack.alloc(gthreads * 112 * sizeof(uint32_t)));
+    if (!c->d_bc.p)
+    {
+        CLHIP_CHECK(c, c->d_bc.alloc(sizeof(BatchCounters)));
+        CLHIP_CHECK(c, hipMemsetAsync(c->d_bc.p, 0, sizeof(BatchCounters), c->stream));
+    }
+    if (!c->d_tot.p)
+    {
+        CLHIP_CHECK(c, c->d_tot.alloc(sizeof(TotalCounters)));
+        CLHIP_CHECK(c, hipMemsetAsync(c->d_tot.p, 0, sizeof(TotalCounters), c->stream));
+    }
+    if (!c->d_ctr.p)
+    {
+        CLHIP_CHECK(c, c->d_ctr.alloc(4 * sizeof(unsigned long long)));
+        CLHIP_CHECK(c, hipMemsetAsync(c->d_ctr.p, 0, 4 * sizeof(unsigned long long), c->stream));
+    }
+    return CLHIP_OK;
+}
+
+hipEvent_t pool_event(clhip_ctx* c)
+{
+    if (!c->event_pool.empty())
+    {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void recycle_events(clhip_ctx* c)
+{
+    for (auto& t : c->timed)
+    {
+        c->event_pool.push_back(t.a);
+        c->event_pool.push_back(t.b);
+    }
+    c->timed.clear();
+}
+
+/* Scratch device buffer for the batch entry points. */
+struct Tmp
+{
+    DevBuf b;
+    template <typename T>
+    hipError_t from_host(const T* h, size_t count)
+    {
+        hipError_t e = b.alloc(std::max<size_t>(count * sizeof(T), 16));
+        if (e != hipSuccess || count == 0)
+            return e;
+        return hipMemcpy(b.p, h, count * sizeof(T), hipMemcpyHostToDevice);
+    }
+    hipError_t zeros(size_t bytes)
+    {
+        hipError_t e = b.alloc(std::max<size_t>(bytes, 16));
+        if (e != hipSuccess)
+            return e;
+        return hipMemset(b.p, 0, std::max<size_t>(bytes, 16));
+    }
+    template <typename T>
+    hipError_t to_host(T* h, size_t count)
+    {
+        if (count == 0)
+            return hipSuccess;
+        return hipMemcpy(h, b.p, count * sizeof(T), hipMemcpyDeviceToHost);
+    }
+};
+} // namespace
+
+extern "C" {
+
+const char* clhip_last_error(const clhip_ctx* ctx)
+{
+    if (ctx)
+        return ctx->err.c_str();
+    return g_last_error.c_str();
+}
+
+int clhip_create(uint32_t width, uint32_t height, uint32_t integrator_type, clhip_ctx** out)
+{
+    if (!out)
+        return fail(nullptr, CLHIP_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 30))
+        return fail(nullptr, CLHIP_E_INVALID, "bad film size %ux%u", width, height);
+    if (integrator_type != CLHIP_INTEGRATOR_PATH_TRACING && integrator_type != CLHIP_INTEGRATOR_DIRECT_LIGHTING)
+        return fail(nullptr, CLHIP_E_INVALID, "unknown integrator type %u", integrator_type);
+    int        ndev = 0;
+    hipError_t e    = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0)
+        return fail(nullptr, CLHIP_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    auto c   = std::make_unique<clhip_ctx>();
+    c->width = width;
+    c->height = height;
+    c->itype  = integrator_type;
+    e         = hipGetDevice(&c->device);
+    if (e != hipSuccess)
+        return fail(nullptr, CLHIP_E_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+    int cus = 0;
+    e       = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    if (e != hipSuccess || cus <= 0)
+        cus = 256;
+    c->num_cus = (uint32_t)cus;
+    c->blocks  = c->num_cus * 8;
+    e          = hipStreamCreate(&c->stream);
+    if (e != hipSuccess)
+        return fail(nullptr, CLHIP_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    c->fr.width  = width;
+    c->fr.height = height;
+    c->fr.npix   = width * height;
+    *out         = c.release();
+    return CLHIP_OK;
+}
+
+int clhip_destroy(clhip_ctx* ctx)
+{
+    if (!ctx)
+        return CLHIP_OK;
+    if (ctx->stream)
+        (void)hipStreamSynchronize(ctx->stream);
+    recycle_events(ctx);
+    for (hipEvent_t e : ctx->event_pool)
+        (void)hipEventDestroy(e);
+    if (ctx->ev_begin)
+        (void)hipEventDestroy(ctx->ev_begin);
+    if (ctx->ev_end)
+        (void)hipEventDestroy(ctx->ev_end);
+    if (ctx->stream)
+        (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CLHIP_OK;
+}
+
+int clhip_create_texture(clhip_ctx* ctx, const float* data, uint32_t width, uint32_t height, uint32_t channels,
+                         int32_t* texture_id)
+{
+    if (!ctx || !data || !texture_id || width == 0 || height == 0 || (channels != 3 && channels != 4))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_create_texture: bad arguments");
+    const size_t       n = (size_t)width * height;
+    std::vector<float> rgba(4 * n);
+    for (size_t i = 0; i < n; ++i)
+    {
+        rgba[4 * i]     = data[channels * i];
+        rgba[4 * i + 1] = data[channels * i + 1];
+        rgba[4 * i + 2] = data[channels * i + 2];
+        rgba[4 * i + 3] = channels == 4 ? data[4 * i + 3] : 1.0f;
+    }
+    TexHost t;
+    t.buf = std::make_unique<DevBuf>();
+    t.w   = width;
+    t.h   = height;
+    CLHIP_CHECK(ctx, upload(*t.buf, rgba));
+    ctx->textures.push_back(std::move(t));
+    *texture_id = (int32_t)ctx->textures.size() - 1;
+    return CLHIP_OK;
+}
+
+int clhip_build_blas(clhip_ctx* ctx, const clhip_mesh* meshes, uint32_t num_meshes)
+{
+    if (!ctx || (!meshes && num_meshes))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_build_blas: bad arguments");
+    std::vector<Node4>    nodes;
+    std::vector<TriRec>   tris;
+    std::vector<MeshInfo> infos(num_meshes);
+    std::vector<MeshRec>  recs(num_meshes);
+    std::vector<float>    pos, nrm, tan, uv;
+    std::vector<uint32_t> idx;
+    bool                  any_n = false, any_t = false, any_uv = false;
+    for (uint32_t m = 0; m < num_meshes; ++m)
+    {
+        any_n |= meshes[m].normals != nullptr;
+        any_t |= meshes[m].tangents != nullptr;
+        any_uv |= meshes[m].uvs != nullptr;
+    }
+    uint64_t total_tris = 0;
+    for (uint32_t m = 0; m < num_meshes; ++m)
+        total_tris += meshes[m].num_triangles;
+    if (total_tris >= CLPT_MAX_TRIS)
+        return fail(ctx, CLHIP_E_UNSUPPORTED, "%llu triangles exceed the 2^27 limit", (unsigned long long)total_tris);
+    for (uint32_t m = 0; m < num_meshes; ++m)
+    {
+        const clhip_mesh& me = meshes[m];
+        if (!me.vertices || (!me.indices && me.num_triangles))
+            return fail(ctx, CLHIP_E_INVALID, "mesh %u: missing vertices or indices", m);
+        const uint32_t nt = me.num_triangles, nv = me.num_vertices;
+        for (size_t i = 0; i < 3 * (size_t)nt; ++i)
+            if (me.indices[i] >= nv)
+                return fail(ctx, CLHIP_E_INVALID, "mesh %u: index %u out of range (%u vertices)", m, me.indices[i], nv);
+        std::vector<float> lo(3 * (size_t)nt), hi(3 * (size_t)nt);
+        for (uint32_t t = 0; t < nt; ++t)
+            for (int k = 0; k < 3; ++k)
+            {
+                float a = me.vertices[3 * (size_t)me.indices[3 * (size_t)t] + k];
+                float b = me.vertices[3 * (size_t)me.indices[3 * (size_t)t + 1] + k];
+                float c = me.vertices[3 * (size_t)me.indices[3 * (size_t)t + 2] + k];
+                lo[3 * (size_t)t + k] = std::min(a, std::min(b, c));
+                hi[3 * (size_t)t + k] = std::max(a, std::max(b, c));
+            }
+        const uint32_t        tri_base = (uint32_t)tris.size();
+        std::vector<uint32_t> order;
+        clpt::Bvh4Result      r = clpt::build_bvh4(lo.data(), hi.data(), nt, CLPT_MAX_LEAF_TRIS, nodes, order,
+                                                   [tri_base](uint32_t first, uint32_t count) {
+                                                       return CLPT_TRI_LEAF(tri_base + first, count);
+                                                   });
+        for (uint32_t i = 0; i < nt; ++i)
+        {
+            const uint32_t p  = order[i];
+            const float*   v0 = me.vertices + 3 * (size_t)me.indices[3 * (size_t)p];
+            const float*   v1 = me.vertices + 3 * (size_t)me.indices[3 * (size_t)p + 1];
+            const float*   v2 = me.vertices + 3 * (size_t)me.indices[3 * (size_t)p + 2];
+            TriRec         tr;
+            uint32_t       pu = p;
+            float          pf;
+            std::memcpy(&pf, &pu, 4);
+            tr.v0p = make_float4(v0[0], v0[1], v0[2], pf);
+            tr.v1  = make_float4(v1[0], v1[1], v1[2], 0.0f);
+            tr.v2  = make_float4(v2[0], v2[1], v2[2], 0.0f);
+            tris.push_back(tr);
+        }
+        MeshInfo& info = infos[m];
+        info.tri_base  = tri_base;
+        info.ntris     = nt;
+        info.nverts    = nv;
+        info.root      = r.root;
+        info.depth     = r.max_depth;
+        std::memcpy(info.lo, r.lo, sizeof(info.lo));
+        std::memcpy(info.hi, r.hi, sizeof(info.hi));
+        MeshRec& rec = recs[m];
+        rec.idx_off  = (uint32_t)(idx.size() / 3);
+        rec.vtx_off  = (uint32_t)(pos.size() / 3);
+        rec.ntris    = nt;
+        rec.flags    = (me.normals ? MESH_HAS_NORMALS : 0u) | (me.tangents ? MESH_HAS_TANGENTS : 0u) |
+            (me.uvs ? MESH_HAS_UVS : 0u);
+        pos.insert(pos.end(), me.vertices, me.vertices + 3 * (size_t)nv);
+        if (any_n)
+        {
+            if (me.normals)
+                nrm.insert(nrm.end(), me.normals, me.normals + 3 * (size_t)nv);
+            else
+                nrm.resize(nrm.size() + 3 * (size_t)nv, 0.0f);
+        }
+        if (any_t)
+        {
+            if (me.tangents)
+                tan.insert(tan.end(), me.tangents, me.tangents + 3 * (size_t)nv);
+            else
+                tan.resize(tan.size() + 3 * (size_t)nv, 0.0f);
+        }
+        if (any_uv)
+        {
+            if (me.uvs)
+                uv.insert(uv.end(), me.uvs, me.uvs + 2 * (size_t)nv);
+            else
+                uv.resize(uv.size() + 2 * (size_t)nv, 0.0f);
+        }
+        idx.insert(idx.end(), me.indices, me.indices + 3 * (size_t)nt);
+    }
+    CLHIP_CHECK(ctx, upload(ctx->d_tris, tris));
+    CLHIP_CHECK(ctx, upload(ctx->d_meshes, recs));
+    CLHIP_CHECK(ctx, upload(ctx->d_pos, pos));
+    CLHIP_CHECK(ctx, upload(ctx->d_nrm, nrm));
+    CLHIP_CHECK(ctx, upload(ctx->d_tan, tan));
+    CLHIP_CHECK(ctx, upload(ctx->d_uv, uv));
+    CLHIP_CHECK(ctx, upload(ctx->d_idx, idx));
+    ctx->blas_nodes = std::move(nodes);
+    ctx->meshes     = std::move(infos);
+    ctx->mesh_recs  = std::move(recs);
+    ctx->have_blas  = true;
+    ctx->have_tlas  = false; /* roots moved: the TLAS must be rebuilt */
+    return CLHIP_OK;
+}
+
+int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uint32_t* instance_ids,
+                     const float* xforms, uint32_t num_instances)
+{
+    if (!ctx || (!mesh_of_instance && num_instances))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_build_tlas: bad arguments");
+    if (!ctx->have_blas)
+        return fail(ctx, CLHIP_E_STATE, "clhip_build_tlas before clhip_build_blas");
+    if (num_instances >= CLPT_MAX_INSTANCES)
+        return fail(ctx, CLHIP_E_UNSUPPORTED, "too many instances");
+    std::vector<InstanceRec> inst(num_instances);
+    std::vector<uint32_t>    ent(num_instances);
+    std::vector<float>       lo(3 * (size_t)num_instances), hi(3 * (size_t)num_instances);
+    uint32_t                 max_blas_depth = 0;
+    for (uint32_t i = 0; i < num_instances; ++i)
+    {
+        const uint32_t m = mesh_of_instance[i];
+        if (m >= ctx->meshes.size())
+            return fail(ctx, CLHIP_E_INVALID, "instance %u: mesh %u of %zu", i, m, ctx->meshes.size());
+        InstanceRec& r = inst[i];
+        static const float id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        std::memcpy(r.xf, xforms ? xforms + 12 * (size_t)i : id, sizeof(r.xf));
+        inverse3x4(r.xf, r.inv);
+        for (int k = 0; k < 12; ++k)
+            if (!std::isfinite(r.inv[k]))
+                return fail(ctx, CLHIP_E_INVALID, "instance %u: singular transform", i);
+        r.blas_root = ctx->meshes[m].root;
+        r.mesh      = m;
+        r.material  = 0;
+        r.entity    = instance_ids ? instance_ids[i] : i;
+        ent[i]      = r.entity;
+        max_blas_depth = std::max(max_blas_depth, ctx->meshes[m].depth);
+        const MeshInfo& mi = ctx->meshes[m];
+        for (int k = 0; k < 3; ++k)
+        {
+            lo[3 * (size_t)i + k] = INFINITY;
+            hi[3 * (size_t)i + k] = -INFINITY;
+        }
+        if (mi.ntris == 0)
+        {
+            /* an empty mesh gets an empty box far from everything */
+            for (int k = 0; k < 3; ++k)
+                lo[3 * (size_t)i + k] = hi[3 * (size_t)i + k] = 0.0f;
+            continue;
+        }
+        for (int c = 0; c < 8; ++c)
+        {
+            const double p[3] = {(c & 1) ? mi.hi[0] : mi.lo[0], (c & 2) ? mi.hi[1] : mi.lo[1],
+                                 (c & 4) ? mi.hi[2] : mi.lo[2]};
+            for (int k = 0; k < 3; ++k)
+            {
+                const double q = (double)r.xf[4 * k] * p[0] + (double)r.xf[4 * k + 1] * p[1] +
+                    (double)r.xf[4 * k + 2] * p[2] + (double)r.xf[4 * k + 3];
+                lo[3 * (size_t)i + k] = std::min(lo[3 * (size_t)i + k], std::nextafter((float)q, -INFINITY));
+                hi[3 * (size_t)i + k] = std::max(hi[3 * (size_t)i + k], std::nextafter((float)q, INFINITY));
+            }
+        }
+    }
+    std::vector<Node4>    nodes = ctx->blas_nodes;
+    std::vector<uint32_t> order;
+    clpt::Bvh4Result      r = clpt::build_bvh4(lo.data(), hi.data(), num_instances, 1u, nodes, order,
+                                               [&order](uint32_t first, uint32_t) { return CLPT_INST_LEAF(order[first]); });
+    /* stack bound: each 4-wide level leaves at most 3 entries behind, plus
+     * the instance marker and the popped entry */
+    const uint32_t need = 3u * (r.max_depth + max_blas_depth) + 4u;
+    if (need > 128u)
+        return fail(ctx, CLHIP_E_UNSUPPORTED, "tree too deep for the traversal stack (TLAS %u + BLAS %u levels)",
+                    r.max_depth, max_blas_depth);
+    CLHIP_CHECK(ctx, upload(ctx->d_nodes, nodes));
+    ctx->inst        = std::move(inst);
+    ctx->inst_entity = std::move(ent);
+    ctx->tlas_root   = r.root;
+    ctx->tlas_depth  = r.max_depth;
+    ctx->have_tlas   = true;
+    ctx->inst_dirty  = true;
+    return CLHIP_OK;
+}
+
+int clhip_build_materials(clhip_ctx* ctx, const clhip_material* materials, uint32_t n)
+{
+    if (!ctx || (!materials && n))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_build_materials: bad arguments");
+    std::vector<MaterialRec> recs(n);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        const clhip_material& m = materials[i];
+        if (m.type > CLHIP_MTL_GLASS)
+            return fail(ctx, CLHIP_E_INVALID, "material %u: unknown type %u", i, m.type);
+        if (m.type == CLHIP_MTL_GLASS && m.interior_ior == 1.0f)
+            return fail(ctx, CLHIP_E_INVALID, "material %u: glass IOR must differ from 1 (roughdielectric.h:28)", i);
+        MaterialRec& r = recs[i];
+        std::memset(&r, 0, sizeof(r));
+        r.type = m.type;
+        for (int k = 0; k < 3; ++k)
+        {
+            r.r[k]   = m.reflectance[k];
+            r.eta[k] = m.eta[k];
+            r.k[k]   = m.k[k];
+        }
+        const float rough = m.roughness;
+        r.alpha           = std::fmax(rough * rough, 1e-4f); /* metal.h:131, microfacet.h:27 */
+        r.ior             = m.interior_ior;
+    }
+    CLHIP_CHECK(ctx, upload(ctx->d_mtl, recs));
+    ctx->mtl        = std::move(recs);
+    ctx->inst_dirty = true;
+    return CLHIP_OK;
+}
+
+int clhip_build_entities(clhip_ctx* ctx, const uint32_t* material_index, uint32_t n)
+{
+    if (!ctx || (!material_index && n))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_build_entities: bad arguments");
+    ctx->entity_material.assign(material_index, material_index + n);
+    ctx->inst_dirty = true;
+    return CLHIP_OK;
+}
+
+int clhip_build_emitters(clhip_ctx* ctx, const clhip_emitter* emitters, uint32_t n, int32_t dome_index)
+{
+    if (!ctx || (!emitters && n))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_build_emitters: bad arguments");
+    if (dome_index >= (int32_t)n)
+        return fail(ctx, CLHIP_E_INVALID, "dome index %d out of %u emitters", dome_index, n);
+    std::vector<EmitterRec> recs(n);
+    int32_t                 dome_tex = -1;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        const clhip_emitter& e = emitters[i];
+        EmitterRec&          r = recs[i];
+        std::memset(&r, 0, sizeof(r));
+        r.type = e.type;
+        for (int k = 0; k < 3; ++k)
+        {
+            r.intensity[k] = e.intensity[k];
+            r.dir[k]       = e.direction[k];
+        }
+        if (e.type == CLHIP_EMITTER_DIRECTIONAL)
+        {
+            r.tan_r = (float)std::tan((double)e.angular_radius);
+            if (!(r.tan_r > 0.0f))
+                return fail(ctx, CLHIP_E_INVALID, "emitter %u: angular radius must be in (0, pi/2)", i);
+        }
+        else if (e.type == CLHIP_EMITTER_HDRIDOME)
+        {
+            if (e.texture < 0 || e.texture >= (int32_t)ctx->textures.size())
+                return fail(ctx, CLHIP_E_INVALID, "emitter %u: bad texture id %d", i, e.texture);
+            if ((int32_t)i != dome_index)
+                return fail(ctx, CLHIP_E_INVALID, "emitter %u is a dome but dome_index is %d", i, dome_index);
+            dome_tex = e.texture;
+        }
+        else
+            return fail(ctx, CLHIP_E_INVALID, "emitter %u: unknown type %u", i, e.type);
+    }
+    if (dome_index >= 0 && dome_tex < 0)
+        return fail(ctx, CLHIP_E_INVALID, "dome_index %d is not an HDRI dome", dome_index);
+    CLHIP_CHECK(ctx, upload(ctx->d_emitters, recs));
+    ctx->emitters   = std::move(recs);
+    ctx->dome_index = dome_index;
+    ctx->dome_tex   = dome_tex;
+    if (dome_tex >= 0)
+    {
+        const TexHost& t = ctx->textures[dome_tex];
+        ctx->dome_w      = t.w;
+        ctx->dome_h      = t.h;
+        const size_t W   = (size_t)t.w + 1;
+        CLHIP_CHECK(ctx, ctx->d_pcond.alloc(W * t.h * sizeof(float)));
+        CLHIP_CHECK(ctx, ctx->d_cdfcond.alloc(W * t.h * sizeof(float)));
+        CLHIP_CHECK(ctx, ctx->d_pv.alloc(((size_t)t.h + 1) * sizeof(float)));
+        CLHIP_CHECK(ctx, ctx->d_cdfv.alloc(((size_t)t.h + 1) * sizeof(float)));
+        CLHIP_CHECK(ctx, ctx->d_rowint.alloc((size_t)t.h * sizeof(double)));
+        CLHIP_CHECK(ctx, clpt::launch_dome_tables(ctx->stream, t.buf->as<const float4>(), t.w, t.h,
+                                                  ctx->d_pcond.as<float>(), ctx->d_cdfcond.as<float>(),
+                                                  ctx->d_pv.as<float>(), ctx->d_cdfv.as<float>(),
+                                                  ctx->d_rowint.as<double>()));
+        CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return CLHIP_OK;
+}
+
+int clhip_set_camera(clhip_ctx* ctx, const float pos[3], const float d00[3], const float ddu[3], const float ddv[3])
+{
+    if (!ctx || !pos || !d00 || !ddu || !ddv)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_set_camera: bad arguments");
+    for (int k = 0; k < 3; ++k)
+    {
+        ctx->fr.pos[k] = pos[k];
+        ctx->fr.d00[k] = d00[k];
+        ctx->fr.ddu[k] = ddu[k];
+        ctx->fr.ddv[k] = ddv[k];
+    }
+    ctx->have_camera = true;
+    return clhip_reset(ctx);
+}
+
+int clhip_set_params(clhip_ctx* ctx, const clhip_params* p)
+{
+    if (!ctx || !p)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_set_params: bad arguments");
+    if (p->max_depth < 1 || p->max_depth > CLHIP_MAX_DEPTH)
+        return fail(ctx, CLHIP_E_INVALID, "max_depth %u outside [1, %d]", p->max_depth, CLHIP_MAX_DEPTH);
+    if (p->view >= (1u << 12))
+        return fail(ctx, CLHIP_E_INVALID, "view %u outside [0, 4096)", p->view);
+    ctx->params = *p;
+    return clhip_reset(ctx);
+}
+
+int clhip_reset(clhip_ctx* ctx)
+{
+    if (!ctx)
+        return fail(nullptr, CLHIP_E_INVALID, "ctx is NULL");
+    ctx->spp = 0;
+    if (ctx->d_accum.p)
+        CLHIP_CHECK(ctx, hipMemsetAsync(ctx->d_accum.p, 0, ctx->d_accum.bytes, ctx->stream));
+    return CLHIP_OK;
+}
+
+int clhip_resize(clhip_ctx* ctx, uint32_t width, uint32_t height)
+{
+    if (!ctx || width == 0 || height == 0 || (uint64_t)width * height > (1ull << 30))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_resize: bad size");
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->width     = width;
+    ctx->height    = height;
+    ctx->fr.width  = width;
+    ctx->fr.height = height;
+    ctx->fr.npix   = width * height;
+    ctx->d_accum.release();
+    ctx->d_out.release();
+    ctx->spp = 0;
+    return CLHIP_OK;
+}
+
+int clhip_get_film_size(const clhip_ctx* ctx, uint32_t* width, uint32_t* height, uint32_t* spp)
+{
+    if (!ctx)
+        return fail(nullptr, CLHIP_E_INVALID, "ctx is NULL");
+    if (width)
+        *width = ctx->width;
+    if (height)
+        *height = ctx->height;
+    if (spp)
+        *spp = ctx->spp;
+    return CLHIP_OK;
+}
+
+int clhip_set_timing(clhip_ctx* ctx, int timing)
+{
+    if (!ctx)
+        return fail(nullptr, CLHIP_E_INVALID, "ctx is NULL");
+    ctx->timing = timing != 0;
+    return CLHIP_OK;
+}
+
+int clhip_render(clhip_ctx* ctx, uint32_t spp)
+{
+    if (!ctx)
+        return fail(nullptr, CLHIP_E_INVALID, "ctx is NULL");
+    if (ctx->itype != CLHIP_INTEGRATOR_PATH_TRACING)
+        return fail(ctx, CLHIP_E_UNSUPPORTED, "integrator type %u is not available yet", ctx->itype);
+    if (!ctx->have_camera)
+        return fail(ctx, CLHIP_E_STATE, "no camera: call clhip_set_camera first");
+    int rc = sync_instances(ctx);
+    if (rc)
+        return rc;
+    if (spp == 0)
+        return CLHIP_OK;
+    const uint64_t npix = (uint64_t)ctx->width * ctx->height;
+    uint32_t       spb  = ctx->params.samples_per_batch;
+    if (spb == 0)
+        spb = (uint32_t)std::max<uint64_t>(1, (2u << 20) / npix);
+    spb = std::min(spb, spp);
+    while ((uint64_t)spb * npix > 0xffffffffull / 2)
+        spb = std::max(1u, spb / 2);
+    const bool fresh = ctx->d_accum.p == nullptr;
+    rc               = ensure_render_buffers(ctx, (size_t)spb * npix);
+    if (rc)
+        return rc;
+    if (fresh || ctx->spp == 0)
+        CLHIP_CHECK(ctx, hipMemsetAsync(ctx->d_accum.p, 0, ctx->d_accum.bytes, ctx->stream));
+    CLHIP_CHECK(ctx, hipMemsetAsync(ctx->d_tot.p, 0, sizeof(TotalCounters), ctx->stream));
+    CLHIP_CHECK(ctx, hipMemsetAsync(ctx->d_ctr.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    recycle_events(ctx);
+    ctx->trace_launches = ctx->shadow_launches = 0;
+    if (!ctx->ev_begin)
+    {
+        CLHIP_CHECK(ctx, hipEventCreate(&ctx->ev_begin));
+        CLHIP_CHECK(ctx, hipEventCreate(&ctx->ev_end));
+    }
+    CLHIP_CHECK(ctx, hipEventRecord(ctx->ev_begin, ctx->stream));
+
+    FrameView fr  = ctx->fr;
+    fr.max_depth  = ctx->params.max_depth;
+    fr.rr_depth   = ctx->params.rr_depth;
+    fr.view       = ctx->params.view;
+    const SceneView sc = scene_view(ctx);
+    RayQueue        q[2];
+    BatchCounters*  bc = ctx->d_bc.as<BatchCounters>();
+    for (int k = 0; k < 2; ++k)
+    {
+        q[k].o = ctx->d_q[k][0].as<float4>();
+        q[k].d = ctx->d_q[k][1].as<float4>();
+        q[k].T = ctx->d_q[k][2].as<float4>();
+    }
+    HitBuf      hb{ctx->d_hit.as<float4>(), ctx->d_prim.as<uint32_t>()};
+    ShadowQueue sq{ctx->d_sq[0].as<float4>(), ctx->d_sq[1].as<float4>(), ctx->d_sq[2].as<float2>(), nullptr};
+    double*     accum = ctx->d_accum.as<double>();
+    auto*       ctr   = ctx->d_ctr.as<unsigned long long>();
+    uint32_t*   gst   = ctx->d_gstack.as<uint32_t>();
+    const uint32_t blocks = ctx->blocks;
+
+    for (uint32_t done = 0; done < spp; done += spb)
+    {
+        const uint32_t nb          = std::min(spb, spp - done);
+        const uint32_t npaths      = (uint32_t)(nb * npix);
+        const uint32_t sample_base = ctx->spp + done;
+        CLHIP_CHECK(ctx, clpt::launch_camera(ctx->stream, fr, sample_base, npaths, q[0], bc, blocks));
+        for (uint32_t depth = 1; depth <= fr.max_depth; ++depth)
+        {
+            const RayQueue& qin  = q[(depth - 1) & 1];
+            const RayQueue& qout = q[depth & 1];
+            TimedLaunch     tl{};
+            if (ctx->timing)
+            {
+                tl.a = pool_event(ctx);
+                tl.b = pool_event(ctx);
+                tl.kind = 0;
+                CLHIP_CHECK(ctx, hipEventRecord(tl.a, ctx->stream));
+            }
+            CLHIP_CHECK(ctx, clpt::launch_trace(ctx->stream, sc, qin, &bc->rays[depth], depth, hb, gst, ctr, blocks));
+            ++ctx->trace_launches;
+            if (ctx->timing)
+            {
+                CLHIP_CHECK(ctx, hipEventRecord(tl.b, ctx->stream));
+                ctx->timed.push_back(tl);
+                tl.a    = pool_event(ctx);
+                tl.b    = pool_event(ctx);
+                tl.kind = 1;
+                CLHIP_CHECK(ctx, hipEventRecord(tl.a, ctx->stream));
+            }
+            CLHIP_CHECK(ctx, clpt::launch_shade(ctx->stream, sc, fr, depth, sample_base, qin, hb, qout, sq, accum, bc, blocks));
+            if (ctx->timing)
+            {
+                CLHIP_CHECK(ctx, hipEventRecord(tl.b, ctx->stream));
+                ctx->timed.push_back(tl);
+            }
+            if (depth < fr.max_depth)
+            {
+                if (ctx->timing)
+                {
+                    tl.a    = pool_event(ctx);
+                    tl.b    = pool_event(ctx);
+                    tl.kind = 2;
+                    CLHIP_CHECK(ctx, hipEventRecord(tl.a, ctx->stream));
+                }
+                CLHIP_CHECK(ctx, clpt::launch_shadow(ctx->stream, sc, sq, depth, accum, bc, gst, ctr, blocks));
+                ++ctx->shadow_launches;
+                if (ctx->timing)
+                {
+                    CLHIP_CHECK(ctx, hipEventRecord(tl.b, ctx->stream));
+                    ctx->timed.push_back(tl);
+                }
+            }
+        }
+        CLHIP_CHECK(ctx, clpt::launch_batch_end(ctx->stream, bc, ctx->d_tot.as<TotalCounters>()));
+    }
+    CLHIP_CHECK(ctx, hipEventRecord(ctx->ev_end, ctx->stream));
+    ctx->have_render_events = true;
+    ctx->spp += spp;
+    return CLHIP_OK;
+}
+
+int clhip_sync(clhip_ctx* ctx)
+{
+    if (!ctx)
+        return fail(nullptr, CLHIP_E_INVALID, "ctx is NULL");
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return CLHIP_OK;
+}
+
+int clhip_readback(clhip_ctx* ctx, float* rgba, size_t bytes)
+{
+    if (!ctx || !rgba)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_readback: bad arguments");
+    const size_t npix = (size_t)ctx->width * ctx->height;
+    if (bytes < npix * 4 * sizeof(float))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_readback: buffer of %zu bytes, need %zu", bytes, npix * 16);
+    if (!ctx->d_accum.p)
+    {
+        std::memset(rgba, 0, npix * 16);
+        for (size_t i = 0; i < npix; ++i)
+            rgba[4 * i + 3] = 1.0f;
+        return CLHIP_OK;
+    }
+    CLHIP_CHECK(ctx, clpt::launch_resolve(ctx->stream, ctx->d_accum.as<double>(), (uint32_t)npix, ctx->spp,
+                                          ctx->d_out.as<float4>()));
+    CLHIP_CHECK(ctx, hipMemcpyAsync(rgba, ctx->d_out.p, npix * 16, hipMemcpyDeviceToHost, ctx->stream));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return CLHIP_OK;
+}
+
+int clhip_readback_sums(clhip_ctx* ctx, double* rgb, size_t bytes)
+{
+    if (!ctx || !rgb)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_readback_sums: bad arguments");
+    const size_t npix = (size_t)ctx->width * ctx->height;
+    if (bytes < npix * 3 * sizeof(double))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_readback_sums: buffer too small");
+    if (!ctx->d_accum.p)
+    {
+        std::memset(rgb, 0, npix * 3 * sizeof(double));
+        return CLHIP_OK;
+    }
+    CLHIP_CHECK(ctx, hipMemcpyAsync(rgb, ctx->d_accum.p, npix * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return CLHIP_OK;
+}
+
+int clhip_counters(clhip_ctx* ctx, clhip_stats* out)
+{
+    if (!ctx || !out)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_counters: bad arguments");
+    std::memset(out, 0, sizeof(*out));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->d_tot.p)
+    {
+        TotalCounters      t;
+        unsigned long long c[4];
+        CLHIP_CHECK(ctx, hipMemcpy(&t, ctx->d_tot.p, sizeof(t), hipMemcpyDeviceToHost));
+        CLHIP_CHECK(ctx, hipMemcpy(c, ctx->d_ctr.p, sizeof(c), hipMemcpyDeviceToHost));
+        for (int b = 0; b <= CLHIP_MAX_DEPTH && b < CLPT_MAXD + 2; ++b)
+        {
+            out->rays[b]    = t.rays[b];
+            out->hits[b]    = t.hits[b];
+            out->shadow[b]  = t.shadow[b];
+            out->visible[b] = t.visible[b];
+        }
+        out->trace_nodes  = c[0];
+        out->trace_tris   = c[1];
+        out->shadow_nodes = c[2];
+        out->shadow_tris  = c[3];
+    }
+    out->trace_launches  = ctx->trace_launches;
+    out->shadow_launches = ctx->shadow_launches;
+    for (auto& tl : ctx->timed)
+    {
+        float ms = 0.0f;
+        CLHIP_CHECK(ctx, hipEventElapsedTime(&ms, tl.a, tl.b));
+        if (tl.kind == 0)
+            out->trace_ms += ms;
+        else if (tl.kind == 1)
+            out->shade_ms += ms;
+        else
+            out->shadow_ms += ms;
+    }
+    if (ctx->have_render_events)
+    {
+        float ms = 0.0f;
+        CLHIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
+        out->render_ms = ms;
+    }
+    return CLHIP_OK;
+}
+
+/* ------------------------------------------------------------ batch APIs */
+
+int clhip_rng_batch(clhip_ctx* ctx, const uint32_t* seeds4, uint32_t n, uint32_t ndraws, uint32_t* out_u32, float* out_f)
+{
+    if (!ctx || !seeds4 || !out_u32 || !out_f)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_rng_batch: bad arguments");
+    Tmp s, ou, of;
+    CLHIP_CHECK(ctx, s.from_host(seeds4, 4 * (size_t)n));
+    CLHIP_CHECK(ctx, ou.zeros((size_t)n * ndraws * 4));
+    CLHIP_CHECK(ctx, of.zeros((size_t)n * ndraws * 4));
+    CLHIP_CHECK(ctx, clpt::launch_rng_batch(ctx->stream, s.b.as<const uint4>(), n, ndraws, ou.b.as<uint32_t>(), of.b.as<float>()));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, ou.to_host(out_u32, (size_t)n * ndraws));
+    CLHIP_CHECK(ctx, of.to_host(out_f, (size_t)n * ndraws));
+    return CLHIP_OK;
+}
+
+int clhip_camera_rays(clhip_ctx* ctx, const uint32_t* pixel_sample2, uint32_t n, float* out8)
+{
+    if (!ctx || !pixel_sample2 || !out8)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_camera_rays: bad arguments");
+    if (!ctx->have_camera)
+        return fail(ctx, CLHIP_E_STATE, "no camera");
+    for (uint32_t i = 0; i < n; ++i)
+        if (pixel_sample2[2 * i] >= ctx->fr.npix)
+            return fail(ctx, CLHIP_E_INVALID, "pixel %u out of range", pixel_sample2[2 * i]);
+    FrameView fr = ctx->fr;
+    fr.view      = ctx->params.view;
+    Tmp ps, out;
+    CLHIP_CHECK(ctx, ps.from_host(pixel_sample2, 2 * (size_t)n));
+    CLHIP_CHECK(ctx, out.zeros((size_t)n * 32));
+    CLHIP_CHECK(ctx, clpt::launch_camera_batch(ctx->stream, fr, ps.b.as<const uint2>(), n, out.b.as<float4>()));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, out.to_host(out8, 8 * (size_t)n));
+    return CLHIP_OK;
+}
+
+int clhip_bsdf_batch(clhip_ctx* ctx, const uint32_t* material, const float* wi3, const float* wo3, const float* u3,
+                     uint32_t n, float* eval_f3, float* eval_pdf, float* smp_wi3, float* smp_f3, float* smp_pdf)
+{
+    if (!ctx || !material || !wi3 || !wo3 || !u3 || !eval_f3 || !eval_pdf || !smp_wi3 || !smp_f3 || !smp_pdf)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_bsdf_batch: bad arguments");
+    for (uint32_t i = 0; i < n; ++i)
+        if (material[i] >= ctx->mtl.size())
+            return fail(ctx, CLHIP_E_INVALID, "item %u: material %u of %zu", i, material[i], ctx->mtl.size());
+    const SceneView sc = scene_view(ctx);
+    Tmp             m, a, b, u, ef, ep, sw, sf, sp;
+    CLHIP_CHECK(ctx, m.from_host(material, n));
+    CLHIP_CHECK(ctx, a.from_host(wi3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, b.from_host(wo3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, u.from_host(u3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, ef.zeros(12 * (size_t)n));
+    CLHIP_CHECK(ctx, ep.zeros(4 * (size_t)n));
+    CLHIP_CHECK(ctx, sw.zeros(12 * (size_t)n));
+    CLHIP_CHECK(ctx, sf.zeros(12 * (size_t)n));
+    CLHIP_CHECK(ctx, sp.zeros(4 * (size_t)n));
+    CLHIP_CHECK(ctx, clpt::launch_bsdf_batch(ctx->stream, sc, m.b.as<const uint32_t>(), a.b.as<const float>(),
+                                             b.b.as<const float>(), u.b.as<const float>(), n, ef.b.as<float>(),
+                                             ep.b.as<float>(), sw.b.as<float>(), sf.b.as<float>(), sp.b.as<float>()));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, ef.to_host(eval_f3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, ep.to_host(eval_pdf, n));
+    CLHIP_CHECK(ctx, sw.to_host(smp_wi3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, sf.to_host(smp_f3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, sp.to_host(smp_pdf, n));
+    return CLHIP_OK;
+}
+
+int clhip_emitter_batch(clhip_ctx* ctx, const float* u2, uint32_t n, float* dir3, float* value3, float* pdf, uint32_t* which)
+{
+    if (!ctx || !u2 || !dir3 || !value3 || !pdf || !which)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_emitter_batch: bad arguments");
+    const SceneView sc = scene_view(ctx);
+    Tmp             u, d, v, p, w;
+    CLHIP_CHECK(ctx, u.from_host(u2, 2 * (size_t)n));
+    CLHIP_CHECK(ctx, d.zeros(12 * (size_t)n));
+    CLHIP_CHECK(ctx, v.zeros(12 * (size_t)n));
+    CLHIP_CHECK(ctx, p.zeros(4 * (size_t)n));
+    CLHIP_CHECK(ctx, w.zeros(4 * (size_t)n));
+    CLHIP_CHECK(ctx, clpt::launch_emitter_batch(ctx->stream, sc, u.b.as<const float>(), n, d.b.as<float>(),
+                                                v.b.as<float>(), p.b.as<float>(), w.b.as<uint32_t>()));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, d.to_host(dir3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, v.to_host(value3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, p.to_host(pdf, n));
+    CLHIP_CHECK(ctx, w.to_host(which, n));
+    return CLHIP_OK;
+}
+
+int clhip_dome_eval_batch(clhip_ctx* ctx, const float* dir3, uint32_t n, float* le3, float* pdf)
+{
+    if (!ctx || !dir3 || !le3 || !pdf)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_dome_eval_batch: bad arguments");
+    if (ctx->dome_tex < 0)
+        return fail(ctx, CLHIP_E_STATE, "no HDRI dome");
+    const SceneView sc = scene_view(ctx);
+    Tmp             d, l, p;
+    CLHIP_CHECK(ctx, d.from_host(dir3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, l.zeros(12 * (size_t)n));
+    CLHIP_CHECK(ctx, p.zeros(4 * (size_t)n));
+    CLHIP_CHECK(ctx, clpt::launch_dome_eval_batch(ctx->stream, sc, d.b.as<const float>(), n, l.b.as<float>(), p.b.as<float>()));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, l.to_host(le3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, p.to_host(pdf, n));
+    return CLHIP_OK;
+}
+
+int clhip_dome_tables(clhip_ctx* ctx, float* pcond, float* cdfcond, float* pv, float* cdfv, uint32_t* w, uint32_t* h)
+{
+    if (!ctx)
+        return fail(nullptr, CLHIP_E_INVALID, "ctx is NULL");
+    if (ctx->dome_tex < 0)
+        return fail(ctx, CLHIP_E_STATE, "no HDRI dome");
+    if (w)
+        *w = ctx->dome_w;
+    if (h)
+        *h = ctx->dome_h;
+    const size_t W = (size_t)ctx->dome_w + 1;
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (pcond)
+        CLHIP_CHECK(ctx, hipMemcpy(pcond, ctx->d_pcond.p, W * ctx->dome_h * 4, hipMemcpyDeviceToHost));
+    if (cdfcond)
+        CLHIP_CHECK(ctx, hipMemcpy(cdfcond, ctx->d_cdfcond.p, W * ctx->dome_h * 4, hipMemcpyDeviceToHost));
+    if (pv)
+        CLHIP_CHECK(ctx, hipMemcpy(pv, ctx->d_pv.p, ((size_t)ctx->dome_h + 1) * 4, hipMemcpyDeviceToHost));
+    if (cdfv)
+        CLHIP_CHECK(ctx, hipMemcpy(cdfv, ctx->d_cdfv.p, ((size_t)ctx->dome_h + 1) * 4, hipMemcpyDeviceToHost));
+    return CLHIP_OK;
+}
+
+static int trace_common(clhip_ctx* ctx, const float* rays8, uint32_t n, int any, std::vector<float4>& out,
+                        std::vector<uint32_t>& prim)
+{
+    int rc = sync_instances(ctx);
+    if (rc)
+        return rc;
+    const SceneView sc = scene_view(ctx);
+    const uint32_t  blocks = std::min<uint32_t>(ctx->blocks, std::max<uint32_t>(1, (n + 255) / 256));
+    Tmp             r, o, p, g;
+    CLHIP_CHECK(ctx, r.from_host(rays8, 8 * (size_t)n));
+    CLHIP_CHECK(ctx, o.zeros(16 * (size_t)n));
+    CLHIP_CHECK(ctx, p.zeros(4 * (size_t)n));
+    CLHIP_CHECK(ctx, g.zeros((size_t)blocks * 256 * 112 * 4));
+    CLHIP_CHECK(ctx, clpt::launch_trace_batch(ctx->stream, sc, r.b.as<const float4>(), n, any, o.b.as<float4>(),
+                                              p.b.as<uint32_t>(), g.b.as<uint32_t>(), blocks));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    out.resize(n);
+    prim.resize(n);
+    CLHIP_CHECK(ctx, o.to_host(out.data(), n));
+    CLHIP_CHECK(ctx, p.to_host(prim.data(), n));
+    return CLHIP_OK;
+}
+
+int clhip_trace_batch(clhip_ctx* ctx, const float* rays8, uint32_t n, float* t, float* b12, uint32_t* inst, uint32_t* prim)
+{
+    if (!ctx || !rays8 || !t || !b12 || !inst || !prim)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_trace_batch: bad arguments");
+    std::vector<float4>   out;
+    std::vector<uint32_t> pr;
+    int                   rc = trace_common(ctx, rays8, n, 0, out, pr);
+    if (rc)
+        return rc;
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        t[i]           = out[i].x;
+        b12[2 * i]     = out[i].y;
+        b12[2 * i + 1] = out[i].z;
+        uint32_t iu;
+        std::memcpy(&iu, &out[i].w, 4);
+        inst[i] = iu;
+        prim[i] = pr[i];
+    }
+    return CLHIP_OK;
+}
+
+int clhip_occlusion_batch(clhip_ctx* ctx, const float* rays8, uint32_t n, uint8_t* occluded)
+{
+    if (!ctx || !rays8 || !occluded)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_occlusion_batch: bad arguments");
+    std::vector<float4>   out;
+    std::vector<uint32_t> pr;
+    int                   rc = trace_common(ctx, rays8, n, 1, out, pr);
+    if (rc)
+        return rc;
+    for (uint32_t i = 0; i < n; ++i)
+        occluded[i] = out[i].x != 0.0f ? 1u : 0u;
+    return CLHIP_OK;
+}
+
+int clhip_instance_inverses(clhip_ctx* ctx, float* out12, uint32_t n)
+{
+    if (!ctx || !out12)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_instance_inverses: bad arguments");
+    if (n > ctx->inst.size())
+        return fail(ctx, CLHIP_E_INVALID, "asked for %u of %zu instances", n, ctx->inst.size());
+    for (uint32_t i = 0; i < n; ++i)
+        std::memcpy(out12 + 12 * (size_t)i, ctx->inst[i].inv, 12 * sizeof(float));
+    return CLHIP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,169 @@
+/*
+ * clpt_scene.h -- device-resident scene and wavefront layout (host/device
+ * PODs).  Everything lives in HBM as flat arrays addressed by 32-bit
+ * indices; all per-ray state is structure-of-arrays in float4 planes so a
+ * wave64 reads and writes whole 1 KiB lines.
+ */
+#ifndef CLPT_SCENE_H
+#define CLPT_SCENE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+/* ---- BVH4 node: 128 bytes, one L2 line pair, loaded as 8 dwordx4.
+ * Child boxes are stored per axis (SoA inside the node). ---- */
+struct alignas(128) Node4
+{
+    float    lox[4], hix[4];
+    float    loy[4], hiy[4];
+    float    loz[4], hiz[4];
+    uint32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 is two 64-byte halves");
+
+/* Child reference encoding. */
+#define CLPT_CHILD_EMPTY 0xffffffffu
+#define CLPT_STACK_MARKER 0xfffffffeu /* pops back to the world-space ray */
+#define CLPT_LEAF_BIT 0x80000000u
+#define CLPT_INST_BIT 0x40000000u
+/* triangle leaf: LEAF | (count-1) << 27 | first triangle (27 bits) */
+#define CLPT_TRI_LEAF(first, count) (CLPT_LEAF_BIT | ((uint32_t)((count)-1) << 27) | (uint32_t)(first))
+#define CLPT_INST_LEAF(idx) (CLPT_LEAF_BIT | CLPT_INST_BIT | (uint32_t)(idx))
+#define CLPT_MAX_LEAF_TRIS 4u
+#define CLPT_MAX_TRIS (1u << 27)
+#define CLPT_MAX_INSTANCES 0x3ffffff0u
+
+/* Triangle in leaf order: v0.xyz + original primitive index (bits), v1, v2. */
+struct TriRec
+{
+    float4 v0p;
+    float4 v1;
+    float4 v2;
+};
+
+struct InstanceRec
+{
+    float    xf[12];  /* object -> world */
+    float    inv[12]; /* world -> object */
+    uint32_t blas_root;
+    uint32_t mesh;
+    uint32_t material;
+    uint32_t entity;
+};
+
+enum : uint32_t
+{
+    MESH_HAS_NORMALS  = 1u,
+    MESH_HAS_TANGENTS = 2u,
+    MESH_HAS_UVS      = 4u
+};
+
+struct MeshRec
+{
+    uint32_t idx_off; /* first index (x3) in the global index array */
+    uint32_t vtx_off; /* first vertex in the global vertex arrays */
+    uint32_t ntris;
+    uint32_t flags;
+};
+
+/* Material: type, reflectance, GGX alpha = max(roughness^2, 1e-4), eta, k, ior. */
+struct alignas(16) MaterialRec
+{
+    uint32_t type;
+    float    r[3];
+    float    alpha;
+    float    eta[3];
+    float    k[3];
+    float    ior;
+    float    pad[2];
+};
+
+struct alignas(16) EmitterRec
+{
+    uint32_t type;
+    float    intensity[3];
+    float    dir[3];
+    float    tan_r; /* tan(angular radius) rounded once on the host */
+};
+
+/* Everything a kernel needs to find the scene; passed by value. */
+struct SceneView
+{
+    const Node4*       nodes;
+    const TriRec*      tris;
+    const InstanceRec* inst;
+    const MeshRec*     meshes;
+    const float*       pos; /* xyz per vertex */
+    const float*       nrm;
+    const float*       tan;
+    const float*       uv;
+    const uint32_t*    idx;
+    const MaterialRec* mtl;
+    const EmitterRec*  emitters;
+    uint32_t           nemitters;
+    int32_t            dome_index;
+    const float4*      dome_tex;
+    uint32_t           dome_w, dome_h;
+    const float*       pcond;
+    const float*       cdfcond;
+    const float*       pv;
+    const float*       cdfv;
+    uint32_t           tlas_root;
+    uint32_t           ninst;
+};
+
+/* Camera and frame parameters of one render. */
+struct FrameView
+{
+    uint32_t width, height;
+    float    pos[3], d00[3], ddu[3], ddv[3];
+    uint32_t max_depth, rr_depth, view;
+    uint32_t npix;
+};
+
+/* A ray queue: 3 float4 planes + count (device). */
+struct RayQueue
+{
+    float4*   o;  /* origin.xyz, previous BSDF pdf */
+    float4*   d;  /* direction.xyz, path id (bits) */
+    float4*   T;  /* throughput.xyz, unused */
+    uint32_t* count;
+};
+
+struct HitBuf
+{
+    float4*   h;    /* t, b1, b2, instance (bits) */
+    uint32_t* prim; /* original primitive index */
+};
+
+struct ShadowQueue
+{
+    float4*   o; /* origin.xyz, Lo.r */
+    float4*   d; /* direction.xyz, Lo.g */
+    float2*   x; /* Lo.b, pixel (bits) */
+    uint32_t* count;
+};
+
+/* Per-batch exact counters (device): queue counts per depth and traversal
+ * work.  Layout of the u32 array: [0..MAXD] ray counts (set by the producer
+ * of each queue), [MAXD+1 ..] hits, shadows, visible. */
+#define CLPT_MAXD 32
+struct BatchCounters
+{
+    uint32_t rays[CLPT_MAXD + 2];
+    uint32_t hits[CLPT_MAXD + 2];
+    uint32_t shadow[CLPT_MAXD + 2];
+    uint32_t visible[CLPT_MAXD + 2];
+};
+
+struct TotalCounters
+{
+    unsigned long long rays[CLPT_MAXD + 2];
+    unsigned long long hits[CLPT_MAXD + 2];
+    unsigned long long shadow[CLPT_MAXD + 2];
+    unsigned long long visible[CLPT_MAXD + 2];
+    unsigned long long trace_nodes, trace_tris, shadow_nodes, shadow_tris;
+};
+
+#endif

@@ -1,0 +1,909 @@
+/*
+ * kernels.hip -- gfx950 wavefront path-tracing kernels.
+ *
+ * Pipeline of one batch of paths (replaces the 30 synchronous launches per
+ * spp of WavefrontPathTracingIntegrator::render, path.cpp:84-231):
+ *   k_camera                         generateCameraRays (raygeneration.cu:59-103)
+ *   for depth = 1..max_depth:
+ *     k_trace   closest hit          OptiX primaryRay raygen + closest-hit/miss (optix.cu:38-75, 112-262)
+ *     k_shade   escape + NEE + BSDF  evaluateEscapedRays (raygeneration.cu:105-136) and
+ *                                    evaluateMaterialsAndLightsPathTracing (:285-456)
+ *     k_shadow  any hit + accumulate OptiX shadowRay raygen + miss (optix.cu:77-110, 265-284)
+ *   k_batch_end                      per-batch counters -> totals
+ * All kernels are grid-stride loops over device-side queue counts, so a
+ * batch needs no host synchronisation (the reference resets counters with
+ * <<<1,1>>> launches and synchronises after each kernel, cudadevice.h:185-219).
+ * Queue pushes are wave64-aggregated: one atomic per wave (ballot + mbcnt),
+ * replacing one atomicAdd per item (soa.h:39-49).
+ */
+#include "clpt_device.h"
+#include "clpt_kernels.h"
+
+#define CLPT_LDS_STACK 16
+#define CLPT_STACK_TOTAL 128
+
+/* ------------------------------------------------------------ wave utils */
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+/* Reserve `want` slots with one atomic per wave; returns this lane's slot. */
+__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool want)
+{
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0ull)
+        return 0u;
+    const uint32_t leader = (uint32_t)(__ffsll((long long)mask) - 1);
+    uint32_t       base   = 0;
+    if (lane_id() == leader)
+        base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+    const unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
+    return base + (uint32_t)__popcll(below);
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ void wave_count(uint32_t* counter, bool pred)
+{
+    const unsigned long long mask = __ballot(pred);
+    if (mask != 0ull && lane_id() == (uint32_t)(__ffsll((long long)mask) - 1))
+        atomicAdd(counter, (uint32_t)__popcll(mask));
+}
+
+__device__ __forceinline__ void accum_add(double* acc, uint32_t pixel, f3 c)
+{
+    unsafeAtomicAdd(acc + 3 * (size_t)pixel + 0, (double)c.x);
+    unsafeAtomicAdd(acc + 3 * (size_t)pixel + 1, (double)c.y);
+    unsafeAtomicAdd(acc + 3 * (size_t)pixel + 2, (double)c.z);
+}
+
+/* ------------------------------------------------------------ traversal */
+
+struct TraverseStack
+{
+    uint32_t* lds;     /* [CLPT_LDS_STACK][blockDim] */
+    uint32_t* gmem;    /* [CLPT_STACK_TOTAL - CLPT_LDS_STACK][gstride] */
+    uint32_t  tid, gid, gstride;
+    int       sp;
+    __device__ __forceinline__ void push(uint32_t v)
+    {
+        if (sp < CLPT_LDS_STACK)
+            lds[sp * blockDim.x + tid] = v;
+        else if (sp < CLPT_STACK_TOTAL)
+            gmem[(size_t)(sp - CLPT_LDS_STACK) * gstride + gid] = v;
+        /* deeper pushes are dropped; the builder bounds the tree depth so
+         * that CLPT_STACK_TOTAL is never reached */
+        ++sp;
+    }
+    __device__ __forceinline__ uint32_t pop()
+    {
+        --sp;
+        if (sp < CLPT_LDS_STACK)
+            return lds[sp * blockDim.x + tid];
+        if (sp < CLPT_STACK_TOTAL)
+            return gmem[(size_t)(sp - CLPT_LDS_STACK) * gstride + gid];
+        return CLPT_CHILD_EMPTY;
+    }
+};
+
+struct HitRec
+{
+    float    t, b1, b2;
+    uint32_t inst, prim;
+};
+
+/* Two-level BVH4 traversal; ANY = occlusion (terminate at the first hit,
+ * D15: optix.cu:106 traces shadow rays with closest-hit flags). */
+template <bool ANY>
+__device__ __forceinline__ bool traverse(const SceneView& sc, const RayW& wr, float tmax, HitRec& best,
+                                         TraverseStack& st, uint32_t& nnodes, uint32_t& ntris)
+{
+    RayW     r        = wr;
+    uint32_t cur_inst = 0xffffffffu;
+    best.t            = tmax;
+    best.inst         = 0xffffffffu;
+    best.prim         = 0xffffffffu;
+    best.b1 = best.b2 = 0.0f;
+    bool found        = false;
+    st.sp             = 0;
+    st.push(sc.tlas_root);
+    while (st.sp > 0)
+    {
+        uint32_t ref = st.pop();
+        if (ref == CLPT_STACK_MARKER)
+        {
+            r        = wr;
+            cur_inst = 0xffffffffu;
+            continue;
+        }
+        if (ref == CLPT_CHILD_EMPTY)
+            continue;
+        if (ref & CLPT_LEAF_BIT)
+        {
+            cur_inst              = ref & ~(CLPT_LEAF_BIT | CLPT_INST_BIT);
+            const InstanceRec* ip = sc.inst + cur_inst;
+            float              m[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k)
+                m[k] = ip->inv[k];
+            ray_to_object(m, wr, r);
+            st.push(CLPT_STACK_MARKER);
+            ref = ip->blas_root;
+        }
+        const float4* np  = reinterpret_cast<const float4*>(sc.nodes + ref);
+        const float4  lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+        const uint4   ch  = *reinterpret_cast<const uint4*>(np + 6);
+        ++nnodes;
+        const float lo_x[4] = {lox.x, lox.y, lox.z, lox.w}, hi_x[4] = {hix.x, hix.y, hix.z, hix.w};
+        const float lo_y[4] = {loy.x, loy.y, loy.z, loy.w}, hi_y[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+        const float lo_z[4] = {loz.x, loz.y, loz.z, loz.w}, hi_z[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+        const uint32_t child[4] = {ch.x, ch.y, ch.z, ch.w};
+        float          tn[4];
+        uint32_t       cref[4];
+        int            ncand = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            const float t0x = (lo_x[c] - r.o.x) * r.inv.x, t1x = (hi_x[c] - r.o.x) * r.inv.x;
+            const float t0y = (lo_y[c] - r.o.y) * r.inv.y, t1y = (hi_y[c] - r.o.y) * r.inv.y;
+            const float t0z = (lo_z[c] - r.o.z) * r.inv.z, t1z = (hi_z[c] - r.o.z) * r.inv.z;
+            float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+            float tfar  = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best.t));
+            tnear       = tnear * 0.99999f;
+            tfar        = tfar * 1.00001f;
+            const bool hitc = child[c] != CLPT_CHILD_EMPTY && tnear <= tfar;
+            if (!hitc)
+                continue;
+            if ((child[c] & (CLPT_LEAF_BIT | CLPT_INST_BIT)) == CLPT_LEAF_BIT)
+            {
+                const uint32_t first = child[c] & (CLPT_MAX_TRIS - 1u);
+                const uint32_t cnt   = ((child[c] >> 27) & 7u) + 1u;
+                for (uint32_t j = 0; j < cnt; ++j)
+                {
+                    const TriRec tr = sc.tris[first + j];
+                    ++ntris;
+                    float tt, b1, b2;
+                    if (tri_test(r, best.t, f3{tr.v0p.x, tr.v0p.y, tr.v0p.z}, f3{tr.v1.x, tr.v1.y, tr.v1.z},
+                                 f3{tr.v2.x, tr.v2.y, tr.v2.z}, &tt, &b1, &b2))
+                    {
+                        const uint32_t prim = __float_as_uint(tr.v0p.w);
+                        if (ANY)
+                            return true;
+                        if (hit_better(tt, cur_inst, prim, best.t, best.inst, best.prim))
+                        {
+                            best.t    = tt;
+                            best.b1   = b1;
+                            best.b2   = b2;
+                            best.inst = cur_inst;
+                            best.prim = prim;
+                            found     = true;
+                        }
+                    }
+                }
+            }
+            else
+            {
+                tn[ncand]   = tnear;
+                cref[ncand] = child[c];
+                ++ncand;
+            }
+        }
+        /* push far-to-near so the nearest child is popped next */
+        for (int a = 1; a < ncand; ++a)
+            for (int b = a; b > 0 && tn[b] > tn[b - 1]; --b)
+            {
+                const float    tf = tn[b];
+                const uint32_t rf = cref[b];
+                tn[b]             = tn[b - 1];
+                cref[b]           = cref[b - 1];
+                tn[b - 1]         = tf;
+                cref[b - 1]       = rf;
+            }
+        for (int a = 0; a < ncand; ++a)
+            st.push(cref[a]);
+    }
+    return found;
+}
+
+/* ------------------------------------------------------------ shading */
+
+struct Surface
+{
+    f3       p, ng;
+    Frame    f;
+    float    u, v;
+    uint32_t material;
+};
+
+/* Closest-hit epilogue (optix.cu:130-228) with the renderer's corrections:
+ * degenerate Ns tested before normalising; frame on Ns (D10). */
+__device__ __forceinline__ void surface_interaction(const SceneView& sc, const HitRec& h, Surface& si)
+{
+    const InstanceRec* ip = sc.inst + h.inst;
+    const MeshRec      me = sc.meshes[ip->mesh];
+    const uint32_t     i0 = sc.idx[3 * (size_t)(me.idx_off + h.prim) + 0] + me.vtx_off;
+    const uint32_t     i1 = sc.idx[3 * (size_t)(me.idx_off + h.prim) + 1] + me.vtx_off;
+    const uint32_t     i2 = sc.idx[3 * (size_t)(me.idx_off + h.prim) + 2] + me.vtx_off;
+    const float        b1 = h.b1, b2 = h.b2;
+    const float        b0 = 1.0f - b1 - b2;
+    const f3 p0 = ld3(sc.pos + 3 * (size_t)i0), p1 = ld3(sc.pos + 3 * (size_t)i1), p2 = ld3(sc.pos + 3 * (size_t)i2);
+    const f3 po = add3(add3(sclL3(b0, p0), sclL3(b1, p1)), sclL3(b2, p2));
+    if (me.flags & MESH_HAS_UVS)
+    {
+        const float* t0 = sc.uv + 2 * (size_t)i0;
+        const float* t1 = sc.uv + 2 * (size_t)i1;
+        const float* t2 = sc.uv + 2 * (size_t)i2;
+        si.u            = b0 * t0[0] + b1 * t1[0] + b2 * t2[0];
+        si.v            = b0 * t0[1] + b1 * t1[1] + b2 * t2[1];
+    }
+    else
+    {
+        si.u = b0;
+        si.v = b1;
+    }
+    const f3 ngo = cross3(sub3(p1, p0), sub3(p2, p0));
+    f3       nso = ngo;
+    if (me.flags & MESH_HAS_NORMALS)
+        nso = add3(add3(sclL3(b0, ld3(sc.nrm + 3 * (size_t)i0)), sclL3(b1, ld3(sc.nrm + 3 * (size_t)i1))),
+                   sclL3(b2, ld3(sc.nrm + 3 * (size_t)i2)));
+    f3 tgo{0.0f, 0.0f, 0.0f};
+    if (me.flags & MESH_HAS_TANGENTS)
+        tgo = add3(add3(sclL3(b0, ld3(sc.tan + 3 * (size_t)i0)), sclL3(b1, ld3(sc.tan + 3 * (size_t)i1))),
+                   sclL3(b2, ld3(sc.tan + 3 * (size_t)i2)));
+    float M[12], I[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+    {
+        M[k] = ip->xf[k];
+        I[k] = ip->inv[k];
+    }
+    si.p = f3{M[0] * po.x + M[1] * po.y + M[2] * po.z + M[3], M[4] * po.x + M[5] * po.y + M[6] * po.z + M[7],
+              M[8] * po.x + M[9] * po.y + M[10] * po.z + M[11]};
+    const f3 ngw = f3{I[0] * ngo.x + I[4] * ngo.y + I[8] * ngo.z, I[1] * ngo.x + I[5] * ngo.y + I[9] * ngo.z,
+                      I[2] * ngo.x + I[6] * ngo.y + I[10] * ngo.z};
+    const f3 nsw = f3{I[0] * nso.x + I[4] * nso.y + I[8] * nso.z, I[1] * nso.x + I[5] * nso.y + I[9] * nso.z,
+                      I[2] * nso.x + I[6] * nso.y + I[10] * nso.z};
+    const f3 tgw = f3{M[0] * tgo.x + M[1] * tgo.y + M[2] * tgo.z, M[4] * tgo.x + M[5] * tgo.y + M[6] * tgo.z,
+                      M[8] * tgo.x + M[9] * tgo.y + M[10] * tgo.z};
+    f3 Ng = norm3(ngw);
+    f3 Ns = dot3(nsw, nsw) == 0.0f ? Ng : norm3(nsw);
+    if (dot3(Ng, Ns) < 0.0f)
+        Ng = neg3(Ng);
+    f3 dpdu = tgw, dpdv;
+    if (dot3(dpdu, dpdu) > 0.0f)
+    {
+        dpdu = norm3(dpdu);
+        dpdv = cross3(Ns, dpdu);
+        if (dot3(dpdv, dpdv) > 0.0f)
+        {
+            dpdv = norm3(dpdv);
+            dpdu = cross3(dpdv, Ns);
+        }
+        else
+            make_frame(Ns, &dpdu, &dpdv);
+    }
+    else
+        make_frame(Ns, &dpdu, &dpdv);
+    si.ng       = Ng;
+    si.f.s      = dpdv;
+    si.f.t      = dpdu;
+    si.f.n      = Ns;
+    si.material = ip->material;
+}
+
+/* ------------------------------------------------------------ kernels */
+
+__device__ __forceinline__ void camera_ray(const FrameView& fr, uint32_t px, uint32_t py, uint32_t sample, f3* o, f3* d)
+{
+    const uint32_t z  = (fr.view << 20) | sample;
+    const float    u0 = rng_draw(px, py, z, 1u), u1 = rng_draw(px, py, z, 2u);
+    const float    sx = ((float)px + u0) / (float)fr.width;
+    const float    sy = ((float)py + u1) / (float)fr.height;
+    const f3 d00 = ld3(fr.d00), ddu = ld3(fr.ddu), ddv = ld3(fr.ddv);
+    *o           = ld3(fr.pos);
+    *d           = norm3(add3(add3(d00, sclL3(sx, ddu)), sclL3(sy, ddv)));
+}
+
+__global__ __launch_bounds__(256) void k_camera(FrameView fr, uint32_t sample_base, uint32_t npaths, RayQueue q,
+                                                BatchCounters* bc)
+{
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0)
+    {
+        for (int b = 0; b < CLPT_MAXD + 2; ++b)
+        {
+            bc->rays[b]    = 0u;
+            bc->hits[b]    = 0u;
+            bc->shadow[b]  = 0u;
+            bc->visible[b] = 0u;
+        }
+        bc->rays[1] = npaths;
+    }
+    for (uint32_t i = gid; i < npaths; i += gridDim.x * blockDim.x)
+    {
+        const uint32_t pixel  = i % fr.npix;
+        const uint32_t sample = sample_base + i / fr.npix;
+        f3             o, d;
+        camera_ray(fr, pixel % fr.width, pixel / fr.width, sample, &o, &d);
+        q.o[i] = make_float4(o.x, o.y, o.z, 0.0f);
+        q.d[i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));
+        q.T[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_trace(SceneView sc, RayQueue q, const uint32_t* count, uint32_t depth,
+                                               HitBuf hb, uint32_t* gstack, unsigned long long* ctr)
+{
+    __shared__ uint32_t lds[CLPT_LDS_STACK * 256];
+    TraverseStack       st;
+    st.lds     = lds;
+    st.gmem    = gstack;
+    st.tid     = threadIdx.x;
+    st.gid     = blockIdx.x * blockDim.x + threadIdx.x;
+    st.gstride = gridDim.x * blockDim.x;
+    uint32_t       nn = 0, nt = 0;
+    const uint32_t n  = *count;
+    for (uint32_t i = st.gid; i < n; i += st.gstride)
+    {
+        const float4 o = q.o[i], d = q.d[i];
+        RayW         r;
+        r.o    = f3{o.x, o.y, o.z};
+        r.d    = f3{d.x, d.y, d.z};
+        r.tmin = depth == 1 ? 0.0f : CLPT_RAY_EPS;
+        ray_setup(r);
+        HitRec     h;
+        const bool found = traverse<false>(sc, r, CLPT_FLT_MAX, h, st, nn, nt);
+        hb.h[i]          = make_float4(found ? h.t : -1.0f, h.b1, h.b2, __uint_as_float(found ? h.inst : 0xffffffffu));
+        hb.prim[i]       = found ? h.prim : 0xffffffffu;
+    }
+    const unsigned long long sn = wave_sum_u64(nn), stt = wave_sum_u64(nt);
+    if (lane_id() == 0)
+    {
+        atomicAdd(ctr + 0, sn);
+        atomicAdd(ctr + 1, stt);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_shade(SceneView sc, FrameView fr, uint32_t depth, uint32_t sample_base,
+                                               RayQueue qin, HitBuf hb, RayQueue qout, ShadowQueue sq, double* accum,
+                                               BatchCounters* bc)
+{
+    const uint32_t n      = bc->rays[depth];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const bool     dome   = sc.dome_index >= 0;
+    const float    nl     = (float)sc.nemitters;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        const float4   o4 = qin.o[i], d4 = qin.d[i], T4 = qin.T[i];
+        const float4   h4 = hb.h[i];
+        const uint32_t path   = __float_as_uint(d4.w);
+        const uint32_t pixel  = path % fr.npix;
+        const uint32_t sample = sample_base + path / fr.npix;
+        const uint32_t px = pixel % fr.width, py = pixel / fr.width;
+        const uint32_t z  = (fr.view << 20) | sample;
+        const f3       dir{d4.x, d4.y, d4.z};
+        f3             T{T4.x, T4.y, T4.z};
+        const bool     hit = h4.x >= 0.0f;
+        bool           want_shadow = false, want_next = false;
+        f3             Li{0.0f, 0.0f, 0.0f}, ldir{0.0f, 0.0f, 0.0f}, ndir{0.0f, 0.0f, 0.0f}, p{0.0f, 0.0f, 0.0f};
+        float          npdf = 0.0f;
+        if (!hit)
+        {
+            /* escaped ray (raygeneration.cu:105-136, D3 corrected) */
+            if (dome)
+            {
+                const f3 le = dome_le(sc, dir);
+                if (depth == 1)
+                    accum_add(accum, pixel, le);
+                else
+                {
+                    const float lp = dome_pdf(sc, dir) / nl;
+                    const float w  = o4.w / (o4.w + lp);
+                    accum_add(accum, pixel, mul3(T, scl3(le, w)));
+                }
+            }
+        }
+        else if (depth < fr.max_depth)
+        {
+            HitRec hr;
+            hr.t    = h4.x;
+            hr.b1   = h4.y;
+            hr.b2   = h4.z;
+            hr.inst = __float_as_uint(h4.w);
+            hr.prim = hb.prim[i];
+            Surface si;
+            surface_interaction(sc, hr, si);
+            p                    = si.p;
+            const MaterialRec mt = sc.mtl[si.material];
+            const f3          wo = neg3(dir);
+            const f3          woL = to_local(si.f, wo);
+            const uint32_t    k0  = 2u + 4u * (depth - 1u);
+            /* next-event estimation (raygeneration.cu:310-384; D8, D9) */
+            {
+                const float lu = rng_draw(px, py, z, k0 + 1u), lv = rng_draw(px, py, z, k0 + 2u);
+                float       lpdf  = 0.0f;
+                uint32_t    which = 0;
+                const f3    value = sample_emitters(sc, lu, lv, &ldir, &lpdf, &which);
+                if (nonzero3(value) && lpdf > 0.0f)
+                {
+                    const f3 wiL = to_local(si.f, ldir);
+                    const f3 f   = bsdf_eval(mt, wiL, woL);
+                    if (nonzero3(f))
+                    {
+                        float weight = 1.0f;
+                        if (sc.emitters[which].type == 1u)
+                        {
+                            const float bp = bsdf_pdf(mt, wiL, woL);
+                            const float lp = lpdf / nl;
+                            weight         = lp / (lp + bp);
+                        }
+                        const float cosv = fabsf(wiL.z);
+                        Li          = f3{T.x * value.x * f.x * cosv * weight / lpdf, T.y * value.y * f.y * cosv * weight / lpdf,
+                                T.z * value.z * f.z * cosv * weight / lpdf};
+                        want_shadow = true;
+                    }
+                }
+            }
+            /* BSDF sampling (raygeneration.cu:391-453; D7, D11, D13) */
+            const float bu = rng_draw(px, py, z, k0 + 3u), bv = rng_draw(px, py, z, k0 + 4u);
+            const float ul = rng_draw(px, py, z, CLPT_EXTRA_DRAW(depth, CLPT_SLOT_LOBE));
+            f3          wiL{0.0f, 0.0f, 0.0f};
+            float       bpdf = 0.0f;
+            const f3    f    = bsdf_sample(mt, woL, bu, bv, ul, &wiL, &bpdf);
+            if (nonzero3(f) && bpdf > 0.0f)
+            {
+                const float cosv = fabsf(wiL.z);
+                T         = f3{T.x * (f.x * cosv / bpdf), T.y * (f.y * cosv / bpdf), T.z * (f.z * cosv / bpdf)};
+                ndir      = to_world(si.f, wiL);
+                npdf      = bpdf;
+                want_next = true;
+                if (fr.rr_depth > 0u && depth >= fr.rr_depth)
+                {
+                    const float q  = fminf(1.0f, fmaxf(T.x, fmaxf(T.y, T.z)));
+                    const float ur = rng_draw(px, py, z, CLPT_EXTRA_DRAW(depth, CLPT_SLOT_RR));
+                    if (!(ur < q))
+                        want_next = false;
+                    else
+                        T = f3{T.x / q, T.y / q, T.z / q};
+                }
+            }
+        }
+        /* converged: wave-aggregated queue pushes */
+        wave_count(&bc->hits[depth], hit);
+        const uint32_t ss = wave_push(&bc->shadow[depth], want_shadow);
+        if (want_shadow)
+        {
+            sq.o[ss] = make_float4(p.x, p.y, p.z, Li.x);
+            sq.d[ss] = make_float4(ldir.x, ldir.y, ldir.z, Li.y);
+            sq.x[ss] = make_float2(Li.z, __uint_as_float(pixel));
+        }
+        const uint32_t ns = wave_push(&bc->rays[depth + 1], want_next);
+        if (want_next)
+        {
+            qout.o[ns] = make_float4(p.x, p.y, p.z, npdf);
+            qout.d[ns] = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(path));
+            qout.T[ns] = make_float4(T.x, T.y, T.z, 0.0f);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_shadow(SceneView sc, ShadowQueue sq, uint32_t depth, double* accum,
+                                                BatchCounters* bc, uint32_t* gstack, unsigned long long* ctr)
+{
+    __shared__ uint32_t lds[CLPT_LDS_STACK * 256];
+    TraverseStack       st;
+    st.lds     = lds;
+    st.gmem    = gstack;
+    st.tid     = threadIdx.x;
+    st.gid     = blockIdx.x * blockDim.x + threadIdx.x;
+    st.gstride = gridDim.x * blockDim.x;
+    uint32_t       nn = 0, nt = 0;
+    const uint32_t n  = bc->shadow[depth];
+    for (uint32_t i = st.gid; i < n; i += st.gstride)
+    {
+        const float4 o = sq.o[i], d = sq.d[i];
+        const float2 x = sq.x[i];
+        RayW         r;
+        r.o    = f3{o.x, o.y, o.z};
+        r.d    = f3{d.x, d.y, d.z};
+        r.tmin = CLPT_RAY_EPS;
+        ray_setup(r);
+        HitRec     h;
+        const bool occ = traverse<true>(sc, r, CLPT_FLT_MAX, h, st, nn, nt);
+        if (!occ)
+            accum_add(accum, __float_as_uint(x.y), f3{o.w, d.w, x.x});
+        wave_count(&bc->visible[depth], !occ);
+    }
+    const unsigned long long sn = wave_sum_u64(nn), stt = wave_sum_u64(nt);
+    if (lane_id() == 0)
+    {
+        atomicAdd(ctr + 2, sn);
+        atomicAdd(ctr + 3, stt);
+    }
+}
+
+__global__ void k_batch_end(const BatchCounters* bc, TotalCounters* tot)
+{
+    const int b = threadIdx.x;
+    if (b < CLPT_MAXD + 2)
+    {
+        tot->rays[b] += bc->rays[b];
+        tot->hits[b] += bc->hits[b];
+        tot->shadow[b] += bc->shadow[b];
+        tot->visible[b] += bc->visible[b];
+    }
+}
+
+__global__ void k_resolve(const double* accum, uint32_t npix, uint32_t spp, float4* out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x)
+    {
+        const double inv = spp ? 1.0 / (double)spp : 0.0;
+        out[i] = make_float4((float)(accum[3 * (size_t)i] * inv), (float)(accum[3 * (size_t)i + 1] * inv),
+                             (float)(accum[3 * (size_t)i + 2] * inv), 1.0f);
+    }
+}
+
+/* ------------------------------------------------------------ HDRI tables */
+
+/* prefiltering (hdridome.h:311-324): sin(theta_v) * luminance(texel). */
+__global__ void k_dome_prefilter(const float4* tex, uint32_t w, uint32_t h, float* pcond)
+{
+    const uint32_t total = w * h;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x)
+    {
+        const uint32_t v  = i / w, u = i % w;
+        const float    st = clpt_sinf((CLPT_PI_F * ((float)v + 0.5f)) / (float)h);
+        const float4   c  = tex[i];
+        const float    lum = (float)((double)c.x * 0.212671 + (double)c.y * 0.71516 + (double)c.z * 0.072169);
+        pcond[(size_t)v * (w + 1) + u] = st * lum;
+    }
+}
+
+/* Block-wide exclusive scan of one double per thread (256 threads). */
+__device__ double block_exclusive_scan(double v, double* sh, double* total)
+{
+    const uint32_t t    = threadIdx.x;
+    const uint32_t lane = t & 63u, wave = t >> 6;
+    double         x    = v;
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const double y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off)
+            x += y;
+    }
+    if (lane == 63u)
+        sh[wave] = x;
+    __syncthreads();
+    double wave_off = 0.0, tot = 0.0;
+    for (uint32_t k = 0; k < (blockDim.x >> 6); ++k)
+    {
+        if (k < wave)
+            wave_off += sh[k];
+        tot += sh[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return wave_off + x - v;
+}
+
+/* preprocessPCondV (hdridome.h:333-385) in float64: one block per row;
+ * each thread scans a contiguous chunk, then a block scan joins them. */
+__global__ __launch_bounds__(256) void k_dome_rows(uint32_t w, uint32_t h, float* pcond, float* cdfcond, float* pv,
+                                                    double* rowint)
+{
+    __shared__ double sh[8];
+    const uint32_t    v     = blockIdx.x;
+    float*            p     = pcond + (size_t)v * (w + 1);
+    float*            cdf   = cdfcond + (size_t)v * (w + 1);
+    const uint32_t    chunk = (w + blockDim.x - 1) / blockDim.x;
+    const uint32_t    b     = threadIdx.x * chunk;
+    const uint32_t    e     = min(b + chunk, w);
+    double            local = 0.0;
+    for (uint32_t i = b; i < e; ++i)
+        local += (double)p[i] / (double)w;
+    double       total;
+    const double off = block_exclusive_scan(local, sh, &total);
+    if (total <= 0.0)
+    {
+        for (uint32_t i = b; i < e; ++i)
+        {
+            p[i]   = 0.0f;
+            cdf[i] = (float)i / (float)w;
+        }
+        if (threadIdx.x == 0)
+        {
+            cdf[w]    = 1.0f;
+            p[w]      = 0.0f;
+            rowint[v] = 0.0;
+            pv[v]     = 0.0f;
+        }
+        return;
+    }
+    double acc = off;
+    if (threadIdx.x == 0)
+        cdf[0] = 0.0f;
+    for (uint32_t i = b; i < e; ++i)
+    {
+        acc += (double)p[i] / (double)w;
+        cdf[i + 1] = (float)(acc / total);
+    }
+    if (threadIdx.x == 0)
+    {
+        p[w]      = (float)total;
+        rowint[v] = total;
+        pv[v]     = (float)total;
+    }
+}
+
+/* preprocessPV (hdridome.h:387-432) in float64: one block. */
+__global__ __launch_bounds__(256) void k_dome_marginal(uint32_t h, float* pv, float* cdfv, const double* rowint)
+{
+    __shared__ double sh[8];
+    const uint32_t    chunk = (h + blockDim.x - 1) / blockDim.x;
+    const uint32_t    b     = threadIdx.x * chunk;
+    const uint32_t    e     = min(b + chunk, h);
+    double            local = 0.0;
+    for (uint32_t j = b; j < e; ++j)
+        local += rowint[j] / (double)h;
+    double       total;
+    const double off = block_exclusive_scan(local, sh, &total);
+    if (total <= 0.0)
+    {
+        for (uint32_t j = b; j < e; ++j)
+        {
+            pv[j]   = 0.0f;
+            cdfv[j] = (float)j / (float)h;
+        }
+        if (threadIdx.x == 0)
+        {
+            pv[h]   = 0.0f;
+            cdfv[h] = 1.0f;
+        }
+        return;
+    }
+    double acc = off;
+    if (threadIdx.x == 0)
+        cdfv[0] = 0.0f;
+    for (uint32_t j = b; j < e; ++j)
+    {
+        acc += rowint[j] / (double)h;
+        cdfv[j + 1] = (float)(acc / total);
+    }
+    if (threadIdx.x == 0)
+        pv[h] = (float)total;
+}
+
+/* ------------------------------------------------------------ batch kernels */
+
+__global__ void k_rng_batch(const uint4* seeds, uint32_t n, uint32_t ndraws, uint32_t* out_u, float* out_f)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint4 s = seeds[i];
+        for (uint32_t d = 0; d < ndraws; ++d)
+        {
+            const uint32_t x            = pcg4d_x(s.x, s.y, s.z, s.w + d + 1u);
+            out_u[(size_t)i * ndraws + d] = x;
+            out_f[(size_t)i * ndraws + d] = u2f01(x);
+        }
+    }
+}
+
+__global__ void k_camera_batch(FrameView fr, const uint2* ps, uint32_t n, float4* out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint2 q = ps[i];
+        f3          o, d;
+        camera_ray(fr, q.x % fr.width, q.x / fr.width, q.y, &o, &d);
+        out[2 * i]     = make_float4(o.x, o.y, o.z, 0.0f);
+        out[2 * i + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+    }
+}
+
+__global__ void k_bsdf_batch(SceneView sc, const uint32_t* mtl, const float* wi3, const float* wo3, const float* u3,
+                             uint32_t n, float* ef3, float* epdf, float* swi3, float* sf3, float* spdf)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const MaterialRec m  = sc.mtl[mtl[i]];
+        const f3          wi = ld3(wi3 + 3 * (size_t)i), wo = ld3(wo3 + 3 * (size_t)i);
+        const f3          f  = bsdf_eval(m, wi, wo);
+        ef3[3 * i]           = f.x;
+        ef3[3 * i + 1]       = f.y;
+        ef3[3 * i + 2]       = f.z;
+        epdf[i]              = bsdf_pdf(m, wi, wo);
+        f3       w{0.0f, 0.0f, 0.0f};
+        float    p  = 0.0f;
+        const f3 fs = bsdf_sample(m, wo, u3[3 * i], u3[3 * i + 1], u3[3 * i + 2], &w, &p);
+        swi3[3 * i]     = w.x;
+        swi3[3 * i + 1] = w.y;
+        swi3[3 * i + 2] = w.z;
+        sf3[3 * i]      = fs.x;
+        sf3[3 * i + 1]  = fs.y;
+        sf3[3 * i + 2]  = fs.z;
+        spdf[i]         = p;
+    }
+}
+
+__global__ void k_emitter_batch(SceneView sc, const float* u2, uint32_t n, float* dir3, float* val3, float* pdf,
+                                uint32_t* which)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        f3       d{0.0f, 0.0f, 0.0f};
+        float    p = 0.0f;
+        uint32_t w = 0;
+        const f3 v = sample_emitters(sc, u2[2 * i], u2[2 * i + 1], &d, &p, &w);
+        dir3[3 * i]     = d.x;
+        dir3[3 * i + 1] = d.y;
+        dir3[3 * i + 2] = d.z;
+        val3[3 * i]     = v.x;
+        val3[3 * i + 1] = v.y;
+        val3[3 * i + 2] = v.z;
+        pdf[i]          = p;
+        which[i]        = w;
+    }
+}
+
+__global__ void k_dome_eval_batch(SceneView sc, const float* dir3, uint32_t n, float* le3, float* pdf)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const f3 d  = ld3(dir3 + 3 * (size_t)i);
+        const f3 le = dome_le(sc, d);
+        le3[3 * i]     = le.x;
+        le3[3 * i + 1] = le.y;
+        le3[3 * i + 2] = le.z;
+        pdf[i]         = dome_pdf(sc, d);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_trace_batch(SceneView sc, const float4* rays, uint32_t n, int any,
+                                                     float4* out, uint32_t* prim, uint32_t* gstack)
+{
+    __shared__ uint32_t lds[CLPT_LDS_STACK * 256];
+    TraverseStack       st;
+    st.lds     = lds;
+    st.gmem    = gstack;
+    st.tid     = threadIdx.x;
+    st.gid     = blockIdx.x * blockDim.x + threadIdx.x;
+    st.gstride = gridDim.x * blockDim.x;
+    uint32_t nn = 0, nt = 0;
+    for (uint32_t i = st.gid; i < n; i += st.gstride)
+    {
+        const float4 a = rays[2 * i], b = rays[2 * i + 1];
+        RayW         r;
+        r.o    = f3{a.x, a.y, a.z};
+        r.tmin = a.w;
+        r.d    = f3{b.x, b.y, b.z};
+        ray_setup(r);
+        HitRec h;
+        if (any)
+        {
+            const bool occ = traverse<true>(sc, r, b.w, h, st, nn, nt);
+            out[i]         = make_float4(occ ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f);
+            prim[i]        = 0u;
+        }
+        else
+        {
+            const bool found = traverse<false>(sc, r, b.w, h, st, nn, nt);
+            out[i]  = make_float4(found ? h.t : -1.0f, found ? h.b1 : 0.0f, found ? h.b2 : 0.0f,
+                                 __uint_as_float(found ? h.inst : 0xffffffffu));
+            prim[i] = found ? h.prim : 0xffffffffu;
+        }
+    }
+}
+
+/* ------------------------------------------------------------ launchers */
+
+namespace clpt
+{
+static inline uint32_t grid_for(uint64_t items, uint32_t block, uint32_t max_blocks)
+{
+    uint64_t g = (items + block - 1) / block;
+    if (g < 1)
+        g = 1;
+    if (g > max_blocks)
+        g = max_blocks;
+    return (uint32_t)g;
+}
+
+hipError_t launch_camera(hipStream_t s, const FrameView& fr, uint32_t sample_base, uint32_t npaths, const RayQueue& q,
+                         BatchCounters* bc, uint32_t max_blocks)
+{
+    hipLaunchKernelGGL(k_camera, dim3(grid_for(npaths, 256, max_blocks)), dim3(256), 0, s, fr, sample_base, npaths, q, bc);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, const uint32_t* count, uint32_t depth,
+                        const HitBuf& hb, uint32_t* gstack, unsigned long long* ctr, uint32_t blocks)
+{
+    hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(256), 0, s, sc, q, count, depth, hb, gstack, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(hipStream_t s, const SceneView& sc, const FrameView& fr, uint32_t depth, uint32_t sample_base,
+                        const RayQueue& qin, const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq,
+                        double* accum, BatchCounters* bc, uint32_t blocks)
+{
+    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, sc, fr, depth, sample_base, qin, hb, qout, sq, accum, bc);
+    return hipGetLastError();
+}
+
+hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& sq, uint32_t depth, double* accum,
+                         BatchCounters* bc, uint32_t* gstack, unsigned long long* ctr, uint32_t blocks)
+{
+    hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(256), 0, s, sc, sq, depth, accum, bc, gstack, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_end(hipStream_t s, const BatchCounters* bc, TotalCounters* tot)
+{
+    hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, s, bc, tot);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(hipStream_t s, const double* accum, uint32_t npix, uint32_t spp, float4* out)
+{
+    hipLaunchKernelGGL(k_resolve, dim3(grid_for(npix, 256, 4096)), dim3(256), 0, s, accum, npix, spp, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_dome_tables(hipStream_t s, const float4* tex, uint32_t w, uint32_t h, float* pcond, float* cdfcond,
+                              float* pv, float* cdfv, double* rowint)
+{
+    hipLaunchKernelGGL(k_dome_prefilter, dim3(grid_for((uint64_t)w * h, 256, 8192)), dim3(256), 0, s, tex, w, h, pcond);
+    hipLaunchKernelGGL(k_dome_rows, dim3(h), dim3(256), 0, s, w, h, pcond, cdfcond, pv, rowint);
+    hipLaunchKernelGGL(k_dome_marginal, dim3(1), dim3(256), 0, s, h, pv, cdfv, rowint);
+    return hipGetLastError();
+}
+
+hipError_t launch_rng_batch(hipStream_t s, const uint4* seeds, uint32_t n, uint32_t ndraws, uint32_t* ou, float* of)
+{
+    hipLaunchKernelGGL(k_rng_batch, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, seeds, n, ndraws, ou, of);
+    return hipGetLastError();
+}
+
+hipError_t launch_camera_batch(hipStream_t s, const FrameView& fr, const uint2* ps, uint32_t n, float4* out)
+{
+    hipLaunchKernelGGL(k_camera_batch, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, fr, ps, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bsdf_batch(hipStream_t s, const SceneView& sc, const uint32_t* mtl, const float* wi3,
+                             const float* wo3, const float* u3, uint32_t n, float* ef3, float* epdf, float* swi3,
+                             float* sf3, float* spdf)
+{
+    hipLaunchKernelGGL(k_bsdf_batch, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, sc, mtl, wi3, wo3, u3, n, ef3, epdf,
+                       swi3, sf3, spdf);
+    return hipGetLastError();
+}
+
+hipError_t launch_emitter_batch(hipStream_t s, const SceneView& sc, const float* u2, uint32_t n, float* dir3,
+                                float* val3, float* pdf, uint32_t* which)
+{
+    hipLaunchKernelGGL(k_emitter_batch, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, sc, u2, n, dir3, val3, pdf, which);
+    return hipGetLastError();
+}
+
+hipError_t launch_dome_eval_batch(hipStream_t s, const SceneView& sc, const float* dir3, uint32_t n, float* le3,
+                                  float* pdf)
+{
+    hipLaunchKernelGGL(k_dome_eval_batch, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, sc, dir3, n, le3, pdf);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_batch(hipStream_t s, const SceneView& sc, const float4* rays, uint32_t n, int any, float4* out,
+                              uint32_t* prim, uint32_t* gstack, uint32_t blocks)
+{
+    hipLaunchKernelGGL(k_trace_batch, dim3(blocks), dim3(256), 0, s, sc, rays, n, any, out, prim, gstack);
+    return hipGetLastError();
+}
+} // namespace clpt

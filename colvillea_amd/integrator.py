@@ -1,0 +1,314 @@
+"""Host-side mirror of Colvillea's rendering interface over the C ABI.
+
+`Integrator` keeps the method names, argument meaning and call order of
+core::Integrator (include/colvillea/librender/integrator.h:34-103) and of
+Integrator::createIntegrator (src/core/librender/integrator.cpp:10-22), so a
+RenderEngine-style driver (renderengine.cpp:27-121) can call it unchanged.
+The value types mirror the reference's kernel records:
+TriangleMesh (nodes/shapes/trianglemesh.h), DiffuseMtl / MetalMtl / GlassMtl
+(libkernel/materials/*.h), DirectionalEmitter / HDRIDome
+(libkernel/emitters/*.h), core::Camera (librender/nodebase/camera.h:8-27).
+
+Errors raise ClhipError (the reference aborts the process instead,
+src/core/devices/cudacommon.h:13-26).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from enum import IntEnum
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _capi as capi
+from ._capi import check
+
+
+class IntegratorType(IntEnum):
+    """integrator.h:22-32"""
+    None_ = 0
+    WavefrontDirectLighting = 1
+    WavefrontPathTracing = 2
+
+
+@dataclass
+class TriangleMesh:
+    vertices: np.ndarray                 # (n, 3) float32
+    triangles: np.ndarray                # (m, 3) uint32
+    normals: Optional[np.ndarray] = None
+    tangents: Optional[np.ndarray] = None
+    uvs: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        self.vertices = np.ascontiguousarray(self.vertices, np.float32).reshape(-1, 3)
+        self.triangles = np.ascontiguousarray(self.triangles, np.uint32).reshape(-1, 3)
+        for name in ("normals", "tangents"):
+            v = getattr(self, name)
+            if v is not None:
+                setattr(self, name, np.ascontiguousarray(v, np.float32).reshape(-1, 3))
+        if self.uvs is not None:
+            self.uvs = np.ascontiguousarray(self.uvs, np.float32).reshape(-1, 2)
+
+
+@dataclass
+class DiffuseMtl:
+    reflectance: Sequence[float]
+
+
+@dataclass
+class MetalMtl:
+    specularReflectance: Sequence[float]
+    roughness: float
+    eta: Sequence[float]
+    k: Sequence[float]
+
+
+@dataclass
+class GlassMtl:
+    roughness: float
+    interiorIOR: float
+
+
+@dataclass
+class Entity:
+    """kernel::Entity (libkernel/base/entity.h:24-31)"""
+    materialIndex: int
+
+
+@dataclass
+class ImageTexture2D:
+    """RGB(A) float32 image, (h, w, c); row 0 is v = 0."""
+    image: np.ndarray
+
+
+@dataclass
+class DirectionalEmitter:
+    colorMulIntensity: Sequence[float]
+    sunDirection: Sequence[float]
+    sunAngularRadius: float
+
+
+@dataclass
+class HDRIDome:
+    texture: ImageTexture2D
+
+
+@dataclass
+class Camera:
+    """core::Camera: position and the pixel-plane basis d00, ddu, ddv."""
+    m_camera_pos: Sequence[float]
+    m_camera_d00: Sequence[float]
+    m_camera_ddu: Sequence[float]
+    m_camera_ddv: Sequence[float]
+
+
+def _f3(v) -> C.Array:
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+class Integrator:
+    """Wavefront path-tracing integrator on one MI355X (gfx950)."""
+
+    @staticmethod
+    def createIntegrator(type: IntegratorType, width: int, height: int) -> "Integrator":
+        return Integrator(type, width, height)
+
+    def __init__(self, type: IntegratorType, width: int, height: int):
+        self._lib = capi.lib()
+        ctx = C.c_void_p()
+        check(self._lib.clhip_create(int(width), int(height), int(type), C.byref(ctx)))
+        self._ctx = ctx
+        self._type = IntegratorType(type)
+        self._meshes: list = []
+        self._keep: list = []  # host arrays referenced by the last BLAS build
+        self.spp_per_render = 1
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.clhip_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def getIntegratorType(self) -> IntegratorType:
+        return self._type
+
+    def _check(self, rc):
+        check(rc, self._ctx)
+
+    # -- build* (integrator.h:47-58)
+    def buildBLAS(self, trimeshes: Sequence[TriangleMesh]):
+        arr = (capi.clhip_mesh * max(1, len(trimeshes)))()
+        keep = []
+        for i, m in enumerate(trimeshes):
+            arr[i].vertices = capi.fptr(m.vertices)
+            arr[i].indices = capi.uptr(m.triangles)
+            arr[i].normals = capi.fptr(m.normals)
+            arr[i].tangents = capi.fptr(m.tangents)
+            arr[i].uvs = capi.fptr(m.uvs)
+            arr[i].num_vertices = m.vertices.shape[0]
+            arr[i].num_triangles = m.triangles.shape[0]
+            keep.append(m)
+        self._check(self._lib.clhip_build_blas(self._ctx, arr, len(trimeshes)))
+        self._meshes = list(trimeshes)
+
+    def buildTLAS(self, trimeshes: Sequence[TriangleMesh], instanceIDs: Sequence[int],
+                  transforms: Optional[np.ndarray] = None):
+        """One instance per (mesh, entity id); `transforms` (n, 3, 4)
+        object-to-world matrices extend the reference's identity instances."""
+        index = {id(m): i for i, m in enumerate(self._meshes)}
+        try:
+            mesh_of = np.array([index[id(m)] for m in trimeshes], np.uint32)
+        except KeyError as e:
+            raise ValueError("buildTLAS: mesh not passed to buildBLAS") from e
+        ids = np.ascontiguousarray(instanceIDs, np.uint32)
+        xf = None if transforms is None else np.ascontiguousarray(transforms, np.float32).reshape(-1, 12)
+        self._check(self._lib.clhip_build_tlas(self._ctx, capi.uptr(mesh_of), capi.uptr(ids),
+                                               capi.fptr(xf), len(mesh_of)))
+
+    def buildMaterials(self, materials):
+        arr = (capi.clhip_material * max(1, len(materials)))()
+        for i, m in enumerate(materials):
+            if isinstance(m, DiffuseMtl):
+                arr[i].type = capi.MTL_DIFFUSE
+                arr[i].reflectance = _f3(m.reflectance)
+            elif isinstance(m, MetalMtl):
+                arr[i].type = capi.MTL_METAL
+                arr[i].reflectance = _f3(m.specularReflectance)
+                arr[i].roughness = float(m.roughness)
+                arr[i].eta = _f3(m.eta)
+                arr[i].k = _f3(m.k)
+            elif isinstance(m, GlassMtl):
+                arr[i].type = capi.MTL_GLASS
+                arr[i].roughness = float(m.roughness)
+                arr[i].interior_ior = float(m.interiorIOR)
+            else:
+                raise TypeError(f"unknown material {type(m).__name__}")
+        self._check(self._lib.clhip_build_materials(self._ctx, arr, len(materials)))
+
+    def buildGeometryEntities(self, entities: Sequence[Entity]):
+        idx = np.array([e.materialIndex for e in entities], np.uint32)
+        self._check(self._lib.clhip_build_entities(self._ctx, capi.uptr(idx), len(idx)))
+
+    def _texture(self, tex: ImageTexture2D) -> int:
+        img = np.ascontiguousarray(tex.image, np.float32)
+        h, w, ch = img.shape
+        tid = C.c_int32()
+        self._check(self._lib.clhip_create_texture(self._ctx, capi.fptr(img), w, h, ch, C.byref(tid)))
+        return tid.value
+
+    def buildEmitters(self, emitters, domeEmitter=None, domeEmitterTextureResolution=None):
+        """`domeEmitter` is the HDRIDome object (also an element of
+        `emitters`) or None; the resolution is taken from its texture."""
+        arr = (capi.clhip_emitter * max(1, len(emitters)))()
+        dome_index = -1
+        for i, e in enumerate(emitters):
+            if isinstance(e, DirectionalEmitter):
+                arr[i].type = capi.EMITTER_DIRECTIONAL
+                arr[i].intensity = _f3(e.colorMulIntensity)
+                arr[i].direction = _f3(e.sunDirection)
+                arr[i].angular_radius = float(e.sunAngularRadius)
+                arr[i].texture = -1
+            elif isinstance(e, HDRIDome):
+                arr[i].type = capi.EMITTER_HDRIDOME
+                arr[i].texture = self._texture(e.texture)
+                if domeEmitter is not None and e is not domeEmitter:
+                    raise ValueError("only one HDRI dome is supported")
+                dome_index = i
+            else:
+                raise TypeError(f"unknown emitter {type(e).__name__}")
+        self._check(self._lib.clhip_build_emitters(self._ctx, arr, len(emitters), dome_index))
+
+    # -- camera / parameters
+    def updateCamera(self, camera: Camera):
+        self._check(self._lib.clhip_set_camera(self._ctx, _f3(camera.m_camera_pos), _f3(camera.m_camera_d00),
+                                               _f3(camera.m_camera_ddu), _f3(camera.m_camera_ddv)))
+
+    def setParams(self, max_depth: int = 5, rr_depth: int = 0, view: int = 0, samples_per_batch: int = 0):
+        p = capi.clhip_params(max_depth, rr_depth, view, samples_per_batch)
+        self._check(self._lib.clhip_set_params(self._ctx, C.byref(p)))
+
+    # -- render (integrator.h:61): one progressive sample per pixel
+    def render(self, spp: Optional[int] = None):
+        self._check(self._lib.clhip_render(self._ctx, int(self.spp_per_render if spp is None else spp)))
+
+    def sync(self):
+        self._check(self._lib.clhip_sync(self._ctx))
+
+    def resize(self, width: int, height: int):
+        self._check(self._lib.clhip_resize(self._ctx, int(width), int(height)))
+
+    def resetIterationIndex(self):
+        self._check(self._lib.clhip_reset(self._ctx))
+
+    def getFilmSize(self):
+        w, h, s = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self._lib.clhip_get_film_size(self._ctx, C.byref(w), C.byref(h), C.byref(s)))
+        return w.value, h.value
+
+    def iterationIndex(self) -> int:
+        w, h, s = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self._lib.clhip_get_film_size(self._ctx, C.byref(w), C.byref(h), C.byref(s)))
+        return s.value
+
+    def readbackFramebuffer(self) -> np.ndarray:
+        w, h = self.getFilmSize()
+        out = np.empty((h, w, 4), np.float32)
+        self._check(self._lib.clhip_readback(self._ctx, capi.fptr(out), out.nbytes))
+        return out
+
+    def readbackSums(self) -> np.ndarray:
+        w, h = self.getFilmSize()
+        out = np.empty((h, w, 3), np.float64)
+        self._check(self._lib.clhip_readback_sums(self._ctx, out.ctypes.data_as(C.POINTER(C.c_double)), out.nbytes))
+        return out
+
+    def counters(self) -> dict:
+        st = capi.clhip_stats()
+        self._check(self._lib.clhip_counters(self._ctx, C.byref(st)))
+        d = {k: np.array(getattr(st, k)[:], np.uint64) for k in ("rays", "hits", "shadow", "visible")}
+        for k in ("trace_nodes", "trace_tris", "shadow_nodes", "shadow_tris", "trace_launches", "shadow_launches",
+                  "trace_ms", "shade_ms", "shadow_ms", "render_ms"):
+            d[k] = getattr(st, k)
+        return d
+
+    def setTiming(self, on: bool = True):
+        self._check(self._lib.clhip_set_timing(self._ctx, 1 if on else 0))
+
+    # -- GL interop (integrator.h:66-71) has no counterpart without a display
+    def registerFramebuffer(self, glTexture):
+        raise NotImplementedError("GL interop is not available on MI355X; use readbackFramebuffer()")
+
+    unregisterFramebuffer = registerFramebuffer
+    mapFramebuffer = registerFramebuffer
+
+    # -- batch parity surfaces (same kernels)
+    @property
+    def ctx(self):
+        return self._ctx
+
+
+def upload_scene(integ: Integrator, desc) -> None:
+    """Feed a scenes.SceneDesc through the reference's build order
+    (RenderEngine::startRendering, renderengine.cpp:27-51)."""
+    integ.buildMaterials(desc.materials)
+    integ.buildGeometryEntities([Entity(m) for m in desc.instance_material])
+    integ.buildBLAS(desc.meshes)
+    integ.buildTLAS([desc.meshes[m] for m in desc.instance_mesh], list(range(len(desc.instance_mesh))),
+                    desc.instance_xform)
+    dome = next((e for e in desc.emitters if isinstance(e, HDRIDome)), None)
+    integ.buildEmitters(desc.emitters, dome)
+    integ.updateCamera(desc.camera)
+    integ.setParams(desc.max_depth, desc.rr_depth, desc.view, desc.samples_per_batch)

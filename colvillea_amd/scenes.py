@@ -1,0 +1,286 @@
+"""Seeded synthetic scenes and HDRIs for the benchmark configurations.
+
+Each generator returns a SceneDesc that both the renderer (through
+integrator.upload_scene) and the oracle consume, so the two see identical
+float32 inputs.  Conventions follow the reference: Y up, HDRI row 0 at the
+zenith, u = 0.5 looking down +Z (hdridome.h:455-476).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from .integrator import (Camera, DiffuseMtl, DirectionalEmitter, GlassMtl, HDRIDome, ImageTexture2D, MetalMtl,
+                         TriangleMesh)
+
+
+@dataclass
+class SceneDesc:
+    name: str
+    meshes: List[TriangleMesh]
+    instance_mesh: List[int]
+    instance_material: List[int]
+    instance_xform: np.ndarray  # (n, 3, 4) float32 object -> world
+    materials: list
+    emitters: list
+    camera: Camera
+    width: int
+    height: int
+    spp: int
+    max_depth: int
+    rr_depth: int
+    view: int = 0
+    samples_per_batch: int = 0
+    info: dict = field(default_factory=dict)
+
+    @property
+    def hdri(self) -> Optional[np.ndarray]:
+        for e in self.emitters:
+            if isinstance(e, HDRIDome):
+                return e.texture.image
+        return None
+
+    def num_triangles(self) -> int:
+        return int(sum(self.meshes[m].triangles.shape[0] for m in self.instance_mesh))
+
+
+# ------------------------------------------------------------------ camera
+
+def look_at(eye, at, up, fov_deg: float, width: int, height: int, reference_cosfovy: bool = False) -> Camera:
+    """Pinhole camera basis.  The default maps the vertical field of view
+    with 2 tan(fov/2); reference_cosfovy=True reproduces CLViewer's basis
+    (src/colvillea/CLViewer.cpp:244-260, Camera.h:60), which scales by
+    cos(fovy) instead."""
+    eye = np.asarray(eye, np.float64)
+    fwd = np.asarray(at, np.float64) - eye
+    fwd /= np.linalg.norm(fwd)
+    aspect = width / float(height)
+    s = math.cos(math.radians(fov_deg)) if reference_cosfovy else 2.0 * math.tan(math.radians(fov_deg) * 0.5)
+    right = np.cross(fwd, np.asarray(up, np.float64))
+    right /= np.linalg.norm(right)
+    ddu = s * aspect * right
+    upv = np.cross(ddu, fwd)
+    upv /= np.linalg.norm(upv)
+    ddv = s * upv
+    d00 = fwd - 0.5 * ddu - 0.5 * ddv
+    f = lambda v: np.asarray(v, np.float32)
+    return Camera(f(eye), f(d00), f(ddu), f(ddv))
+
+
+# ------------------------------------------------------------------ meshes
+
+def icosphere(subdiv: int):
+    """Unit icosphere: 20 * 4**subdiv triangles, normals = positions."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+                  (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], np.float64)
+    f = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+                  (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+                  (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)], np.int64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(subdiv):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = v.shape[0]
+        v = np.concatenate([v, mid])
+        nf = f.shape[0]
+        ab, bc, ca = inv[:nf] + base, inv[nf:2 * nf] + base, inv[2 * nf:] + base
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1), np.stack([c, ca, bc], 1),
+                            np.stack([ab, bc, ca], 1)])
+    v32 = v.astype(np.float32)
+    return TriangleMesh(v32, f.astype(np.uint32), normals=v32.copy())
+
+
+def quad_y0(half: float) -> TriangleMesh:
+    v = np.array([(-half, 0, -half), (half, 0, -half), (half, 0, half), (-half, 0, half)], np.float32)
+    n = np.tile(np.array([[0, 1, 0]], np.float32), (4, 1))
+    return TriangleMesh(v, np.array([(0, 2, 1), (0, 3, 2)], np.uint32), normals=n)
+
+
+def value_noise_fbm(x: np.ndarray, z: np.ndarray, rng: np.random.Generator, octaves: int = 4) -> np.ndarray:
+    """Fractional Brownian motion of smoothed value noise (lattice values
+    drawn from `rng`), amplitude halving and frequency doubling per octave."""
+    out = np.zeros_like(x, dtype=np.float64)
+    amp, freq, norm = 1.0, 1.0, 0.0
+    for _ in range(octaves):
+        xs, zs = x * freq, z * freq
+        x0, z0 = np.floor(xs), np.floor(zs)
+        ox, oz = x0.min(), z0.min()
+        nx, nz = int(x0.max() - ox) + 2, int(z0.max() - oz) + 2
+        lat = rng.uniform(-1.0, 1.0, size=(nz, nx))
+        ix, iz = (x0 - ox).astype(np.int64), (z0 - oz).astype(np.int64)
+        fx, fz = xs - x0, zs - z0
+        sx, sz = fx * fx * (3 - 2 * fx), fz * fz * (3 - 2 * fz)
+        a = lat[iz, ix] * (1 - sx) + lat[iz, ix + 1] * sx
+        b = lat[iz + 1, ix] * (1 - sx) + lat[iz + 1, ix + 1] * sx
+        out += amp * (a * (1 - sz) + b * sz)
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return out / norm
+
+
+def fbm_grid(nx: int, nz: int, half_x: float, half_z: float, height: float, scale: float,
+             rng: np.random.Generator) -> TriangleMesh:
+    xs = np.linspace(-half_x, half_x, nx)
+    zs = np.linspace(-half_z, half_z, nz)
+    X, Z = np.meshgrid(xs, zs)
+    H = height * value_noise_fbm(X / scale, Z / scale, rng)
+    dx, dz = xs[1] - xs[0], zs[1] - zs[0]
+    gx = np.gradient(H, dx, axis=1)
+    gz = np.gradient(H, dz, axis=0)
+    n = np.stack([-gx, np.ones_like(H), -gz], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    v = np.stack([X, H, Z], -1).reshape(-1, 3).astype(np.float32)
+    i = np.arange(nx * nz, dtype=np.int64).reshape(nz, nx)
+    a, b, c, d = i[:-1, :-1], i[:-1, 1:], i[1:, 1:], i[1:, :-1]
+    tris = np.concatenate([np.stack([a, c, b], -1).reshape(-1, 3), np.stack([a, d, c], -1).reshape(-1, 3)])
+    return TriangleMesh(v, tris.astype(np.uint32), normals=n.reshape(-1, 3).astype(np.float32)), H
+
+
+def translate_scale(center, radius) -> np.ndarray:
+    m = np.zeros((3, 4), np.float32)
+    m[0, 0] = m[1, 1] = m[2, 2] = radius
+    m[:, 3] = center
+    return m
+
+
+IDENTITY = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]], np.float32)
+
+# Complex IOR (eta, k) at roughly 650/550/450 nm for common conductors
+# (published optical constants: Au, Ag, Cu, Al, Cr).
+CONDUCTOR_TABLE = [
+    ((0.143, 0.374, 1.442), (3.983, 2.385, 1.603)),
+    ((0.155, 0.117, 0.138), (4.828, 3.122, 2.147)),
+    ((0.200, 0.924, 1.102), (3.912, 2.452, 2.142)),
+    ((1.657, 0.880, 0.521), (9.224, 6.270, 4.837)),
+    ((3.105, 3.183, 2.312), (3.337, 3.331, 3.137)),
+]
+
+
+# ------------------------------------------------------------------ HDRIs
+
+def hdri_directions(w: int, h: int) -> np.ndarray:
+    """Direction of every texel centre under the dome mapping
+    (HDRIDome::sampleDirect, hdridome.h:207-222)."""
+    u = (np.arange(w) + 0.5) / w
+    v = (np.arange(h) + 0.5) / h
+    U, V = np.meshgrid(u, v)
+    theta, phi = np.pi * V, -2.0 * np.pi * (U - 0.5)
+    return np.stack([np.sin(theta) * np.sin(phi), np.cos(theta), np.sin(theta) * np.cos(phi)], -1)
+
+
+def sky_hdri(w: int, h: int, sun_dir_to: np.ndarray, sun_peak: float = 40.0, sun_sigma_deg: float = 2.0) -> np.ndarray:
+    """Sky gradient plus a Gaussian sun lobe (RGBA32F)."""
+    d = hdri_directions(w, h)
+    y = d[..., 1:2]
+    zenith = np.array([0.25, 0.45, 1.0]) * 0.8
+    horizon = np.array([1.0, 0.95, 0.85]) * 0.9
+    ground = np.array([0.18, 0.16, 0.14]) * 0.6
+    t = np.sqrt(np.clip(y, 0, 1))
+    sky = horizon * (1 - t) + zenith * t
+    col = np.where(y >= 0, sky, ground * (1 + y))
+    s = np.asarray(sun_dir_to, np.float64)
+    s = s / np.linalg.norm(s)
+    ang = np.arccos(np.clip(d @ s, -1, 1))
+    sig = math.radians(sun_sigma_deg)
+    col = col + sun_peak * np.exp(-0.5 * (ang / sig) ** 2)[..., None] * np.array([1.0, 0.9, 0.75])
+    out = np.ones((h, w, 4), np.float32)
+    out[..., :3] = col
+    return out
+
+
+def lognormal_hdri(w: int, h: int, sigma: float, rng: np.random.Generator, hot_factor: float = 1e4) -> np.ndarray:
+    """Smooth log-normal luminance (std `sigma` in log space) with a colour
+    tint, plus one hot spot whose peak is `hot_factor` x the mean."""
+    cw, ch = 32, 16
+    g = rng.standard_normal((ch + 1, cw + 1))
+    g[:, -1] = g[:, 0]  # wrap in u
+    ys, xs = np.linspace(0, ch, h), np.linspace(0, cw, w)
+    y0 = np.minimum(np.floor(ys).astype(int), ch - 1)
+    x0 = np.minimum(np.floor(xs).astype(int), cw - 1)
+    fy, fx = (ys - y0)[:, None], (xs - x0)[None, :]
+    f = (g[y0][:, x0] * (1 - fx) * (1 - fy) + g[y0][:, x0 + 1] * fx * (1 - fy) + g[y0 + 1][:, x0] * (1 - fx) * fy
+         + g[y0 + 1][:, x0 + 1] * fx * fy)
+    f = (f - f.mean()) / f.std()
+    lum = np.exp(sigma * f)
+    tint = 0.75 + 0.25 * rng.uniform(size=(ch + 1, cw + 1, 3))
+    tint = tint[y0][:, x0]
+    col = lum[..., None] * tint
+    mean = col.mean()
+    d = hdri_directions(w, h)
+    th, ph = rng.uniform(0.25 * np.pi, 0.45 * np.pi), rng.uniform(-np.pi, np.pi)
+    hot = np.array([np.sin(th) * np.sin(ph), np.cos(th), np.sin(th) * np.cos(ph)])
+    ang = np.arccos(np.clip(d @ hot, -1, 1))
+    sig = 2.0 * np.pi / w * 2.0
+    col = col + hot_factor * mean * np.exp(-0.5 * (ang / sig) ** 2)[..., None]
+    out = np.ones((h, w, 4), np.float32)
+    out[..., :3] = col
+    return out
+
+
+# ------------------------------------------------------------------ configs
+
+def make_c0(seed: int = 1, width: int = 512, height: int = 512, spp: int = 16, hdri_w: int = 512,
+            hdri_h: int = 256) -> SceneDesc:
+    """8x8 ground quad + 64 icospheres (subdivision 3, 1280 triangles each),
+    diffuse albedo U[0.2, 0.8], sun of 0.5 deg radius and radiance 5, and a
+    procedural sky HDRI.  512x512, 16 spp, depth 5, Russian roulette from 3."""
+    rng = np.random.default_rng(seed)
+    ground, sphere = quad_y0(4.0), icosphere(3)
+    centres = np.stack([rng.uniform(-4, 4, 64), rng.uniform(0.5, 1.5, 64), rng.uniform(-4, 4, 64)], 1)
+    radii = rng.uniform(0.2, 0.6, 64)
+    albedo = rng.uniform(0.2, 0.8, (64, 3))
+    materials = [DiffuseMtl((0.5, 0.5, 0.5))] + [DiffuseMtl(tuple(a)) for a in albedo]
+    xf = np.stack([IDENTITY] + [translate_scale(c, r) for c, r in zip(centres, radii)])
+    el, az = math.radians(50.0), math.radians(30.0)
+    to_sun = np.array([math.cos(el) * math.sin(az), math.sin(el), math.cos(el) * math.cos(az)])
+    sun = DirectionalEmitter((5.0, 5.0, 5.0), tuple(-to_sun), math.radians(0.5))
+    dome = HDRIDome(ImageTexture2D(sky_hdri(hdri_w, hdri_h, to_sun)))
+    cam = look_at((0.0, 5.5, 11.0), (0.0, 0.5, 0.0), (0, 1, 0), 45.0, width, height)
+    return SceneDesc("C0", [ground, sphere], [0] + [1] * 64, list(range(65)), xf.astype(np.float32), materials,
+                     [sun, dome], cam, width, height, spp, 5, 3,
+                     info={"seed": seed, "triangles": 2 + 64 * 1280})
+
+
+def make_c1(seed: int = 2, width: int = 1920, height: int = 1080, spp: int = 64, grid=(1024, 512),
+            n_spheres: int = 200, sphere_subdiv: int = 4, hdri_w: int = 2048, hdri_h: int = 1024) -> SceneDesc:
+    """fBm-displaced grid (4 octaves) + instanced icospheres (subdivision 4),
+    materials 1/3 diffuse / GGX conductor (alpha U[0.05, 0.5], complex IOR
+    from a seeded table) / GGX dielectric (IOR 1.5, alpha U[0.01, 0.3]);
+    log-normal HDRI (sigma 1.5) with a hot spot at 1e4 x the mean.
+    1920x1080, 64 spp, depth 8."""
+    rng = np.random.default_rng(seed)
+    grid_mesh, H = fbm_grid(grid[0], grid[1], 16.0, 8.0, 1.2, 4.0, rng)
+    sphere = icosphere(sphere_subdiv)
+    cx = rng.uniform(-14, 14, n_spheres)
+    cz = rng.uniform(-7, 7, n_spheres)
+    r = rng.uniform(0.25, 0.9, n_spheres)
+    gi = np.clip(((cx + 16.0) / 32.0 * (grid[0] - 1)).round().astype(int), 0, grid[0] - 1)
+    gj = np.clip(((cz + 8.0) / 16.0 * (grid[1] - 1)).round().astype(int), 0, grid[1] - 1)
+    cy = H[gj, gi] + r * rng.uniform(0.3, 1.2, n_spheres)
+    materials = [DiffuseMtl((0.45, 0.42, 0.38))]
+    for i in range(n_spheres):
+        kind = i % 3
+        if kind == 0:
+            materials.append(DiffuseMtl(tuple(rng.uniform(0.2, 0.8, 3))))
+        elif kind == 1:
+            eta, k = CONDUCTOR_TABLE[int(rng.integers(len(CONDUCTOR_TABLE)))]
+            materials.append(MetalMtl((1.0, 1.0, 1.0), float(np.sqrt(rng.uniform(0.05, 0.5))), eta, k))
+        else:
+            materials.append(GlassMtl(float(np.sqrt(rng.uniform(0.01, 0.3))), 1.5))
+    xf = np.stack([IDENTITY] + [translate_scale((x, y, z), rr) for x, y, z, rr in zip(cx, cy, cz, r)])
+    dome = HDRIDome(ImageTexture2D(lognormal_hdri(hdri_w, hdri_h, 1.5, rng)))
+    cam = look_at((0.0, 7.0, 20.0), (0.0, 0.5, 0.0), (0, 1, 0), 45.0, width, height)
+    ntri = grid_mesh.triangles.shape[0] + n_spheres * sphere.triangles.shape[0]
+    return SceneDesc("C1", [grid_mesh, sphere], [0] + [1] * n_spheres, list(range(n_spheres + 1)),
+                     xf.astype(np.float32), materials, [dome], cam, width, height, spp, 8, 0,
+                     info={"seed": seed, "triangles": int(ntri)})

@@ -1,0 +1,242 @@
+/*
+ * clhip.h -- C ABI of libclpt_hip.so, the MI355X (gfx950) wavefront path
+ * tracer that replaces Colvillea's rendering path.
+ *
+ * Drop-in boundary: core::Integrator (include/colvillea/librender/
+ * integrator.h:34-103), created by Integrator::createIntegrator
+ * (src/core/librender/integrator.cpp:10-22) and driven by RenderEngine
+ * (src/core/librender/renderengine.cpp:27-121).  Each entry point below names
+ * the reference method it replaces.  The scene-side inputs are plain POD
+ * mirrors of the reference's kernel value types (kernel::Material,
+ * kernel::Emitter, kernel::Entity, TriangleMesh) so that a reference Scene
+ * can feed them unchanged.
+ *
+ * Conventions:
+ *   - every function returns 0 on success or a negative CLHIP_E_* code and
+ *     records a message readable with clhip_last_error(); nothing exits the
+ *     process (the reference aborts on CUDA errors, cudacommon.h:13-26);
+ *   - inputs are host pointers, copied before the call returns; the context
+ *     owns every device allocation;
+ *   - one context = one HIP stream; a context is not thread-safe (the
+ *     reference is single-threaded as well).
+ */
+#ifndef CLHIP_H
+#define CLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct clhip_ctx clhip_ctx;
+
+enum
+{
+    CLHIP_OK            = 0,
+    CLHIP_E_INVALID     = -1, /* bad argument */
+    CLHIP_E_HIP         = -2, /* HIP runtime error (no device, out of memory, ...) */
+    CLHIP_E_STATE       = -3, /* call order: e.g. render before build */
+    CLHIP_E_UNSUPPORTED = -4
+};
+
+/* IntegratorType (integrator.h:22-32) */
+enum
+{
+    CLHIP_INTEGRATOR_NONE            = 0,
+    CLHIP_INTEGRATOR_DIRECT_LIGHTING = 1,
+    CLHIP_INTEGRATOR_PATH_TRACING    = 2
+};
+
+/* TriangleMesh (include/colvillea/nodes/shapes/trianglemesh.h) and the
+ * device record TriMesh (libkernel/shapes/trimesh.h:16-32). */
+typedef struct
+{
+    const float*    vertices; /* 3 * num_vertices */
+    const uint32_t* indices;  /* 3 * num_triangles */
+    const float*    normals;  /* 3 * num_vertices, or NULL */
+    const float*    tangents; /* 3 * num_vertices, or NULL */
+    const float*    uvs;      /* 2 * num_vertices, or NULL */
+    uint32_t        num_vertices;
+    uint32_t        num_triangles;
+} clhip_mesh;
+
+/* MaterialType (libkernel/base/material.h:24-37) */
+enum
+{
+    CLHIP_MTL_DIFFUSE = 0,
+    CLHIP_MTL_METAL   = 1,
+    CLHIP_MTL_GLASS   = 2
+};
+
+/* kernel::Material payloads: DiffuseMtl (materials/diffusemtl.h:13-17),
+ * MetalMtl (metal.h:93-113), GlassMtl (glass.h:53-56).  GGX alpha is
+ * roughness^2 as in the reference (metal.h:131, glass.h:66). */
+typedef struct
+{
+    uint32_t type;
+    float    reflectance[3]; /* diffuse reflectance, or metal specular reflectance */
+    float    roughness;      /* metal, glass */
+    float    eta[3];         /* metal */
+    float    k[3];           /* metal */
+    float    interior_ior;   /* glass (exterior IOR is 1) */
+} clhip_material;
+
+/* EmitterType (libkernel/base/emitter.h:16-26) */
+enum
+{
+    CLHIP_EMITTER_DIRECTIONAL = 0,
+    CLHIP_EMITTER_HDRIDOME    = 1
+};
+
+/* DirectionalEmitter (emitters/directional.h:225-228); HDRIDome takes a
+ * texture created with clhip_create_texture (hdridome.h:163-175). */
+typedef struct
+{
+    uint32_t type;
+    float    intensity[3];   /* directional: colour * intensity (radiance) */
+    float    direction[3];   /* directional: emitted direction (sun -> scene) */
+    float    angular_radius; /* directional: radians */
+    int32_t  texture;        /* HDRI dome: texture id */
+} clhip_emitter;
+
+/* Per-render parameters (the reference hard-codes depth 5, path.cpp:135). */
+typedef struct
+{
+    uint32_t max_depth;        /* number of path segments; NEE at vertices 1..max_depth-1 */
+    uint32_t rr_depth;         /* Russian roulette from this vertex depth (0 = off) */
+    uint32_t view;             /* RNG view lane: z = (view << 20) | sample index */
+    uint32_t samples_per_batch;/* samples of every pixel in flight together (0 = auto) */
+} clhip_params;
+
+/* Exact counters of the last clhip_render (sum over its samples).  Index b
+ * is the path depth (1..max_depth). */
+#define CLHIP_MAX_DEPTH 32
+typedef struct
+{
+    uint64_t rays[CLHIP_MAX_DEPTH + 1];    /* closest-hit rays traced at depth b */
+    uint64_t hits[CLHIP_MAX_DEPTH + 1];    /* of which hit geometry */
+    uint64_t shadow[CLHIP_MAX_DEPTH + 1];  /* shadow rays spawned at vertex depth b */
+    uint64_t visible[CLHIP_MAX_DEPTH + 1]; /* of which unoccluded */
+    uint64_t trace_nodes;                  /* BVH4 nodes fetched by closest-hit traversal */
+    uint64_t trace_tris;                   /* triangles tested by closest-hit traversal */
+    uint64_t shadow_nodes;                 /* BVH4 nodes fetched by any-hit traversal */
+    uint64_t shadow_tris;                  /* triangles tested by any-hit traversal */
+    uint64_t trace_launches;               /* closest-hit kernel launches */
+    uint64_t shadow_launches;              /* any-hit kernel launches */
+    double   trace_ms;                     /* summed closest-hit kernel time (events; when timing is on) */
+    double   shade_ms;                     /* summed shading kernel time */
+    double   shadow_ms;                    /* summed any-hit kernel time */
+    double   render_ms;                    /* wall time of the last clhip_render on the stream */
+} clhip_stats;
+
+/* ---- context: Integrator::createIntegrator (integrator.cpp:10-22) and
+ *      WavefrontPathTracingIntegrator ctor/dtor (path.cpp:19-40) ---- */
+int         clhip_create(uint32_t width, uint32_t height, uint32_t integrator_type, clhip_ctx** out);
+int         clhip_destroy(clhip_ctx* ctx);
+const char* clhip_last_error(const clhip_ctx* ctx); /* ctx may be NULL: last global error */
+
+/* core::ImageTexture2D ctor (src/core/nodes/textures/imagetex2d.cpp:9-69):
+ * RGB or RGBA float32 data, stored as RGBA32F, wrap + bilinear. */
+int clhip_create_texture(clhip_ctx* ctx, const float* data, uint32_t width, uint32_t height, uint32_t channels,
+                         int32_t* texture_id);
+
+/* Integrator::buildBLAS (integrator.h:47; OptiXDevice::buildOptiXAccelBLASes,
+ * optixdevice.cpp:136-192): one BVH4 per mesh. */
+int clhip_build_blas(clhip_ctx* ctx, const clhip_mesh* meshes, uint32_t num_meshes);
+
+/* Integrator::buildTLAS (integrator.h:49-50; optixdevice.cpp:194-233): one
+ * instance per entity.  instance_ids[i] is the entity index of instance i
+ * (the reference passes entity indices, scene.cpp:232-236).  xforms is NULL
+ * (identity, as in the reference) or num_instances row-major 3x4
+ * object-to-world matrices (an extension). */
+int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uint32_t* instance_ids,
+                     const float* xforms, uint32_t num_instances);
+
+/* Integrator::buildMaterials (integrator.h:52; path.cpp:53-57) */
+int clhip_build_materials(clhip_ctx* ctx, const clhip_material* materials, uint32_t num_materials);
+
+/* Integrator::buildGeometryEntities (integrator.h:54; path.cpp:59-63):
+ * kernel::Entity{materialIndex} (libkernel/base/entity.h:24-31). */
+int clhip_build_entities(clhip_ctx* ctx, const uint32_t* material_index, uint32_t num_entities);
+
+/* Integrator::buildEmitters (integrator.h:56-58; path.cpp:65-82): copies the
+ * emitters and, when dome_index >= 0, builds the HDRI sampling tables on the
+ * GPU (float64 prefix sums; replaces hdridome.cu:15-51). */
+int clhip_build_emitters(clhip_ctx* ctx, const clhip_emitter* emitters, uint32_t num_emitters, int32_t dome_index);
+
+/* Integrator::updateCamera (integrator.h:80; core::Camera, librender/
+ * nodebase/camera.h:8-27).  Resets the progressive accumulation. */
+int clhip_set_camera(clhip_ctx* ctx, const float pos[3], const float d00[3], const float ddu[3], const float ddv[3]);
+int clhip_set_params(clhip_ctx* ctx, const clhip_params* params);
+
+/* Integrator::render (integrator.h:61; path.cpp:84-231): adds `spp`
+ * progressive samples per pixel.  Asynchronous on the context stream. */
+int clhip_render(clhip_ctx* ctx, uint32_t spp);
+int clhip_sync(clhip_ctx* ctx);
+
+/* Integrator::resize / resetIterationIndex / getFilmSize
+ * (integrator.h:64,75,77; path.cpp:233-250) */
+int clhip_resize(clhip_ctx* ctx, uint32_t width, uint32_t height);
+int clhip_reset(clhip_ctx* ctx);
+int clhip_get_film_size(const clhip_ctx* ctx, uint32_t* width, uint32_t* height, uint32_t* spp);
+
+/* Integrator::readbackFramebuffer (integrator.h:73; path.cpp:271-279):
+ * linear HDR RGBA float32 = per-pixel float64 sum / spp (replaces the
+ * running mean and sRGB round trip, raygeneration.cu:75-81, 498-506). */
+int clhip_readback(clhip_ctx* ctx, float* rgba, size_t bytes);
+/* The raw float64 RGB sums (3 * width * height doubles). */
+int clhip_readback_sums(clhip_ctx* ctx, double* rgb, size_t bytes);
+
+/* Counters of the last render (queue counts, node/triangle fetches, kernel
+ * times).  timing != 0 records HIP events around every traversal and
+ * shading launch from the next render on. */
+int clhip_counters(clhip_ctx* ctx, clhip_stats* out);
+int clhip_set_timing(clhip_ctx* ctx, int timing);
+
+/* ---- batch entry points on the same kernels (parity surfaces) ---- */
+
+/* Sampler (libkernel/base/sampler.h:16-78): for each seed (px, py, frame, w0)
+ * emit ndraws successive next1D() draws (u32 hash word and float). */
+int clhip_rng_batch(clhip_ctx* ctx, const uint32_t* seeds4, uint32_t n, uint32_t ndraws, uint32_t* out_u32,
+                    float* out_f);
+
+/* generateCameraRays (raygeneration.cu:59-103) for `n` (pixel, sample)
+ * pairs of the current film/camera: out8 = o.xyz, 0, d.xyz, 0 per ray. */
+int clhip_camera_rays(clhip_ctx* ctx, const uint32_t* pixel_sample2, uint32_t n, float* out8);
+
+/* BSDF::eval/pdf/sample (libkernel/base/bsdf.h:48-96) for materials of the
+ * current material table: per item material index, local wi (eval), wo,
+ * u3 = (u.x, u.y, lobe draw).  Outputs: eval f3 + pdf at (wi, wo); sampled
+ * wi3, f3, pdf. */
+int clhip_bsdf_batch(clhip_ctx* ctx, const uint32_t* material, const float* wi3, const float* wo3, const float* u3,
+                     uint32_t n, float* eval_f3, float* eval_pdf, float* smp_wi3, float* smp_f3, float* smp_pdf);
+
+/* LightSampler::sampleEmitterDirect (emitter.h:168-202) with the current
+ * emitters: per u2 -> direction, selection-scaled value, pdf, emitter index. */
+int clhip_emitter_batch(clhip_ctx* ctx, const float* u2, uint32_t n, float* dir3, float* value3, float* pdf,
+                        uint32_t* which);
+
+/* HDRIDome::evalEnvironment + pdfDirect (hdridome.h:248-297). */
+int clhip_dome_eval_batch(clhip_ctx* ctx, const float* dir3, uint32_t n, float* le3, float* pdf);
+
+/* The HDRI tables built by clhip_build_emitters: (w+1)*h, (w+1)*h, h+1, h+1. */
+int clhip_dome_tables(clhip_ctx* ctx, float* pcond, float* cdfcond, float* pv, float* cdfv, uint32_t* w, uint32_t* h);
+
+/* Closest hit (optix.cu:38-75 + closest-hit/miss programs) for rays8 =
+ * o.xyz, tmin, d.xyz, tmax.  Miss: t = -1, inst = prim = 0xffffffff. */
+int clhip_trace_batch(clhip_ctx* ctx, const float* rays8, uint32_t n, float* t, float* b12, uint32_t* inst,
+                      uint32_t* prim);
+
+/* Any hit (optix.cu:77-110 with terminate-on-first-hit). */
+int clhip_occlusion_batch(clhip_ctx* ctx, const float* rays8, uint32_t n, uint8_t* occluded);
+
+/* World-to-object matrices the context computed for its instances
+ * (num_instances * 12 floats). */
+int clhip_instance_inverses(clhip_ctx* ctx, float* out12, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
