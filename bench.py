@@ -28,27 +28,34 @@ def log(*a):
 
 
 def cpu_baseline(desc, seconds_target: float, threads: int):
-    """Time the float32 oracle on a bounded sample of the same workload: the
-    full-width rows [y0, y1) at 1 spp, chosen to take ~seconds_target."""
+    """Time the float32 oracle on a bounded sample of the same workload:
+    full-width rows spread evenly over the frame (a stratified sample of the
+    image, so sky and geometry are in proportion), 1 spp, sized to take about
+    `seconds_target` seconds."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import numpy as np
     import orc
 
     sc = orc.Scene(desc)
     w, h = desc.width, desc.height
-    y0 = h // 2
-    rows = 4
+
+    def rows_for(n):
+        return np.floor((np.arange(n) + 0.5) * h / n).astype(np.uint32)
+
+    n = 32
     t = time.perf_counter()
-    sc.render("f32", rows=(y0, y0 + rows), samples=(0, 1), nthreads=threads)
+    sc.render_rows(rows_for(n), "f32", samples=(0, 1), nthreads=threads)
     dt = time.perf_counter() - t
-    rate = rows * w / max(dt, 1e-9)
-    rows = int(max(4, min(h - y0, seconds_target * rate / w)))
+    n = int(max(32, min(h, seconds_target * n / max(dt, 1e-9))))
+    rows = rows_for(n)
     t = time.perf_counter()
-    sc.render("f32", rows=(y0, y0 + rows), samples=(0, 1), nthreads=threads)
+    _, cnt = sc.render_rows(rows, "f32", samples=(0, 1), nthreads=threads)
     dt = time.perf_counter() - t
-    samples = rows * w
+    samples = n * w
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"{w}x{rows} pixels (rows {y0}..{y0 + rows}) x 1 spp of the same scene, depth "
-                      f"{desc.max_depth}, float32 oracle, {threads} threads, {dt:.1f} s"}
+            "sample": f"{n} of {h} rows spread evenly over the {w}x{h} frame x 1 spp ({samples} camera paths, "
+                      f"{int(cnt['rays'].sum())} closest-hit rays), depth {desc.max_depth}, float32 oracle, "
+                      f"{threads} threads, {dt:.1f} s"}
 
 
 def main():

@@ -57,7 +57,20 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    build_host_example()
     return LIB
+
+
+HOST_EXAMPLE = os.path.join(HERE, "host", "render_example")
+
+
+def build_host_example() -> str:
+    """The C++ host example over host/hip_integrator.h (plain g++, linked
+    against the library with an rpath to colvillea_amd/lib)."""
+    src = os.path.join(HERE, "host", "render_example.cpp")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{REPO}", src, f"-L{OUT_DIR}", "-lclpt_hip",
+          "-Wl,-rpath,$ORIGIN/../lib", "-o", HOST_EXAMPLE])
+    return HOST_EXAMPLE
 
 
 if __name__ == "__main__":

@@ -126,6 +126,9 @@ float orc_uint_to_float(uint32_t x);
 float orc_draw(uint32_t px, uint32_t py, uint32_t zlane, uint32_t k);
 void  orc_rng_batch(const uint32_t* seeds4, uint32_t n, uint32_t ndraws, uint32_t* out_u32, float* out_f);
 
+/* ---- the renderer's elementary functions, for pinning against libm ---- */
+void orc_elem_batch(int fn, const float* x, const float* y, uint32_t n, float* out);
+
 /* ---- instance transform inverse (shared definition with the renderer) ---- */
 void orc_inverse3x4(const float m[12], float out[12], double outd[12]);
 
@@ -156,7 +159,9 @@ void       orc_scene_destroy(orc_scene* s);
                          uint32_t* inst, uint32_t* prim);                                                             \
     void orc_occluded_##SFX(const orc_scene* s, const float* rays8, uint32_t n, int brute, uint8_t* occ);            \
     void orc_render_##SFX(const orc_scene* s, const orc_render_params* p, uint32_t y0, uint32_t y1, uint32_t s0,     \
-                          uint32_t s1, uint32_t nthreads, double* sum3, orc_counts* counts);
+                          uint32_t s1, uint32_t nthreads, double* sum3, orc_counts* counts);                  \
+    void orc_render_rows_##SFX(const orc_scene* s, const orc_render_params* p, const uint32_t* rows, uint32_t nrows, \
+                               uint32_t s0, uint32_t s1, uint32_t nthreads, double* sum3, orc_counts* counts);
 
 ORC_DECLARE_VARIANT(f32)
 ORC_DECLARE_VARIANT(f64)

@@ -83,6 +83,7 @@ def lib():
         L.orc_draw.restype = C.c_float
         L.orc_rng_batch.argtypes = [u32p, C.c_uint32, C.c_uint32, u32p, f32p]
         L.orc_inverse3x4.argtypes = [f32p, f32p, dp]
+        L.orc_elem_batch.argtypes = [C.c_int, f32p, f32p, C.c_uint32, f32p]
         L.orc_dome_tables_ref32.argtypes = [f32p, C.c_uint32, C.c_uint32, f32p, f32p, f32p, f32p]
         L.orc_dome_tables_f64.argtypes = [f32p, C.c_uint32, C.c_uint32, f32p, f32p, f32p, f32p]
         L.orc_scene_create.argtypes = [C.POINTER(orc_scene_desc)]
@@ -102,6 +103,8 @@ def lib():
             g("occluded").argtypes = [P, f32p, C.c_uint32, C.c_int, u8p]
             g("render").argtypes = [P, C.POINTER(orc_render_params), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                     C.c_uint32, dp, C.POINTER(orc_counts)]
+            g("render_rows").argtypes = [P, C.POINTER(orc_render_params), u32p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_uint32, dp, C.POINTER(orc_counts)]
         _LIB = L
     return _LIB
 
@@ -246,6 +249,18 @@ class Scene:
         counts = {k: np.array(getattr(cnt, k)[:], np.uint64) for k in ("rays", "hits", "shadow", "visible")}
         return out, counts
 
+    def render_rows(self, rows, variant="f32", samples=(0, 1), nthreads=None, params=None):
+        """Float64 RGB sums of an explicit row list (nrows, width, 3)."""
+        p = params or self.params()
+        rows = np.ascontiguousarray(rows, np.uint32)
+        out = np.zeros((len(rows), p.width, 3), np.float64)
+        cnt = orc_counts()
+        nthreads = nthreads or min(16, os.cpu_count() or 1)
+        getattr(lib(), f"orc_render_rows_{variant}")(self.handle, C.byref(p), _up(rows), len(rows), samples[0],
+                                                      samples[1], nthreads, out.ctypes.data_as(dp), C.byref(cnt))
+        counts = {k: np.array(getattr(cnt, k)[:], np.uint64) for k in ("rays", "hits", "shadow", "visible")}
+        return out, counts
+
     def trace(self, rays8: np.ndarray, brute=False, variant="f32"):
         rays8 = np.ascontiguousarray(rays8, np.float32)
         n = rays8.shape[0]
@@ -361,6 +376,17 @@ def tex_eval(rgba, uv, variant="f32"):
     n = uv.shape[0]
     out = np.zeros((n, 4), np.float32)
     getattr(lib(), f"orc_tex_eval_{variant}")(_fp(rgba), rgba.shape[1], rgba.shape[0], _fp(uv), n, _fp(out))
+    return out
+
+
+ELEM = {"sin": 0, "cos": 1, "atan2": 2, "acos": 3, "asin": 4, "exp": 5, "log": 6, "tan": 7}
+
+
+def elem(fn: str, x, y=None):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(x if y is None else y, np.float32)
+    out = np.zeros_like(x)
+    lib().orc_elem_batch(ELEM[fn], _fp(x), _fp(y), x.size, _fp(out))
     return out
 
 

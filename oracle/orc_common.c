@@ -75,6 +75,29 @@ void orc_rng_batch(const uint32_t* seeds4, uint32_t n, uint32_t ndraws, uint32_t
     }
 }
 
+/* The renderer's float32 elementary functions (clpt_elem.h), exposed so
+ * the test suite can pin them against float64 libm: fn 0 sin, 1 cos,
+ * 2 atan2(x, y), 3 acos, 4 asin, 5 exp, 6 log, 7 tan. */
+void orc_elem_batch(int fn, const float* x, const float* y, uint32_t n, float* out)
+{
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        float r;
+        switch (fn)
+        {
+            case 0: r = clpt_sinf(x[i]); break;
+            case 1: r = clpt_cosf(x[i]); break;
+            case 2: r = clpt_atan2f(x[i], y[i]); break;
+            case 3: r = clpt_acosf(x[i]); break;
+            case 4: r = clpt_asinf(x[i]); break;
+            case 5: r = clpt_expf(x[i]); break;
+            case 6: r = clpt_logf(x[i]); break;
+            default: r = clpt_tanf(x[i]); break;
+        }
+        out[i] = r;
+    }
+}
+
 /* ------------------------------------------------------ transform inverse */
 
 /* Inverse of an affine 3x4 [A|t] by the adjugate, in float64, then rounded.

@@ -1192,11 +1192,13 @@ typedef struct
     double*                  sum3;
     orc_counts*              per_worker;
     uint32_t                 y0;
+    const uint32_t*          rows; /* NULL: row i is y0 + i */
 } SFX(render_ctx);
 
-static void SFX(render_row)(void* vctx, uint32_t y, uint32_t worker)
+static void SFX(render_row)(void* vctx, uint32_t i, uint32_t worker)
 {
     SFX(render_ctx)* c = (SFX(render_ctx)*)vctx;
+    const uint32_t   y = c->rows ? c->rows[i] : c->y0 + i;
     for (uint32_t x = 0; x < c->p->width; ++x)
     {
         double acc[3] = {0.0, 0.0, 0.0};
@@ -1208,7 +1210,7 @@ static void SFX(render_row)(void* vctx, uint32_t y, uint32_t worker)
             acc[1] += (double)L[1];
             acc[2] += (double)L[2];
         }
-        double* o = c->sum3 + 3 * ((size_t)(y - c->y0) * c->p->width + x);
+        double* o = c->sum3 + 3 * ((size_t)i * c->p->width + x);
         o[0] += acc[0];
         o[1] += acc[1];
         o[2] += acc[2];
@@ -1375,16 +1377,15 @@ void SFX(orc_occluded)(const orc_scene* s, const float* rays8, uint32_t n, int b
     }
 }
 
-/* Render rows [y0, y1) of the frame, samples [s0, s1); sum3 holds
- * (y1 - y0) * width RGB float64 sums (accumulated into). */
-void SFX(orc_render)(const orc_scene* s, const orc_render_params* p, uint32_t y0, uint32_t y1, uint32_t s0, uint32_t s1,
-                     uint32_t nthreads, double* sum3, orc_counts* counts)
+static void SFX(render_list)(const orc_scene* s, const orc_render_params* p, uint32_t y0, const uint32_t* rows,
+                             uint32_t nrows, uint32_t s0, uint32_t s1, uint32_t nthreads, double* sum3,
+                             orc_counts* counts)
 {
     if (nthreads == 0)
         nthreads = 1;
     orc_counts* pw = (orc_counts*)calloc(nthreads, sizeof(orc_counts));
-    SFX(render_ctx) c = {s, p, s0, s1, sum3, pw, y0};
-    orc_parallel_rows(nthreads, y0, y1, SFX(render_row), &c);
+    SFX(render_ctx) c = {s, p, s0, s1, sum3, pw, y0, rows};
+    orc_parallel_rows(nthreads, 0, nrows, SFX(render_row), &c);
     if (counts)
     {
         for (uint32_t t = 0; t < nthreads; ++t)
@@ -1397,4 +1398,20 @@ void SFX(orc_render)(const orc_scene* s, const orc_render_params* p, uint32_t y0
             }
     }
     free(pw);
+}
+
+/* Render rows [y0, y1) of the frame, samples [s0, s1); sum3 holds
+ * (y1 - y0) * width RGB float64 sums (accumulated into). */
+void SFX(orc_render)(const orc_scene* s, const orc_render_params* p, uint32_t y0, uint32_t y1, uint32_t s0, uint32_t s1,
+                     uint32_t nthreads, double* sum3, orc_counts* counts)
+{
+    SFX(render_list)(s, p, y0, NULL, y1 - y0, s0, s1, nthreads, sum3, counts);
+}
+
+/* Render an explicit list of rows (a stratified sample of the frame); sum3
+ * holds nrows * width RGB sums in list order. */
+void SFX(orc_render_rows)(const orc_scene* s, const orc_render_params* p, const uint32_t* rows, uint32_t nrows,
+                          uint32_t s0, uint32_t s1, uint32_t nthreads, double* sum3, orc_counts* counts)
+{
+    SFX(render_list)(s, p, 0, rows, nrows, s0, s1, nthreads, sum3, counts);
 }
