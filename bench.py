@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="size of the streaming-copy probe")
+    ap.add_argument("--copy-iters", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is described in "
@@ -125,6 +127,17 @@ def main():
                 "algorithmic_bytes_per_launch": int(per_launch),
                 "nodes_per_ray": round(nodes / max(1, rays_traced), 2),
                 "tris_per_ray": round(tris / max(1, rays_traced), 2)}
+    import ctypes as C
+
+    from colvillea_amd import _capi as capi
+
+    copy_ms = C.c_double()
+    copy_bytes = args.copy_mib << 20
+    capi.check(capi.lib().clhip_copy_probe(integ.ctx, copy_bytes, args.copy_iters, C.byref(copy_ms)), integ.ctx)
+    copy_gbs = 2 * copy_bytes / (copy_ms.value * 1e-3) / 1e9
+    roofline["copy_ceiling_GBs"] = round(copy_gbs, 1)
+    roofline["frac_of_copy_ceiling"] = round(achieved / copy_gbs, 4)
+    log(f"copy probe: {args.copy_mib} MiB in {copy_ms.value:.3f} ms -> {copy_gbs:.0f} GB/s (read + write)")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(desc, args.cpu_seconds, args.cpu_threads)

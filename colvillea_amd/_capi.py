@@ -128,6 +128,7 @@ SIGNATURES = {
     "clhip_trace_batch": (C.c_int, [P, f32p, u32, f32p, f32p, u32p, u32p]),
     "clhip_occlusion_batch": (C.c_int, [P, f32p, u32, C.POINTER(C.c_uint8)]),
     "clhip_instance_inverses": (C.c_int, [P, f32p, u32]),
+    "clhip_copy_probe": (C.c_int, [P, C.c_size_t, u32, C.POINTER(C.c_double)]),
 }
 
 

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -119,7 +120,7 @@ struct clhip_ctx
     DevBuf   d_q[2][3], d_hit, d_prim, d_sq[3], d_gstack, d_bc, d_tot, d_ctr;
     size_t   qcap = 0;
     uint32_t spp  = 0;
-    uint32_t blocks = 0;
+    uint32_t blocks = 0, blocks_trace = 0, blocks_shadow = 0, blocks_shade = 0;
 
     /* timing */
     bool                      timing = false;
@@ -269,8 +270,8 @@ int ensure_render_buffers(clhip_ctx* c, size_t paths)
         CLHIP_CHECK(c, c->d_sq[2].alloc(paths * sizeof(float2)));
         c->qcap = paths;
     }
-    const size_t gthreads = (size_t)c->blocks * 256;
-    CLHIP_CHECK(c, c->d_gstack.alloc(gthreads * 112 * sizeof(uint32_t)));
+    const size_t gthreads = (size_t)std::max(c->blocks, std::max(c->blocks_trace, c->blocks_shadow)) * 256;
+    CLHIP_CHECK(c, c->d_gstack.alloc(gthreads * CLPT_GSTACK_ENTRIES * sizeof(uint2)));
     if (!c->d_bc.p)
     {
         CLHIP_CHECK(c, c->d_bc.alloc(sizeof(BatchCounters)));
@@ -376,6 +377,22 @@ int clhip_create(uint32_t width, uint32_t height, uint32_t integrator_type, clhi
         cus = 256;
     c->num_cus = (uint32_t)cus;
     c->blocks  = c->num_cus * 8;
+    {
+        /* persistent grids: every lane refills its own rays, so at most the
+         * blocks that fit on the chip at once.  Measured on C1 (MI355X):
+         * closest hit is fastest at full residency, any hit at 3 blocks/CU
+         * (fewer lanes, more rays per lane, less cache pressure).  The
+         * environment variables override (used by the launch autotuner). */
+        auto env_u = [](const char* name) {
+            const char* v = std::getenv(name);
+            return v ? (uint32_t)std::atoi(v) : 0u;
+        };
+        const uint32_t rt = clpt::resident_blocks_per_cu(0), rs = clpt::resident_blocks_per_cu(1);
+        const uint32_t ot = env_u("CLPT_TRACE_BLOCKS_PER_CU"), os = env_u("CLPT_SHADOW_BLOCKS_PER_CU");
+        c->blocks_trace  = c->num_cus * (ot ? ot : rt);
+        c->blocks_shadow = c->num_cus * (os ? os : std::min(rs, 3u));
+        c->blocks_shade  = c->num_cus * 8;
+    }
     e          = hipStreamCreate(&c->stream);
     if (e != hipSuccess)
         return fail(nullptr, CLHIP_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
@@ -851,7 +868,7 @@ int clhip_render(clhip_ctx* ctx, uint32_t spp)
     ShadowQueue sq{ctx->d_sq[0].as<float4>(), ctx->d_sq[1].as<float4>(), ctx->d_sq[2].as<float2>(), nullptr};
     double*     accum = ctx->d_accum.as<double>();
     auto*       ctr   = ctx->d_ctr.as<unsigned long long>();
-    uint32_t*   gst   = ctx->d_gstack.as<uint32_t>();
+    uint2*      gst   = ctx->d_gstack.as<uint2>();
     const uint32_t blocks = ctx->blocks;
 
     for (uint32_t done = 0; done < spp; done += spb)
@@ -872,7 +889,7 @@ int clhip_render(clhip_ctx* ctx, uint32_t spp)
                 tl.kind = 0;
                 CLHIP_CHECK(ctx, hipEventRecord(tl.a, ctx->stream));
             }
-            CLHIP_CHECK(ctx, clpt::launch_trace(ctx->stream, sc, qin, &bc->rays[depth], depth, hb, gst, ctr, blocks));
+            CLHIP_CHECK(ctx, clpt::launch_trace(ctx->stream, sc, qin, &bc->rays[depth], depth, hb, gst, ctr, ctx->blocks_trace));
             ++ctx->trace_launches;
             if (ctx->timing)
             {
@@ -883,7 +900,7 @@ int clhip_render(clhip_ctx* ctx, uint32_t spp)
                 tl.kind = 1;
                 CLHIP_CHECK(ctx, hipEventRecord(tl.a, ctx->stream));
             }
-            CLHIP_CHECK(ctx, clpt::launch_shade(ctx->stream, sc, fr, depth, sample_base, qin, hb, qout, sq, accum, bc, blocks));
+            CLHIP_CHECK(ctx, clpt::launch_shade(ctx->stream, sc, fr, depth, sample_base, qin, hb, qout, sq, accum, bc, ctx->blocks_shade));
             if (ctx->timing)
             {
                 CLHIP_CHECK(ctx, hipEventRecord(tl.b, ctx->stream));
@@ -898,7 +915,7 @@ int clhip_render(clhip_ctx* ctx, uint32_t spp)
                     tl.kind = 2;
                     CLHIP_CHECK(ctx, hipEventRecord(tl.a, ctx->stream));
                 }
-                CLHIP_CHECK(ctx, clpt::launch_shadow(ctx->stream, sc, sq, depth, accum, bc, gst, ctr, blocks));
+                CLHIP_CHECK(ctx, clpt::launch_shadow(ctx->stream, sc, sq, depth, accum, bc, gst, ctr, ctx->blocks_shadow));
                 ++ctx->shadow_launches;
                 if (ctx->timing)
                 {
@@ -1149,9 +1166,9 @@ static int trace_common(clhip_ctx* ctx, const float* rays8, uint32_t n, int any,
     CLHIP_CHECK(ctx, r.from_host(rays8, 8 * (size_t)n));
     CLHIP_CHECK(ctx, o.zeros(16 * (size_t)n));
     CLHIP_CHECK(ctx, p.zeros(4 * (size_t)n));
-    CLHIP_CHECK(ctx, g.zeros((size_t)blocks * 256 * 112 * 4));
+    CLHIP_CHECK(ctx, g.zeros((size_t)blocks * 256 * CLPT_GSTACK_ENTRIES * sizeof(uint2)));
     CLHIP_CHECK(ctx, clpt::launch_trace_batch(ctx->stream, sc, r.b.as<const float4>(), n, any, o.b.as<float4>(),
-                                              p.b.as<uint32_t>(), g.b.as<uint32_t>(), blocks));
+                                              p.b.as<uint32_t>(), g.b.as<uint2>(), blocks));
     CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     out.resize(n);
     prim.resize(n);
@@ -1193,6 +1210,34 @@ int clhip_occlusion_batch(clhip_ctx* ctx, const float* rays8, uint32_t n, uint8_
         return rc;
     for (uint32_t i = 0; i < n; ++i)
         occluded[i] = out[i].x != 0.0f ? 1u : 0u;
+    return CLHIP_OK;
+}
+
+int clhip_copy_probe(clhip_ctx* ctx, size_t bytes, uint32_t iters, double* ms_per_copy)
+{
+    if (!ctx || bytes < 16 || iters == 0)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_copy_probe: bad arguments");
+    const size_t n = bytes / 16;
+    DevBuf       a, b;
+    CLHIP_CHECK(ctx, a.alloc(n * 16));
+    CLHIP_CHECK(ctx, b.alloc(n * 16));
+    CLHIP_CHECK(ctx, hipMemsetAsync(a.p, 0, n * 16, ctx->stream));
+    hipEvent_t e0, e1;
+    CLHIP_CHECK(ctx, hipEventCreate(&e0));
+    CLHIP_CHECK(ctx, hipEventCreate(&e1));
+    const uint32_t blocks = ctx->num_cus * 16;
+    CLHIP_CHECK(ctx, clpt::launch_copy_probe(ctx->stream, a.as<const float4>(), b.as<float4>(), n, blocks));
+    CLHIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
+    for (uint32_t i = 0; i < iters; ++i)
+        CLHIP_CHECK(ctx, clpt::launch_copy_probe(ctx->stream, a.as<const float4>(), b.as<float4>(), n, blocks));
+    CLHIP_CHECK(ctx, hipEventRecord(e1, ctx->stream));
+    CLHIP_CHECK(ctx, hipEventSynchronize(e1));
+    float ms = 0.0f;
+    CLHIP_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_per_copy)
+        *ms_per_copy = (double)ms / iters;
     return CLHIP_OK;
 }
 

@@ -9,12 +9,12 @@ namespace clpt
 hipError_t launch_camera(hipStream_t s, const FrameView& fr, uint32_t sample_base, uint32_t npaths, const RayQueue& q,
                          BatchCounters* bc, uint32_t max_blocks);
 hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, const uint32_t* count, uint32_t depth,
-                        const HitBuf& hb, uint32_t* gstack, unsigned long long* ctr, uint32_t blocks);
+                        const HitBuf& hb, uint2* gstack, unsigned long long* ctr, uint32_t blocks);
 hipError_t launch_shade(hipStream_t s, const SceneView& sc, const FrameView& fr, uint32_t depth, uint32_t sample_base,
                         const RayQueue& qin, const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq,
                         double* accum, BatchCounters* bc, uint32_t blocks);
 hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& sq, uint32_t depth, double* accum,
-                         BatchCounters* bc, uint32_t* gstack, unsigned long long* ctr, uint32_t blocks);
+                         BatchCounters* bc, uint2* gstack, unsigned long long* ctr, uint32_t blocks);
 hipError_t launch_batch_end(hipStream_t s, const BatchCounters* bc, TotalCounters* tot);
 hipError_t launch_resolve(hipStream_t s, const double* accum, uint32_t npix, uint32_t spp, float4* out);
 hipError_t launch_dome_tables(hipStream_t s, const float4* tex, uint32_t w, uint32_t h, float* pcond, float* cdfcond,
@@ -28,8 +28,11 @@ hipError_t launch_emitter_batch(hipStream_t s, const SceneView& sc, const float*
                                 float* val3, float* pdf, uint32_t* which);
 hipError_t launch_dome_eval_batch(hipStream_t s, const SceneView& sc, const float* dir3, uint32_t n, float* le3,
                                   float* pdf);
+/* Blocks of 256 threads resident per CU for k_trace (0), k_shadow (1), k_shade (2). */
+uint32_t   resident_blocks_per_cu(int which);
+hipError_t launch_copy_probe(hipStream_t s, const float4* src, float4* dst, size_t n, uint32_t blocks);
 hipError_t launch_trace_batch(hipStream_t s, const SceneView& sc, const float4* rays, uint32_t n, int any, float4* out,
-                              uint32_t* prim, uint32_t* gstack, uint32_t blocks);
+                              uint32_t* prim, uint2* gstack, uint32_t blocks);
 } // namespace clpt
 
 #endif

@@ -31,6 +31,12 @@ static_assert(sizeof(Node4) == 128, "Node4 is two 64-byte halves");
 #define CLPT_TRI_LEAF(first, count) (CLPT_LEAF_BIT | ((uint32_t)((count)-1) << 27) | (uint32_t)(first))
 #define CLPT_INST_LEAF(idx) (CLPT_LEAF_BIT | CLPT_INST_BIT | (uint32_t)(idx))
 #define CLPT_MAX_LEAF_TRIS 4u
+
+/* Traversal stack: CLPT_LDS_STACK entries per lane in LDS, the rest (up to
+ * CLPT_STACK_TOTAL) in a global overflow area of (ref, t) pairs. */
+#define CLPT_LDS_STACK 12
+#define CLPT_STACK_TOTAL 128
+#define CLPT_GSTACK_ENTRIES (CLPT_STACK_TOTAL - CLPT_LDS_STACK)
 #define CLPT_MAX_TRIS (1u << 27)
 #define CLPT_MAX_INSTANCES 0x3ffffff0u
 

@@ -19,8 +19,6 @@
 #include "clpt_device.h"
 #include "clpt_kernels.h"
 
-#define CLPT_LDS_STACK 16
-#define CLPT_STACK_TOTAL 128
 
 /* ------------------------------------------------------------ wave utils */
 
@@ -64,32 +62,64 @@ __device__ __forceinline__ void accum_add(double* acc, uint32_t pixel, f3 c)
 
 /* ------------------------------------------------------------ traversal */
 
+/* Per-lane short stack of (child reference, entry distance): the first
+ * CLPT_LDS_STACK entries live in LDS laid out [entry][lane] (a wave touches
+ * 64 consecutive dwords per access), deeper entries in a global overflow area
+ * laid out [entry][global thread].  Entries whose entry distance is beyond the
+ * current closest hit are dropped when popped, without fetching them. */
 struct TraverseStack
 {
-    uint32_t* lds;     /* [CLPT_LDS_STACK][blockDim] */
-    uint32_t* gmem;    /* [CLPT_STACK_TOTAL - CLPT_LDS_STACK][gstride] */
+    uint32_t* lds_ref;
+    float*    lds_t;
+    uint2*    gmem;
     uint32_t  tid, gid, gstride;
     int       sp;
-    __device__ __forceinline__ void push(uint32_t v)
+    __device__ __forceinline__ void push(uint32_t ref, float t)
     {
         if (sp < CLPT_LDS_STACK)
-            lds[sp * blockDim.x + tid] = v;
+        {
+            lds_ref[sp * blockDim.x + tid] = ref;
+            lds_t[sp * blockDim.x + tid]   = t;
+        }
         else if (sp < CLPT_STACK_TOTAL)
-            gmem[(size_t)(sp - CLPT_LDS_STACK) * gstride + gid] = v;
-        /* deeper pushes are dropped; the builder bounds the tree depth so
-         * that CLPT_STACK_TOTAL is never reached */
+            gmem[(size_t)(sp - CLPT_LDS_STACK) * gstride + gid] = make_uint2(ref, __float_as_uint(t));
+        /* deeper pushes are dropped: clhip_build_tlas rejects trees whose
+         * depth could reach CLPT_STACK_TOTAL */
         ++sp;
     }
-    __device__ __forceinline__ uint32_t pop()
+    __device__ __forceinline__ void pop(uint32_t& ref, float& t)
     {
         --sp;
         if (sp < CLPT_LDS_STACK)
-            return lds[sp * blockDim.x + tid];
-        if (sp < CLPT_STACK_TOTAL)
-            return gmem[(size_t)(sp - CLPT_LDS_STACK) * gstride + gid];
-        return CLPT_CHILD_EMPTY;
+        {
+            ref = lds_ref[sp * blockDim.x + tid];
+            t   = lds_t[sp * blockDim.x + tid];
+        }
+        else if (sp < CLPT_STACK_TOTAL)
+        {
+            const uint2 e = gmem[(size_t)(sp - CLPT_LDS_STACK) * gstride + gid];
+            ref           = e.x;
+            t             = __uint_as_float(e.y);
+        }
+        else
+        {
+            ref = CLPT_CHILD_EMPTY;
+            t   = CLPT_FLT_MAX;
+        }
     }
 };
+
+#define CLPT_DECLARE_STACK(st, gstack)                                                                                 \
+    __shared__ uint32_t lds_ref_[CLPT_LDS_STACK * 256];                                                                \
+    __shared__ float    lds_t_[CLPT_LDS_STACK * 256];                                                                  \
+    TraverseStack       st;                                                                                            \
+    st.lds_ref = lds_ref_;                                                                                             \
+    st.lds_t   = lds_t_;                                                                                               \
+    st.gmem    = (gstack);                                                                                             \
+    st.tid     = threadIdx.x;                                                                                          \
+    st.gid     = blockIdx.x * blockDim.x + threadIdx.x;                                                                \
+    st.gstride = gridDim.x * blockDim.x;                                                                               \
+    st.sp      = 0
 
 struct HitRec
 {
@@ -97,118 +127,183 @@ struct HitRec
     uint32_t inst, prim;
 };
 
-/* Two-level BVH4 traversal; ANY = occlusion (terminate at the first hit,
- * D15: optix.cu:106 traces shadow rays with closest-hit flags). */
-template <bool ANY>
-__device__ __forceinline__ bool traverse(const SceneView& sc, const RayW& wr, float tmax, HitRec& best,
-                                         TraverseStack& st, uint32_t& nnodes, uint32_t& ntris)
+__device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb)
 {
-    RayW     r        = wr;
-    uint32_t cur_inst = 0xffffffffu;
-    best.t            = tmax;
-    best.inst         = 0xffffffffu;
-    best.prim         = 0xffffffffu;
-    best.b1 = best.b2 = 0.0f;
-    bool found        = false;
-    st.sp             = 0;
-    st.push(sc.tlas_root);
-    while (st.sp > 0)
+    const bool  sw = tb < ta;
+    const float t0 = sw ? tb : ta, t1 = sw ? ta : tb;
+    const uint32_t r0 = sw ? rb : ra, r1 = sw ? ra : rb;
+    ta = t0;
+    tb = t1;
+    ra = r0;
+    rb = r1;
+}
+
+/* Two-level BVH4 traversal as a per-lane state machine: step() does one
+ * unit of work (one node, or one triangle of a leaf) and reports when the
+ * ray is finished, so a lane whose ray ends can start its next ray at once
+ * instead of idling until the slowest lane of its wave is done.
+ * ANY = occlusion: stop at the first hit (D15: optix.cu:106 traces shadow
+ * rays with closest-hit flags).  The world ray is (wo, wd); inside an
+ * instance the object-space ray replaces it until the marker pushed on entry
+ * is popped. */
+template <bool ANY>
+struct Tracer
+{
+    f3       wo, wd;
+    float    tmin;
+    RayW     r;
+    uint32_t cur_inst;
+    HitRec   best;
+    uint32_t ref;
+    uint32_t leaf_next, leaf_left;
+    bool     found;
+
+    __device__ __forceinline__ void start(const SceneView& sc, TraverseStack& st, f3 o, f3 d, float t0, float t1)
     {
-        uint32_t ref = st.pop();
-        if (ref == CLPT_STACK_MARKER)
+        wo     = o;
+        wd     = d;
+        tmin   = t0;
+        r.o    = o;
+        r.d    = d;
+        r.tmin = t0;
+        ray_setup(r);
+        cur_inst  = 0xffffffffu;
+        best.t    = t1;
+        best.b1   = 0.0f;
+        best.b2   = 0.0f;
+        best.inst = 0xffffffffu;
+        best.prim = 0xffffffffu;
+        found     = false;
+        st.sp     = 0;
+        ref       = sc.tlas_root;
+        leaf_next = 0;
+        leaf_left = 0;
+    }
+
+    /* Returns true when the ray is finished. */
+    __device__ __forceinline__ bool step(const SceneView& sc, TraverseStack& st, uint32_t& nnodes, uint32_t& ntris)
+    {
+        bool need_pop = false;
+        if (leaf_left == 0)
         {
-            r        = wr;
-            cur_inst = 0xffffffffu;
-            continue;
-        }
-        if (ref == CLPT_CHILD_EMPTY)
-            continue;
-        if (ref & CLPT_LEAF_BIT)
-        {
-            cur_inst              = ref & ~(CLPT_LEAF_BIT | CLPT_INST_BIT);
-            const InstanceRec* ip = sc.inst + cur_inst;
-            float              m[12];
-#pragma unroll
-            for (int k = 0; k < 12; ++k)
-                m[k] = ip->inv[k];
-            ray_to_object(m, wr, r);
-            st.push(CLPT_STACK_MARKER);
-            ref = ip->blas_root;
-        }
-        const float4* np  = reinterpret_cast<const float4*>(sc.nodes + ref);
-        const float4  lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-        const uint4   ch  = *reinterpret_cast<const uint4*>(np + 6);
-        ++nnodes;
-        const float lo_x[4] = {lox.x, lox.y, lox.z, lox.w}, hi_x[4] = {hix.x, hix.y, hix.z, hix.w};
-        const float lo_y[4] = {loy.x, loy.y, loy.z, loy.w}, hi_y[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-        const float lo_z[4] = {loz.x, loz.y, loz.z, loz.w}, hi_z[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-        const uint32_t child[4] = {ch.x, ch.y, ch.z, ch.w};
-        float          tn[4];
-        uint32_t       cref[4];
-        int            ncand = 0;
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-        {
-            const float t0x = (lo_x[c] - r.o.x) * r.inv.x, t1x = (hi_x[c] - r.o.x) * r.inv.x;
-            const float t0y = (lo_y[c] - r.o.y) * r.inv.y, t1y = (hi_y[c] - r.o.y) * r.inv.y;
-            const float t0z = (lo_z[c] - r.o.z) * r.inv.z, t1z = (hi_z[c] - r.o.z) * r.inv.z;
-            float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
-            float tfar  = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best.t));
-            tnear       = tnear * 0.99999f;
-            tfar        = tfar * 1.00001f;
-            const bool hitc = child[c] != CLPT_CHILD_EMPTY && tnear <= tfar;
-            if (!hitc)
-                continue;
-            if ((child[c] & (CLPT_LEAF_BIT | CLPT_INST_BIT)) == CLPT_LEAF_BIT)
+            if (ref == CLPT_STACK_MARKER)
             {
-                const uint32_t first = child[c] & (CLPT_MAX_TRIS - 1u);
-                const uint32_t cnt   = ((child[c] >> 27) & 7u) + 1u;
-                for (uint32_t j = 0; j < cnt; ++j)
-                {
-                    const TriRec tr = sc.tris[first + j];
-                    ++ntris;
-                    float tt, b1, b2;
-                    if (tri_test(r, best.t, f3{tr.v0p.x, tr.v0p.y, tr.v0p.z}, f3{tr.v1.x, tr.v1.y, tr.v1.z},
-                                 f3{tr.v2.x, tr.v2.y, tr.v2.z}, &tt, &b1, &b2))
-                    {
-                        const uint32_t prim = __float_as_uint(tr.v0p.w);
-                        if (ANY)
-                            return true;
-                        if (hit_better(tt, cur_inst, prim, best.t, best.inst, best.prim))
-                        {
-                            best.t    = tt;
-                            best.b1   = b1;
-                            best.b2   = b2;
-                            best.inst = cur_inst;
-                            best.prim = prim;
-                            found     = true;
-                        }
-                    }
-                }
+                r.o    = wo;
+                r.d    = wd;
+                r.tmin = tmin;
+                ray_setup(r);
+                cur_inst = 0xffffffffu;
+                need_pop = true;
+            }
+            else if ((ref & (CLPT_LEAF_BIT | CLPT_INST_BIT)) == CLPT_LEAF_BIT)
+            {
+                leaf_next = ref & (CLPT_MAX_TRIS - 1u);
+                leaf_left = ((ref >> 27) & 7u) + 1u;
             }
             else
             {
-                tn[ncand]   = tnear;
-                cref[ncand] = child[c];
-                ++ncand;
+                if (ref & CLPT_LEAF_BIT)
+                {
+                    /* instance leaf: continue in the instance's BLAS */
+                    cur_inst              = ref & ~(CLPT_LEAF_BIT | CLPT_INST_BIT);
+                    const InstanceRec* ip = sc.inst + cur_inst;
+                    float              m[12];
+#pragma unroll
+                    for (int k = 0; k < 12; ++k)
+                        m[k] = ip->inv[k];
+                    RayW w;
+                    w.o    = wo;
+                    w.d    = wd;
+                    w.tmin = tmin;
+                    ray_to_object(m, w, r);
+                    st.push(CLPT_STACK_MARKER, -CLPT_FLT_MAX);
+                    ref = ip->blas_root;
+                }
+                const float4* np  = reinterpret_cast<const float4*>(sc.nodes + ref);
+                const float4  lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+                const uint4   ch  = *reinterpret_cast<const uint4*>(np + 6);
+                ++nnodes;
+                float       tn[4];
+                uint32_t    cr[4]   = {ch.x, ch.y, ch.z, ch.w};
+                const float lo_x[4] = {lox.x, lox.y, lox.z, lox.w}, hi_x[4] = {hix.x, hix.y, hix.z, hix.w};
+                const float lo_y[4] = {loy.x, loy.y, loy.z, loy.w}, hi_y[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+                const float lo_z[4] = {loz.x, loz.y, loz.z, loz.w}, hi_z[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                {
+                    const float t0x = (lo_x[c] - r.o.x) * r.inv.x, t1x = (hi_x[c] - r.o.x) * r.inv.x;
+                    const float t0y = (lo_y[c] - r.o.y) * r.inv.y, t1y = (hi_y[c] - r.o.y) * r.inv.y;
+                    const float t0z = (lo_z[c] - r.o.z) * r.inv.z, t1z = (hi_z[c] - r.o.z) * r.inv.z;
+                    float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+                    float tfar  = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best.t));
+                    tnear       = tnear * 0.99999f;
+                    tfar        = tfar * 1.00001f;
+                    tn[c]       = (cr[c] != CLPT_CHILD_EMPTY && tnear <= tfar) ? tnear : CLPT_FLT_MAX;
+                }
+                /* nearest first: a 5-comparator sorting network, fixed indices */
+                cswap(tn[0], cr[0], tn[1], cr[1]);
+                cswap(tn[2], cr[2], tn[3], cr[3]);
+                cswap(tn[0], cr[0], tn[2], cr[2]);
+                cswap(tn[1], cr[1], tn[3], cr[3]);
+                cswap(tn[1], cr[1], tn[2], cr[2]);
+                if (tn[3] < CLPT_FLT_MAX)
+                    st.push(cr[3], tn[3]);
+                if (tn[2] < CLPT_FLT_MAX)
+                    st.push(cr[2], tn[2]);
+                if (tn[1] < CLPT_FLT_MAX)
+                    st.push(cr[1], tn[1]);
+                if (tn[0] < CLPT_FLT_MAX)
+                    ref = cr[0];
+                else
+                    need_pop = true;
             }
         }
-        /* push far-to-near so the nearest child is popped next */
-        for (int a = 1; a < ncand; ++a)
-            for (int b = a; b > 0 && tn[b] > tn[b - 1]; --b)
+        if (leaf_left > 0)
+        {
+            /* one triangle of the current leaf */
+            const TriRec tr = sc.tris[leaf_next];
+            ++ntris;
+            ++leaf_next;
+            --leaf_left;
+            float tt, b1, b2;
+            if (tri_test(r, best.t, f3{tr.v0p.x, tr.v0p.y, tr.v0p.z}, f3{tr.v1.x, tr.v1.y, tr.v1.z},
+                         f3{tr.v2.x, tr.v2.y, tr.v2.z}, &tt, &b1, &b2))
             {
-                const float    tf = tn[b];
-                const uint32_t rf = cref[b];
-                tn[b]             = tn[b - 1];
-                cref[b]           = cref[b - 1];
-                tn[b - 1]         = tf;
-                cref[b - 1]       = rf;
+                if (ANY)
+                {
+                    found = true;
+                    return true;
+                }
+                const uint32_t prim = __float_as_uint(tr.v0p.w);
+                if (hit_better(tt, cur_inst, prim, best.t, best.inst, best.prim))
+                {
+                    best.t    = tt;
+                    best.b1   = b1;
+                    best.b2   = b2;
+                    best.inst = cur_inst;
+                    best.prim = prim;
+                    found     = true;
+                }
             }
-        for (int a = 0; a < ncand; ++a)
-            st.push(cref[a]);
+            if (leaf_left == 0)
+                need_pop = true;
+        }
+        if (need_pop)
+        {
+            /* the next entry that can still hold a closer hit */
+            for (;;)
+            {
+                if (st.sp == 0)
+                    return true;
+                float t;
+                st.pop(ref, t);
+                if (t <= best.t)
+                    break;
+            }
+        }
+        return false;
     }
-    return found;
-}
+};
 
 /* ------------------------------------------------------------ shading */
 
@@ -337,29 +432,35 @@ __global__ __launch_bounds__(256) void k_camera(FrameView fr, uint32_t sample_ba
 }
 
 __global__ __launch_bounds__(256) void k_trace(SceneView sc, RayQueue q, const uint32_t* count, uint32_t depth,
-                                               HitBuf hb, uint32_t* gstack, unsigned long long* ctr)
+                                               HitBuf hb, uint2* gstack, unsigned long long* ctr)
 {
-    __shared__ uint32_t lds[CLPT_LDS_STACK * 256];
-    TraverseStack       st;
-    st.lds     = lds;
-    st.gmem    = gstack;
-    st.tid     = threadIdx.x;
-    st.gid     = blockIdx.x * blockDim.x + threadIdx.x;
-    st.gstride = gridDim.x * blockDim.x;
+    CLPT_DECLARE_STACK(st, gstack);
     uint32_t       nn = 0, nt = 0;
-    const uint32_t n  = *count;
-    for (uint32_t i = st.gid; i < n; i += st.gstride)
+    const uint32_t n    = *count;
+    const float    tmin = depth == 1 ? 0.0f : CLPT_RAY_EPS;
+    uint32_t       i    = st.gid;
+    bool           active = i < n;
+    Tracer<false>  tr;
+    if (active)
     {
         const float4 o = q.o[i], d = q.d[i];
-        RayW         r;
-        r.o    = f3{o.x, o.y, o.z};
-        r.d    = f3{d.x, d.y, d.z};
-        r.tmin = depth == 1 ? 0.0f : CLPT_RAY_EPS;
-        ray_setup(r);
-        HitRec     h;
-        const bool found = traverse<false>(sc, r, CLPT_FLT_MAX, h, st, nn, nt);
-        hb.h[i]          = make_float4(found ? h.t : -1.0f, h.b1, h.b2, __uint_as_float(found ? h.inst : 0xffffffffu));
-        hb.prim[i]       = found ? h.prim : 0xffffffffu;
+        tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, tmin, CLPT_FLT_MAX);
+    }
+    while (active)
+    {
+        if (tr.step(sc, st, nn, nt))
+        {
+            hb.h[i]    = make_float4(tr.found ? tr.best.t : -1.0f, tr.best.b1, tr.best.b2,
+                                     __uint_as_float(tr.found ? tr.best.inst : 0xffffffffu));
+            hb.prim[i] = tr.found ? tr.best.prim : 0xffffffffu;
+            i += st.gstride;
+            active = i < n;
+            if (active)
+            {
+                const float4 o = q.o[i], d = q.d[i];
+                tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, tmin, CLPT_FLT_MAX);
+            }
+        }
     }
     const unsigned long long sn = wave_sum_u64(nn), stt = wave_sum_u64(nt);
     if (lane_id() == 0)
@@ -493,37 +594,46 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, FrameView fr, uint3
 }
 
 __global__ __launch_bounds__(256) void k_shadow(SceneView sc, ShadowQueue sq, uint32_t depth, double* accum,
-                                                BatchCounters* bc, uint32_t* gstack, unsigned long long* ctr)
+                                                BatchCounters* bc, uint2* gstack, unsigned long long* ctr)
 {
-    __shared__ uint32_t lds[CLPT_LDS_STACK * 256];
-    TraverseStack       st;
-    st.lds     = lds;
-    st.gmem    = gstack;
-    st.tid     = threadIdx.x;
-    st.gid     = blockIdx.x * blockDim.x + threadIdx.x;
-    st.gstride = gridDim.x * blockDim.x;
-    uint32_t       nn = 0, nt = 0;
-    const uint32_t n  = bc->shadow[depth];
-    for (uint32_t i = st.gid; i < n; i += st.gstride)
+    CLPT_DECLARE_STACK(st, gstack);
+    uint32_t       nn = 0, nt = 0, nvis = 0;
+    const uint32_t n      = bc->shadow[depth];
+    uint32_t       i      = st.gid;
+    bool           active = i < n;
+    Tracer<true>   tr;
+    if (active)
     {
         const float4 o = sq.o[i], d = sq.d[i];
-        const float2 x = sq.x[i];
-        RayW         r;
-        r.o    = f3{o.x, o.y, o.z};
-        r.d    = f3{d.x, d.y, d.z};
-        r.tmin = CLPT_RAY_EPS;
-        ray_setup(r);
-        HitRec     h;
-        const bool occ = traverse<true>(sc, r, CLPT_FLT_MAX, h, st, nn, nt);
-        if (!occ)
-            accum_add(accum, __float_as_uint(x.y), f3{o.w, d.w, x.x});
-        wave_count(&bc->visible[depth], !occ);
+        tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_RAY_EPS, CLPT_FLT_MAX);
     }
-    const unsigned long long sn = wave_sum_u64(nn), stt = wave_sum_u64(nt);
+    while (active)
+    {
+        if (tr.step(sc, st, nn, nt))
+        {
+            if (!tr.found)
+            {
+                const float4 o = sq.o[i], d = sq.d[i];
+                const float2 x = sq.x[i];
+                accum_add(accum, __float_as_uint(x.y), f3{o.w, d.w, x.x});
+                ++nvis;
+            }
+            i += st.gstride;
+            active = i < n;
+            if (active)
+            {
+                const float4 o = sq.o[i], d = sq.d[i];
+                tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_RAY_EPS, CLPT_FLT_MAX);
+            }
+        }
+    }
+    const unsigned long long sn = wave_sum_u64(nn), stt = wave_sum_u64(nt), sv = wave_sum_u64(nvis);
     if (lane_id() == 0)
     {
         atomicAdd(ctr + 2, sn);
         atomicAdd(ctr + 3, stt);
+        if (sv)
+            atomicAdd(&bc->visible[depth], (uint32_t)sv);
     }
 }
 
@@ -766,39 +876,44 @@ __global__ void k_dome_eval_batch(SceneView sc, const float* dir3, uint32_t n, f
 }
 
 __global__ __launch_bounds__(256) void k_trace_batch(SceneView sc, const float4* rays, uint32_t n, int any,
-                                                     float4* out, uint32_t* prim, uint32_t* gstack)
+                                                     float4* out, uint32_t* prim, uint2* gstack)
 {
-    __shared__ uint32_t lds[CLPT_LDS_STACK * 256];
-    TraverseStack       st;
-    st.lds     = lds;
-    st.gmem    = gstack;
-    st.tid     = threadIdx.x;
-    st.gid     = blockIdx.x * blockDim.x + threadIdx.x;
-    st.gstride = gridDim.x * blockDim.x;
+    CLPT_DECLARE_STACK(st, gstack);
     uint32_t nn = 0, nt = 0;
     for (uint32_t i = st.gid; i < n; i += st.gstride)
     {
         const float4 a = rays[2 * i], b = rays[2 * i + 1];
-        RayW         r;
-        r.o    = f3{a.x, a.y, a.z};
-        r.tmin = a.w;
-        r.d    = f3{b.x, b.y, b.z};
-        ray_setup(r);
-        HitRec h;
+        const f3     ro{a.x, a.y, a.z}, rd{b.x, b.y, b.z};
         if (any)
         {
-            const bool occ = traverse<true>(sc, r, b.w, h, st, nn, nt);
-            out[i]         = make_float4(occ ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f);
-            prim[i]        = 0u;
+            Tracer<true> tr;
+            tr.start(sc, st, ro, rd, a.w, b.w);
+            while (!tr.step(sc, st, nn, nt))
+            {
+            }
+            out[i]  = make_float4(tr.found ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f);
+            prim[i] = 0u;
         }
         else
         {
-            const bool found = traverse<false>(sc, r, b.w, h, st, nn, nt);
-            out[i]  = make_float4(found ? h.t : -1.0f, found ? h.b1 : 0.0f, found ? h.b2 : 0.0f,
-                                 __uint_as_float(found ? h.inst : 0xffffffffu));
-            prim[i] = found ? h.prim : 0xffffffffu;
+            Tracer<false> tr;
+            tr.start(sc, st, ro, rd, a.w, b.w);
+            while (!tr.step(sc, st, nn, nt))
+            {
+            }
+            out[i]  = make_float4(tr.found ? tr.best.t : -1.0f, tr.found ? tr.best.b1 : 0.0f,
+                                 tr.found ? tr.best.b2 : 0.0f, __uint_as_float(tr.found ? tr.best.inst : 0xffffffffu));
+            prim[i] = tr.found ? tr.best.prim : 0xffffffffu;
         }
     }
+}
+
+/* Plain streaming copy (16-byte loads and stores): the reference kernel for
+ * HBM byte counters and the measured copy ceiling. */
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 /* ------------------------------------------------------------ launchers */
@@ -823,7 +938,7 @@ hipError_t launch_camera(hipStream_t s, const FrameView& fr, uint32_t sample_bas
 }
 
 hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, const uint32_t* count, uint32_t depth,
-                        const HitBuf& hb, uint32_t* gstack, unsigned long long* ctr, uint32_t blocks)
+                        const HitBuf& hb, uint2* gstack, unsigned long long* ctr, uint32_t blocks)
 {
     hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(256), 0, s, sc, q, count, depth, hb, gstack, ctr);
     return hipGetLastError();
@@ -838,7 +953,7 @@ hipError_t launch_shade(hipStream_t s, const SceneView& sc, const FrameView& fr,
 }
 
 hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& sq, uint32_t depth, double* accum,
-                         BatchCounters* bc, uint32_t* gstack, unsigned long long* ctr, uint32_t blocks)
+                         BatchCounters* bc, uint2* gstack, unsigned long long* ctr, uint32_t blocks)
 {
     hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(256), 0, s, sc, sq, depth, accum, bc, gstack, ctr);
     return hipGetLastError();
@@ -900,8 +1015,27 @@ hipError_t launch_dome_eval_batch(hipStream_t s, const SceneView& sc, const floa
     return hipGetLastError();
 }
 
+uint32_t resident_blocks_per_cu(int which)
+{
+    int n = 0;
+    hipError_t e;
+    if (which == 0)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace, 256, 0);
+    else if (which == 1)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shadow, 256, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shade, 256, 0);
+    return (e == hipSuccess && n > 0) ? (uint32_t)n : 4u;
+}
+
+hipError_t launch_copy_probe(hipStream_t s, const float4* src, float4* dst, size_t n, uint32_t blocks)
+{
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_trace_batch(hipStream_t s, const SceneView& sc, const float4* rays, uint32_t n, int any, float4* out,
-                              uint32_t* prim, uint32_t* gstack, uint32_t blocks)
+                              uint32_t* prim, uint2* gstack, uint32_t blocks)
 {
     hipLaunchKernelGGL(k_trace_batch, dim3(blocks), dim3(256), 0, s, sc, rays, n, any, out, prim, gstack);
     return hipGetLastError();

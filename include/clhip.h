@@ -236,6 +236,11 @@ int clhip_occlusion_batch(clhip_ctx* ctx, const float* rays8, uint32_t n, uint8_
  * (num_instances * 12 floats). */
 int clhip_instance_inverses(clhip_ctx* ctx, float* out12, uint32_t n);
 
+/* Measurement helper with no reference counterpart: `iters` launches of a
+ * plain 16-byte streaming copy of `bytes` bytes; returns ms per copy.  Used
+ * as the known-traffic kernel for HBM byte counters and the copy ceiling. */
+int clhip_copy_probe(clhip_ctx* ctx, size_t bytes, uint32_t iters, double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

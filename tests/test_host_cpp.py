@@ -32,6 +32,7 @@ def test_host_example_renders():
     lines = dict(l.split(" ", 1) for l in r.stdout.strip().splitlines())
     mean = [float(x) for x in lines["mean"].split()]
     rays = [int(x) for x in lines["rays"].split()]
-    assert rays[0] == 64 * 48 * 4
+    # the example calls render(1) four times; counters cover the last call
+    assert rays[0] == 64 * 48
     assert all(0.05 < m < 5.0 for m in mean)
     assert rays[1] > 0 and rays[1] <= rays[0]
