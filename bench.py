@@ -46,16 +46,68 @@ def cpu_baseline(desc, seconds_target: float, threads: int):
     t = time.perf_counter()
     sc.render_rows(rows_for(n), "f32", samples=(0, 1), nthreads=threads)
     dt = time.perf_counter() - t
-    n = int(max(32, min(h, seconds_target * n / max(dt, 1e-9))))
+    full = dt * h / n  # estimated seconds for the whole frame at 1 spp
+    if full >= seconds_target:
+        n, k = int(max(32, seconds_target / full * h)), 1
+    else:  # whole frame, the first k samples of every pixel
+        n, k = h, int(max(1, min(desc.spp, round(seconds_target / full))))
     rows = rows_for(n)
     t = time.perf_counter()
-    _, cnt = sc.render_rows(rows, "f32", samples=(0, 1), nthreads=threads)
+    _, cnt = sc.render_rows(rows, "f32", samples=(0, k), nthreads=threads)
     dt = time.perf_counter() - t
-    samples = n * w
+    samples = n * w * k
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"{n} of {h} rows spread evenly over the {w}x{h} frame x 1 spp ({samples} camera paths, "
+            "sample": f"{n} of {h} rows spread evenly over the {w}x{h} frame x {k} spp ({samples} camera paths, "
                       f"{int(cnt['rays'].sum())} closest-hit rays), depth {desc.max_depth}, float32 oracle, "
                       f"{threads} threads, {dt:.1f} s"}
+
+
+def batched_views(integ, args) -> dict:
+    """256 views of the same scene (square films, side log-uniform in
+    [64, 512], spp U{4..32}, depth U{3..10}) rendered by one
+    clhip_render_views per step, launch shape from the autotuner."""
+    import numpy as np
+
+    from colvillea_amd import autotune, scenes
+
+    views = scenes.make_c2_views()
+    nsamp = sum(v.width * v.height * v.spp for v in views)
+    tune = None
+    if not args.no_autotune:
+        t = time.perf_counter()
+        tune = autotune.tune(integ, autotune.sample_views(views), reps=2)
+        log(f"autotune: {len(tune['trials'])} trials in {time.perf_counter() - t:.1f} s -> {tune['choice']}")
+        if args.autotune_log:
+            autotune.save(tune, args.autotune_log, workload="256 views, sample = spp // 4")
+    integ.renderViews(views)
+    integ.sync()
+    log("batched warm-up done")
+    t_total, dev_ms = 0.0, 0.0
+    for i in range(args.c2_steps):
+        t = time.perf_counter()
+        integ.renderViews(views)
+        integ.sync()
+        dt = time.perf_counter() - t
+        t_total += dt
+        c = integ.counters()
+        dev_ms += c["render_ms"]
+        log(f"batched step {i + 1}/{args.c2_steps}: {dt * 1e3:.1f} ms, {nsamp / dt / 1e6:.1f} Msamples/s, "
+            f"{c['batches']} batches, {c['graph_replays']} graph replays")
+    lat = integ.viewLatencies()
+    edges = [0, 25, 50, 100, 200, 400, 800, 1600, 3200, 1e9]
+    hist = np.histogram(lat, bins=edges)[0].tolist()
+    return {"workload": f"256 views of the 1080p scene, {nsamp} samples, square sides log-uniform 64..512, "
+                        f"spp U{{4..32}}, depth U{{3..10}}, seed 3",
+            "value": round(nsamp * args.c2_steps / t_total / 1e6, 2), "unit": "Msamples/s",
+            "ms_per_step": round(t_total / args.c2_steps * 1e3, 2),
+            "device_ms_per_step": round(dev_ms / args.c2_steps, 2), "steps": args.c2_steps,
+            "batches": int(c["batches"]), "graph_replays": int(c["graph_replays"]),
+            "launch": integ.getLaunch(),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                           "p95": round(float(np.percentile(lat, 95)), 2),
+                           "p99": round(float(np.percentile(lat, 99)), 2), "max": round(float(lat.max()), 2)},
+            "latency_histogram": {"edges_ms": edges[:-1] + ["inf"], "counts": hist},
+            "autotune": None if tune is None else {"trials": len(tune["trials"]), "best_sample_ms": tune["ms"]}}
 
 
 def main():
@@ -71,6 +123,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the streaming-copy probe")
     ap.add_argument("--copy-iters", type=int, default=20)
+    ap.add_argument("--no-c2", action="store_true", help="skip the batched 256-view render")
+    ap.add_argument("--c2-steps", type=int, default=2)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune-log", default=None, help="write the autotuner's trials here (JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is described in "
@@ -94,9 +150,7 @@ def main():
         integ.sync()
         log(f"warmup {i + 1}/{args.warmup} done")
 
-    integ.setTiming(True)
-    trace_ms = shade_ms = shadow_ms = 0.0
-    nodes = tris = launches = rays_traced = 0
+    # timed steps: plain renders (batch graphs replayed, no per-kernel events)
     t_total = 0.0
     for i in range(args.steps):
         integ.resetIterationIndex()
@@ -107,17 +161,24 @@ def main():
         dt = time.perf_counter() - t
         t_total += dt
         c = integ.counters()
-        trace_ms += c["trace_ms"]
-        shade_ms += c["shade_ms"]
-        shadow_ms += c["shadow_ms"]
-        nodes += c["trace_nodes"]
-        tris += c["trace_tris"]
-        launches += c["trace_launches"]
-        rays_traced += int(c["rays"][1:desc.max_depth + 1].sum())
         log(f"step {i + 1}/{args.steps}: {dt * 1e3:.1f} ms, {desc.width * desc.height * desc.spp / dt / 1e6:.1f} "
-            f"Msamples/s (trace {c['trace_ms']:.1f} ms, shade {c['shade_ms']:.1f} ms, shadow {c['shadow_ms']:.1f} ms)")
+            f"Msamples/s (device {c['render_ms']:.1f} ms, {c['batches']} batches, {c['graph_replays']} graph replays)")
+    # one more step with HIP events around every launch, for the per-kernel
+    # times of the roofline (not part of `value`)
+    integ.setTiming(True)
+    integ.resetIterationIndex()
+    integ.render(desc.spp)
+    integ.sync()
+    c = integ.counters()
+    integ.setTiming(False)
+    trace_ms, shade_ms, shadow_ms = c["trace_ms"], c["shade_ms"], c["shadow_ms"]
+    nodes, tris, launches = c["trace_nodes"], c["trace_tris"], c["trace_launches"]
+    rays_traced = int(c["rays"][1:desc.max_depth + 1].sum())
+    log(f"timed step: trace {trace_ms:.1f} ms, shade {shade_ms:.1f} ms, shadow {shadow_ms:.1f} ms over "
+        f"{launches} closest-hit launches")
     samples = desc.width * desc.height * desc.spp * args.steps
     value = samples / t_total / 1e6
+    batched = None if args.no_c2 else batched_views(integ, args)
     algo_bytes = rays_traced * QUEUE_BYTES_PER_RAY + nodes * NODE_BYTES + tris * TRI_BYTES
     achieved = algo_bytes / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
     per_launch = algo_bytes / max(1, launches)
@@ -149,7 +210,7 @@ def main():
                                   f"{desc.num_triangles()} triangles (fBm grid + {len(desc.instance_mesh) - 1} "
                                   f"instanced icospheres), 2048x1024 log-normal HDRI, seed {desc.info['seed']}",
                       "width": desc.width, "height": desc.height, "spp": desc.spp, "max_depth": desc.max_depth},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu, "batched": batched}
     integ.close()
     print(json.dumps(out), flush=True)
 

@@ -90,6 +90,35 @@ class clhip_stats(C.Structure):
         ("shade_ms", C.c_double),
         ("shadow_ms", C.c_double),
         ("render_ms", C.c_double),
+        ("batches", C.c_uint64),
+        ("graph_replays", C.c_uint64),
+        ("graph_captures", C.c_uint64),
+    ]
+
+
+class clhip_view(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("spp", C.c_uint32),
+        ("max_depth", C.c_uint32),
+        ("rr_depth", C.c_uint32),
+        ("lane", C.c_uint32),
+        ("pos", C.c_float * 3),
+        ("d00", C.c_float * 3),
+        ("ddu", C.c_float * 3),
+        ("ddv", C.c_float * 3),
+    ]
+
+
+class clhip_launch(C.Structure):
+    _fields_ = [
+        ("trace_blocks_per_cu", C.c_uint32),
+        ("shadow_blocks_per_cu", C.c_uint32),
+        ("shade_blocks_per_cu", C.c_uint32),
+        ("paths_per_batch", C.c_uint32),
+        ("graph", C.c_uint32),
+        ("order", C.c_uint32),
     ]
 
 
@@ -129,6 +158,12 @@ SIGNATURES = {
     "clhip_occlusion_batch": (C.c_int, [P, f32p, u32, C.POINTER(C.c_uint8)]),
     "clhip_instance_inverses": (C.c_int, [P, f32p, u32]),
     "clhip_copy_probe": (C.c_int, [P, C.c_size_t, u32, C.POINTER(C.c_double)]),
+    "clhip_set_launch": (C.c_int, [P, C.POINTER(clhip_launch)]),
+    "clhip_get_launch": (C.c_int, [P, C.POINTER(clhip_launch)]),
+    "clhip_render_views": (C.c_int, [P, C.POINTER(clhip_view), u32]),
+    "clhip_view_readback": (C.c_int, [P, u32, f32p, C.c_size_t]),
+    "clhip_view_sums": (C.c_int, [P, u32, C.POINTER(C.c_double), C.c_size_t]),
+    "clhip_view_latency": (C.c_int, [P, C.POINTER(C.c_double), u32]),
 }
 
 

@@ -76,6 +76,26 @@ struct TimedLaunch
     hipEvent_t a, b;
     int        kind; /* 0 trace, 1 shade, 2 shadow */
 };
+
+/* Pinned host buffer holding one render's batch tables until the stream
+ * has copied them (event `done`). */
+struct StagingSlot
+{
+    void*      p       = nullptr;
+    size_t     cap     = 0;
+    hipEvent_t done    = nullptr;
+    bool       pending = false;
+};
+
+/* One instantiated batch graph and the launch arguments it was captured
+ * with (kernel arguments are baked into the graph). */
+struct GraphSlot
+{
+    std::string     key;
+    hipGraphExec_t  exec = nullptr;
+    hipGraph_t      graph = nullptr;
+    uint64_t        used  = 0;
+};
 } // namespace
 
 struct clhip_ctx
@@ -110,8 +130,8 @@ struct clhip_ctx
     uint32_t                dome_w = 0, dome_h = 0;
     DevBuf                  d_emitters, d_pcond, d_cdfcond, d_pv, d_cdfv, d_rowint;
 
-    /* camera / parameters */
-    FrameView    fr{};
+    /* camera / parameters (the single-view film) */
+    ViewRec      cam{};
     bool         have_camera = false;
     clhip_params params{5u, 0u, 0u, 0u};
 
@@ -120,7 +140,24 @@ struct clhip_ctx
     DevBuf   d_q[2][3], d_hit, d_prim, d_sq[3], d_gstack, d_bc, d_tot, d_ctr;
     size_t   qcap = 0;
     uint32_t spp  = 0;
-    uint32_t blocks = 0, blocks_trace = 0, blocks_shadow = 0, blocks_shade = 0;
+    uint32_t blocks_trace = 0, blocks_shadow = 0, blocks_shade = 0;
+    clhip_launch launch{};      /* as set; zero fields = defaults */
+    uint32_t     def_trace = 0, def_shadow = 0;
+
+    /* batch tables: device copies at stable addresses (grow only), filled
+     * from a ring of pinned staging buffers on the stream */
+    DevBuf                d_views, d_batches, d_entries, d_cursor;
+    StagingSlot           staging[4];
+    uint32_t              staging_next = 0;
+    std::vector<GraphSlot> graphs;
+    uint64_t              graph_clock = 0;
+
+    /* multi-view film */
+    DevBuf                d_vaccum;
+    std::vector<ViewRec>  vviews;
+    std::vector<uint32_t> vspp, vlast_batch;
+    std::vector<hipEvent_t> batch_events;
+    bool                  have_views = false;
 
     /* timing */
     bool                      timing = false;
@@ -129,6 +166,7 @@ struct clhip_ctx
     hipEvent_t                ev_begin = nullptr, ev_end = nullptr;
     bool                      have_render_events = false;
     uint64_t                  trace_launches = 0, shadow_launches = 0;
+    uint64_t                  nbatches = 0, graph_replays = 0, graph_captures = 0;
 };
 
 namespace
@@ -256,6 +294,10 @@ int sync_instances(clhip_ctx* c)
 int ensure_render_buffers(clhip_ctx* c, size_t paths)
 {
     const size_t npix = (size_t)c->width * c->height;
+    const size_t gthreads = (size_t)std::max(c->blocks_trace, c->blocks_shadow) * 256;
+    if (paths > c->qcap || 3 * npix * sizeof(double) > c->d_accum.bytes ||
+        gthreads * CLPT_GSTACK_ENTRIES * sizeof(uint2) > c->d_gstack.bytes)
+        CLHIP_CHECK(c, hipStreamSynchronize(c->stream)); /* buffers about to move may be in use */
     CLHIP_CHECK(c, c->d_accum.alloc(3 * npix * sizeof(double)));
     CLHIP_CHECK(c, c->d_out.alloc(npix * sizeof(float4)));
     if (paths > c->qcap)
@@ -270,8 +312,9 @@ int ensure_render_buffers(clhip_ctx* c, size_t paths)
         CLHIP_CHECK(c, c->d_sq[2].alloc(paths * sizeof(float2)));
         c->qcap = paths;
     }
-    const size_t gthreads = (size_t)std::max(c->blocks, std::max(c->blocks_trace, c->blocks_shadow)) * 256;
     CLHIP_CHECK(c, c->d_gstack.alloc(gthreads * CLPT_GSTACK_ENTRIES * sizeof(uint2)));
+    if (!c->d_cursor.p)
+        CLHIP_CHECK(c, c->d_cursor.alloc(256));
     if (!c->d_bc.p)
     {
         CLHIP_CHECK(c, c->d_bc.alloc(sizeof(BatchCounters)));
@@ -311,6 +354,271 @@ void recycle_events(clhip_ctx* c)
         c->event_pool.push_back(t.b);
     }
     c->timed.clear();
+}
+
+void recycle_batch_events(clhip_ctx* c)
+{
+    for (hipEvent_t e : c->batch_events)
+        c->event_pool.push_back(e);
+    c->batch_events.clear();
+}
+
+/* Grid sizes from the launch shape (zero = default). */
+void apply_launch(clhip_ctx* c)
+{
+    const clhip_launch& l = c->launch;
+    c->blocks_trace       = c->num_cus * (l.trace_blocks_per_cu ? l.trace_blocks_per_cu : c->def_trace);
+    c->blocks_shadow      = c->num_cus * (l.shadow_blocks_per_cu ? l.shadow_blocks_per_cu : c->def_shadow);
+    c->blocks_shade       = c->num_cus * (l.shade_blocks_per_cu ? l.shade_blocks_per_cu : 8u);
+}
+
+bool use_graph(const clhip_ctx* c) { return !c->timing && c->launch.graph != 2u; }
+
+/* Copies the batch tables to their device arrays, in stream order: the
+ * previous render may still read the old tables, so the copies are queued
+ * behind it from pinned staging that stays untouched until they ran. */
+int upload_tables(clhip_ctx* c, const std::vector<ViewRec>& views, const std::vector<BatchRec>& batches,
+                  const std::vector<BatchEntry>& entries)
+{
+    const size_t vb = views.size() * sizeof(ViewRec), bb = batches.size() * sizeof(BatchRec),
+                 eb = entries.size() * sizeof(BatchEntry);
+    if (vb > c->d_views.bytes || bb > c->d_batches.bytes || eb > c->d_entries.bytes)
+    {
+        /* growth moves the arrays: nothing in flight may still read them */
+        CLHIP_CHECK(c, hipStreamSynchronize(c->stream));
+        CLHIP_CHECK(c, c->d_views.alloc(std::max<size_t>(2 * vb, 4096)));
+        CLHIP_CHECK(c, c->d_batches.alloc(std::max<size_t>(2 * bb, 4096)));
+        CLHIP_CHECK(c, c->d_entries.alloc(std::max<size_t>(2 * eb, 4096)));
+    }
+    StagingSlot& s = c->staging[c->staging_next++ % 4];
+    if (s.pending)
+    {
+        CLHIP_CHECK(c, hipEventSynchronize(s.done));
+        s.pending = false;
+    }
+    const size_t need = vb + bb + eb;
+    if (need > s.cap)
+    {
+        if (s.p)
+            CLHIP_CHECK(c, hipHostFree(s.p));
+        s.p   = nullptr;
+        s.cap = 0;
+        CLHIP_CHECK(c, hipHostMalloc(&s.p, std::max<size_t>(2 * need, 4096), hipHostMallocDefault));
+        s.cap = std::max<size_t>(2 * need, 4096);
+    }
+    if (!s.done)
+        CLHIP_CHECK(c, hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    char* h = static_cast<char*>(s.p);
+    std::memcpy(h, views.data(), vb);
+    std::memcpy(h + vb, batches.data(), bb);
+    std::memcpy(h + vb + bb, entries.data(), eb);
+    CLHIP_CHECK(c, hipMemcpyAsync(c->d_views.p, h, vb, hipMemcpyHostToDevice, c->stream));
+    CLHIP_CHECK(c, hipMemcpyAsync(c->d_batches.p, h + vb, bb, hipMemcpyHostToDevice, c->stream));
+    CLHIP_CHECK(c, hipMemcpyAsync(c->d_entries.p, h + vb + bb, eb, hipMemcpyHostToDevice, c->stream));
+    CLHIP_CHECK(c, hipEventRecord(s.done, c->stream));
+    s.pending = true;
+    return CLHIP_OK;
+}
+
+/* Everything the kernels of one batch take as arguments. */
+struct BatchArgs
+{
+    SceneView      sc;
+    BatchTable     bt;
+    RayQueue       q[2];
+    HitBuf         hb;
+    ShadowQueue    sq;
+    double*        accum;
+    BatchCounters* bc;
+    TotalCounters* tot;
+    uint2*         gst;
+    unsigned long long* ctr;
+    uint32_t       max_depth, blocks_trace, blocks_shadow, blocks_shade;
+};
+
+BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
+{
+    BatchArgs a{};
+    a.sc = scene_view(c);
+    a.bt = BatchTable{c->d_views.as<const ViewRec>(), c->d_batches.as<const BatchRec>(),
+                      c->d_entries.as<const BatchEntry>(), c->d_cursor.as<uint32_t>()};
+    for (int k = 0; k < 2; ++k)
+    {
+        a.q[k].o     = c->d_q[k][0].as<float4>();
+        a.q[k].d     = c->d_q[k][1].as<float4>();
+        a.q[k].T     = c->d_q[k][2].as<float4>();
+        a.q[k].count = nullptr;
+    }
+    a.hb            = HitBuf{c->d_hit.as<float4>(), c->d_prim.as<uint32_t>()};
+    a.sq            = ShadowQueue{c->d_sq[0].as<float4>(), c->d_sq[1].as<float4>(), c->d_sq[2].as<float2>(), nullptr};
+    a.accum         = accum;
+    a.bc            = c->d_bc.as<BatchCounters>();
+    a.tot           = c->d_tot.as<TotalCounters>();
+    a.gst           = c->d_gstack.as<uint2>();
+    a.ctr           = c->d_ctr.as<unsigned long long>();
+    a.max_depth     = max_depth;
+    a.blocks_trace  = c->blocks_trace;
+    a.blocks_shadow = c->blocks_shadow;
+    a.blocks_shade  = c->blocks_shade;
+    return a;
+}
+
+/* The launches of one batch: camera rays, then per depth closest hit,
+ * shading and (below the last depth) shadow rays, then the batch end that
+ * folds the counters and advances the cursor.  With `timed`, HIP events
+ * bracket every traversal and shading launch. */
+int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
+{
+    CLHIP_CHECK(c, clpt::launch_camera(c->stream, a.bt, a.q[0], a.bc, a.blocks_shade));
+    for (uint32_t depth = 1; depth <= a.max_depth; ++depth)
+    {
+        const RayQueue& qin  = a.q[(depth - 1) & 1];
+        const RayQueue& qout = a.q[depth & 1];
+        for (int kind = 0; kind < 3; ++kind)
+        {
+            if (kind == 2 && depth == a.max_depth)
+                break;
+            TimedLaunch tl{};
+            if (timed)
+            {
+                tl.a    = pool_event(c);
+                tl.b    = pool_event(c);
+                tl.kind = kind;
+                CLHIP_CHECK(c, hipEventRecord(tl.a, c->stream));
+            }
+            if (kind == 0)
+                CLHIP_CHECK(c, clpt::launch_trace(c->stream, a.sc, qin, &a.bc->rays[depth], depth, a.hb, a.gst, a.ctr,
+                                                  a.blocks_trace));
+            else if (kind == 1)
+                CLHIP_CHECK(c, clpt::launch_shade(c->stream, a.sc, a.bt, depth, qin, a.hb, qout, a.sq, a.accum, a.bc,
+                                                  a.blocks_shade));
+            else
+                CLHIP_CHECK(c, clpt::launch_shadow(c->stream, a.sc, a.sq, depth, a.accum, a.bc, a.gst, a.ctr,
+                                                   a.blocks_shadow));
+            if (timed)
+            {
+                CLHIP_CHECK(c, hipEventRecord(tl.b, c->stream));
+                c->timed.push_back(tl);
+            }
+        }
+    }
+    CLHIP_CHECK(c, clpt::launch_batch_end(c->stream, a.bc, a.tot, a.bt.cursor));
+    return CLHIP_OK;
+}
+
+/* The instantiated graph of one batch for these arguments, captured on
+ * first use.  At most four shapes are kept (least recently used goes). */
+int batch_graph(clhip_ctx* c, const BatchArgs& a, hipGraphExec_t* out)
+{
+    const std::string key(reinterpret_cast<const char*>(&a), sizeof(a));
+    ++c->graph_clock;
+    for (GraphSlot& g : c->graphs)
+        if (g.exec && g.key == key)
+        {
+            g.used = c->graph_clock;
+            *out   = g.exec;
+            return CLHIP_OK;
+        }
+    GraphSlot* slot = nullptr;
+    if (c->graphs.size() < 4)
+    {
+        c->graphs.emplace_back();
+        slot = &c->graphs.back();
+    }
+    else
+    {
+        slot = &c->graphs[0];
+        for (GraphSlot& g : c->graphs)
+            if (g.used < slot->used)
+                slot = &g;
+        /* the old graph may still be running */
+        CLHIP_CHECK(c, hipStreamSynchronize(c->stream));
+        if (slot->exec)
+            (void)hipGraphExecDestroy(slot->exec);
+        if (slot->graph)
+            (void)hipGraphDestroy(slot->graph);
+        slot->exec  = nullptr;
+        slot->graph = nullptr;
+    }
+    CLHIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    const int  rc = issue_batch(c, a, false);
+    hipGraph_t g  = nullptr;
+    hipError_t e  = hipStreamEndCapture(c->stream, &g);
+    if (rc)
+    {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return rc;
+    }
+    CLHIP_CHECK(c, e);
+    hipGraphExec_t x = nullptr;
+    e                = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess)
+    {
+        (void)hipGraphDestroy(g);
+        CLHIP_CHECK(c, e);
+    }
+    slot->key   = key;
+    slot->graph = g;
+    slot->exec  = x;
+    slot->used  = c->graph_clock;
+    ++c->graph_captures;
+    *out = x;
+    return CLHIP_OK;
+}
+
+/* Runs batches 0..nbatches-1 of the uploaded tables into `accum`.  Counters
+ * and the render events cover this call; with `batch_events` an event is
+ * recorded after every batch. */
+int run_batches(clhip_ctx* c, uint32_t nbatches, uint32_t max_depth, double* accum, bool batch_events)
+{
+    CLHIP_CHECK(c, hipMemsetAsync(c->d_tot.p, 0, sizeof(TotalCounters), c->stream));
+    CLHIP_CHECK(c, hipMemsetAsync(c->d_ctr.p, 0, 4 * sizeof(unsigned long long), c->stream));
+    recycle_events(c);
+    recycle_batch_events(c);
+    c->trace_launches = c->shadow_launches = 0;
+    c->nbatches = c->graph_replays = c->graph_captures = 0;
+    if (!c->ev_begin)
+    {
+        CLHIP_CHECK(c, hipEventCreate(&c->ev_begin));
+        CLHIP_CHECK(c, hipEventCreate(&c->ev_end));
+    }
+    CLHIP_CHECK(c, clpt::launch_set_cursor(c->stream, c->d_cursor.as<uint32_t>(), 0u));
+    const BatchArgs a = batch_args(c, accum, max_depth);
+    hipGraphExec_t  x = nullptr;
+    if (use_graph(c))
+    {
+        int rc = batch_graph(c, a, &x);
+        if (rc)
+            return rc;
+    }
+    CLHIP_CHECK(c, hipEventRecord(c->ev_begin, c->stream));
+    for (uint32_t b = 0; b < nbatches; ++b)
+    {
+        if (x)
+        {
+            CLHIP_CHECK(c, hipGraphLaunch(x, c->stream));
+            ++c->graph_replays;
+        }
+        else
+        {
+            int rc = issue_batch(c, a, c->timing);
+            if (rc)
+                return rc;
+        }
+        if (batch_events)
+        {
+            hipEvent_t e = pool_event(c);
+            CLHIP_CHECK(c, hipEventRecord(e, c->stream));
+            c->batch_events.push_back(e);
+        }
+    }
+    CLHIP_CHECK(c, hipEventRecord(c->ev_end, c->stream));
+    c->have_render_events = true;
+    c->nbatches           = nbatches;
+    c->trace_launches     = (uint64_t)nbatches * max_depth;
+    c->shadow_launches    = (uint64_t)nbatches * (max_depth - 1);
+    return CLHIP_OK;
 }
 
 /* Scratch device buffer for the batch entry points. */
@@ -376,29 +684,28 @@ int clhip_create(uint32_t width, uint32_t height, uint32_t integrator_type, clhi
     if (e != hipSuccess || cus <= 0)
         cus = 256;
     c->num_cus = (uint32_t)cus;
-    c->blocks  = c->num_cus * 8;
     {
         /* persistent grids: every lane refills its own rays, so at most the
-         * blocks that fit on the chip at once.  Measured on C1 (MI355X):
-         * closest hit is fastest at full residency, any hit at 3 blocks/CU
-         * (fewer lanes, more rays per lane, less cache pressure).  The
-         * environment variables override (used by the launch autotuner). */
+         * blocks that fit on the chip at once.  Measured on the 1080p scene
+         * (MI355X): closest hit is fastest at full residency, any hit at 3
+         * blocks/CU (fewer lanes, more rays per lane, less cache pressure).
+         * clhip_set_launch and the environment variables override. */
         auto env_u = [](const char* name) {
             const char* v = std::getenv(name);
             return v ? (uint32_t)std::atoi(v) : 0u;
         };
         const uint32_t rt = clpt::resident_blocks_per_cu(0), rs = clpt::resident_blocks_per_cu(1);
         const uint32_t ot = env_u("CLPT_TRACE_BLOCKS_PER_CU"), os = env_u("CLPT_SHADOW_BLOCKS_PER_CU");
-        c->blocks_trace  = c->num_cus * (ot ? ot : rt);
-        c->blocks_shadow = c->num_cus * (os ? os : std::min(rs, 3u));
-        c->blocks_shade  = c->num_cus * 8;
+        c->def_trace  = ot ? ot : std::max(1u, rt);
+        c->def_shadow = os ? os : std::max(1u, std::min(rs, 3u));
+        apply_launch(c.get());
     }
     e          = hipStreamCreate(&c->stream);
     if (e != hipSuccess)
         return fail(nullptr, CLHIP_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
-    c->fr.width  = width;
-    c->fr.height = height;
-    c->fr.npix   = width * height;
+    c->cam.width  = width;
+    c->cam.height = height;
+    c->cam.npix   = width * height;
     *out         = c.release();
     return CLHIP_OK;
 }
@@ -410,8 +717,23 @@ int clhip_destroy(clhip_ctx* ctx)
     if (ctx->stream)
         (void)hipStreamSynchronize(ctx->stream);
     recycle_events(ctx);
+    recycle_batch_events(ctx);
     for (hipEvent_t e : ctx->event_pool)
         (void)hipEventDestroy(e);
+    for (GraphSlot& g : ctx->graphs)
+    {
+        if (g.exec)
+            (void)hipGraphExecDestroy(g.exec);
+        if (g.graph)
+            (void)hipGraphDestroy(g.graph);
+    }
+    for (StagingSlot& st : ctx->staging)
+    {
+        if (st.done)
+            (void)hipEventDestroy(st.done);
+        if (st.p)
+            (void)hipHostFree(st.p);
+    }
     if (ctx->ev_begin)
         (void)hipEventDestroy(ctx->ev_begin);
     if (ctx->ev_end)
@@ -746,10 +1068,10 @@ int clhip_set_camera(clhip_ctx* ctx, const float pos[3], const float d00[3], con
         return fail(ctx, CLHIP_E_INVALID, "clhip_set_camera: bad arguments");
     for (int k = 0; k < 3; ++k)
     {
-        ctx->fr.pos[k] = pos[k];
-        ctx->fr.d00[k] = d00[k];
-        ctx->fr.ddu[k] = ddu[k];
-        ctx->fr.ddv[k] = ddv[k];
+        ctx->cam.pos[k] = pos[k];
+        ctx->cam.d00[k] = d00[k];
+        ctx->cam.ddu[k] = ddu[k];
+        ctx->cam.ddv[k] = ddv[k];
     }
     ctx->have_camera = true;
     return clhip_reset(ctx);
@@ -784,9 +1106,9 @@ int clhip_resize(clhip_ctx* ctx, uint32_t width, uint32_t height)
     CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->width     = width;
     ctx->height    = height;
-    ctx->fr.width  = width;
-    ctx->fr.height = height;
-    ctx->fr.npix   = width * height;
+    ctx->cam.width  = width;
+    ctx->cam.height = height;
+    ctx->cam.npix   = width * height;
     ctx->d_accum.release();
     ctx->d_out.release();
     ctx->spp = 0;
@@ -827,6 +1149,8 @@ int clhip_render(clhip_ctx* ctx, uint32_t spp)
         return rc;
     if (spp == 0)
         return CLHIP_OK;
+    if ((uint64_t)ctx->spp + spp > (1u << 20))
+        return fail(ctx, CLHIP_E_INVALID, "%u + %u samples exceed the 2^20 sample indices of a view", ctx->spp, spp);
     const uint64_t npix = (uint64_t)ctx->width * ctx->height;
     uint32_t       spb  = ctx->params.samples_per_batch;
     if (spb == 0)
@@ -840,95 +1164,219 @@ int clhip_render(clhip_ctx* ctx, uint32_t spp)
         return rc;
     if (fresh || ctx->spp == 0)
         CLHIP_CHECK(ctx, hipMemsetAsync(ctx->d_accum.p, 0, ctx->d_accum.bytes, ctx->stream));
-    CLHIP_CHECK(ctx, hipMemsetAsync(ctx->d_tot.p, 0, sizeof(TotalCounters), ctx->stream));
-    CLHIP_CHECK(ctx, hipMemsetAsync(ctx->d_ctr.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
-    recycle_events(ctx);
-    ctx->trace_launches = ctx->shadow_launches = 0;
-    if (!ctx->ev_begin)
-    {
-        CLHIP_CHECK(ctx, hipEventCreate(&ctx->ev_begin));
-        CLHIP_CHECK(ctx, hipEventCreate(&ctx->ev_end));
-    }
-    CLHIP_CHECK(ctx, hipEventRecord(ctx->ev_begin, ctx->stream));
 
-    FrameView fr  = ctx->fr;
-    fr.max_depth  = ctx->params.max_depth;
-    fr.rr_depth   = ctx->params.rr_depth;
-    fr.view       = ctx->params.view;
-    const SceneView sc = scene_view(ctx);
-    RayQueue        q[2];
-    BatchCounters*  bc = ctx->d_bc.as<BatchCounters>();
-    for (int k = 0; k < 2; ++k)
-    {
-        q[k].o = ctx->d_q[k][0].as<float4>();
-        q[k].d = ctx->d_q[k][1].as<float4>();
-        q[k].T = ctx->d_q[k][2].as<float4>();
-    }
-    HitBuf      hb{ctx->d_hit.as<float4>(), ctx->d_prim.as<uint32_t>()};
-    ShadowQueue sq{ctx->d_sq[0].as<float4>(), ctx->d_sq[1].as<float4>(), ctx->d_sq[2].as<float2>(), nullptr};
-    double*     accum = ctx->d_accum.as<double>();
-    auto*       ctr   = ctx->d_ctr.as<unsigned long long>();
-    uint2*      gst   = ctx->d_gstack.as<uint2>();
-    const uint32_t blocks = ctx->blocks;
-
+    /* one view; batch b holds samples [spp_done + b*spb, ...) of every pixel */
+    ViewRec v     = ctx->cam;
+    v.max_depth   = ctx->params.max_depth;
+    v.rr_depth    = ctx->params.rr_depth;
+    v.lane        = ctx->params.view;
+    v.film_offset = 0;
+    std::vector<ViewRec>    views{v};
+    std::vector<BatchRec>   batches;
+    std::vector<BatchEntry> entries;
     for (uint32_t done = 0; done < spp; done += spb)
     {
-        const uint32_t nb          = std::min(spb, spp - done);
-        const uint32_t npaths      = (uint32_t)(nb * npix);
-        const uint32_t sample_base = ctx->spp + done;
-        CLHIP_CHECK(ctx, clpt::launch_camera(ctx->stream, fr, sample_base, npaths, q[0], bc, blocks));
-        for (uint32_t depth = 1; depth <= fr.max_depth; ++depth)
-        {
-            const RayQueue& qin  = q[(depth - 1) & 1];
-            const RayQueue& qout = q[depth & 1];
-            TimedLaunch     tl{};
-            if (ctx->timing)
-            {
-                tl.a = pool_event(ctx);
-                tl.b = pool_event(ctx);
-                tl.kind = 0;
-                CLHIP_CHECK(ctx, hipEventRecord(tl.a, ctx->stream));
-            }
-            CLHIP_CHECK(ctx, clpt::launch_trace(ctx->stream, sc, qin, &bc->rays[depth], depth, hb, gst, ctr, ctx->blocks_trace));
-            ++ctx->trace_launches;
-            if (ctx->timing)
-            {
-                CLHIP_CHECK(ctx, hipEventRecord(tl.b, ctx->stream));
-                ctx->timed.push_back(tl);
-                tl.a    = pool_event(ctx);
-                tl.b    = pool_event(ctx);
-                tl.kind = 1;
-                CLHIP_CHECK(ctx, hipEventRecord(tl.a, ctx->stream));
-            }
-            CLHIP_CHECK(ctx, clpt::launch_shade(ctx->stream, sc, fr, depth, sample_base, qin, hb, qout, sq, accum, bc, ctx->blocks_shade));
-            if (ctx->timing)
-            {
-                CLHIP_CHECK(ctx, hipEventRecord(tl.b, ctx->stream));
-                ctx->timed.push_back(tl);
-            }
-            if (depth < fr.max_depth)
-            {
-                if (ctx->timing)
-                {
-                    tl.a    = pool_event(ctx);
-                    tl.b    = pool_event(ctx);
-                    tl.kind = 2;
-                    CLHIP_CHECK(ctx, hipEventRecord(tl.a, ctx->stream));
-                }
-                CLHIP_CHECK(ctx, clpt::launch_shadow(ctx->stream, sc, sq, depth, accum, bc, gst, ctr, ctx->blocks_shadow));
-                ++ctx->shadow_launches;
-                if (ctx->timing)
-                {
-                    CLHIP_CHECK(ctx, hipEventRecord(tl.b, ctx->stream));
-                    ctx->timed.push_back(tl);
-                }
-            }
-        }
-        CLHIP_CHECK(ctx, clpt::launch_batch_end(ctx->stream, bc, ctx->d_tot.as<TotalCounters>()));
+        const uint32_t nb = std::min(spb, spp - done);
+        batches.push_back(BatchRec{(uint32_t)entries.size(), 1u, (uint32_t)(nb * npix), 0u});
+        entries.push_back(BatchEntry{0u, 0u, ctx->spp + done, 0u});
     }
-    CLHIP_CHECK(ctx, hipEventRecord(ctx->ev_end, ctx->stream));
-    ctx->have_render_events = true;
+    rc = upload_tables(ctx, views, batches, entries);
+    if (rc)
+        return rc;
+    rc = run_batches(ctx, (uint32_t)batches.size(), v.max_depth, ctx->d_accum.as<double>(), false);
+    if (rc)
+        return rc;
     ctx->spp += spp;
+    return CLHIP_OK;
+}
+
+int clhip_set_launch(clhip_ctx* ctx, const clhip_launch* l)
+{
+    if (!ctx || !l)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_set_launch: bad arguments");
+    const uint32_t rt = clpt::resident_blocks_per_cu(0), rs = clpt::resident_blocks_per_cu(1);
+    if (l->trace_blocks_per_cu > std::max(1u, rt) || l->shadow_blocks_per_cu > std::max(1u, rs))
+        return fail(ctx, CLHIP_E_INVALID,
+                    "persistent grids hold at most %u (closest hit) and %u (any hit) resident blocks per CU", rt, rs);
+    if (l->shade_blocks_per_cu > 64 || l->graph > 2 || l->order > 2)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_set_launch: field out of range");
+    if (l->paths_per_batch && l->paths_per_batch < 4096)
+        return fail(ctx, CLHIP_E_INVALID, "paths_per_batch %u below 4096", l->paths_per_batch);
+    ctx->launch = *l;
+    apply_launch(ctx);
+    return CLHIP_OK;
+}
+
+int clhip_get_launch(const clhip_ctx* ctx, clhip_launch* out)
+{
+    if (!ctx || !out)
+        return fail(nullptr, CLHIP_E_INVALID, "clhip_get_launch: bad arguments");
+    out->trace_blocks_per_cu  = ctx->blocks_trace / ctx->num_cus;
+    out->shadow_blocks_per_cu = ctx->blocks_shadow / ctx->num_cus;
+    out->shade_blocks_per_cu  = ctx->blocks_shade / ctx->num_cus;
+    out->paths_per_batch      = ctx->launch.paths_per_batch ? ctx->launch.paths_per_batch : (4u << 20);
+    out->graph                = use_graph(ctx) ? 1u : 2u;
+    out->order                = ctx->launch.order ? ctx->launch.order : 1u;
+    return CLHIP_OK;
+}
+
+int clhip_render_views(clhip_ctx* ctx, const clhip_view* views, uint32_t nviews)
+{
+    if (!ctx || (!views && nviews))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_render_views: bad arguments");
+    if (ctx->itype != CLHIP_INTEGRATOR_PATH_TRACING)
+        return fail(ctx, CLHIP_E_UNSUPPORTED, "integrator type %u is not available yet", ctx->itype);
+    int rc = sync_instances(ctx);
+    if (rc)
+        return rc;
+    uint64_t film = 0;
+    uint32_t maxd = 1, maxpix = 1;
+    for (uint32_t i = 0; i < nviews; ++i)
+    {
+        const clhip_view& w = views[i];
+        if (w.width == 0 || w.height == 0 || (uint64_t)w.width * w.height > (1u << 26))
+            return fail(ctx, CLHIP_E_INVALID, "view %u: bad film size %ux%u", i, w.width, w.height);
+        if (w.spp == 0 || w.spp > (1u << 20))
+            return fail(ctx, CLHIP_E_INVALID, "view %u: spp %u outside [1, 2^20]", i, w.spp);
+        if (w.max_depth < 1 || w.max_depth > CLHIP_MAX_DEPTH)
+            return fail(ctx, CLHIP_E_INVALID, "view %u: max_depth %u outside [1, %d]", i, w.max_depth, CLHIP_MAX_DEPTH);
+        if (w.lane >= (1u << 12))
+            return fail(ctx, CLHIP_E_INVALID, "view %u: lane %u outside [0, 4096)", i, w.lane);
+        film += (uint64_t)w.width * w.height;
+        maxd   = std::max(maxd, w.max_depth);
+        maxpix = std::max(maxpix, w.width * w.height);
+    }
+    if (film > (1ull << 30))
+        return fail(ctx, CLHIP_E_INVALID, "views hold %llu pixels, more than 2^30", (unsigned long long)film);
+
+    ctx->vviews.assign(nviews, ViewRec{});
+    ctx->vspp.assign(nviews, 0u);
+    ctx->vlast_batch.assign(nviews, 0u);
+    uint32_t off = 0;
+    for (uint32_t i = 0; i < nviews; ++i)
+    {
+        const clhip_view& w = views[i];
+        ViewRec&          v = ctx->vviews[i];
+        v.width       = w.width;
+        v.height      = w.height;
+        v.npix        = w.width * w.height;
+        v.max_depth   = w.max_depth;
+        v.rr_depth    = w.rr_depth;
+        v.lane        = w.lane;
+        v.film_offset = off;
+        for (int k = 0; k < 3; ++k)
+        {
+            v.pos[k] = w.pos[k];
+            v.d00[k] = w.d00[k];
+            v.ddu[k] = w.ddu[k];
+            v.ddv[k] = w.ddv[k];
+        }
+        ctx->vspp[i] = w.spp;
+        off += v.npix;
+    }
+
+    /* batches of whole samples: view i contributes samples [s0, s1) of all
+     * its pixels to a batch; a batch holds at most `cap` paths */
+    std::vector<uint32_t> order(nviews);
+    for (uint32_t i = 0; i < nviews; ++i)
+        order[i] = i;
+    if (ctx->launch.order == 2u)
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            const uint64_t wx = (uint64_t)ctx->vviews[x].npix * ctx->vspp[x] * ctx->vviews[x].max_depth;
+            const uint64_t wy = (uint64_t)ctx->vviews[y].npix * ctx->vspp[y] * ctx->vviews[y].max_depth;
+            return wx < wy;
+        });
+    const uint32_t cap = std::max(maxpix, ctx->launch.paths_per_batch ? ctx->launch.paths_per_batch : (4u << 20));
+    std::vector<BatchRec>   batches;
+    std::vector<BatchEntry> entries;
+    uint32_t                fill = cap;
+    for (uint32_t i : order)
+    {
+        const uint32_t npix = ctx->vviews[i].npix;
+        for (uint32_t s0 = 0; s0 < ctx->vspp[i];)
+        {
+            if (cap - fill < npix)
+            {
+                batches.push_back(BatchRec{(uint32_t)entries.size(), 0u, 0u, 0u});
+                fill = 0;
+            }
+            const uint32_t ns = std::min((cap - fill) / npix, ctx->vspp[i] - s0);
+            BatchRec&      b  = batches.back();
+            entries.push_back(BatchEntry{i, fill, s0, 0u});
+            b.nentries += 1;
+            fill += ns * npix;
+            b.npaths = fill;
+            s0 += ns;
+            ctx->vlast_batch[i] = (uint32_t)batches.size() - 1;
+        }
+    }
+
+    rc = ensure_render_buffers(ctx, cap);
+    if (rc)
+        return rc;
+    if (3 * (size_t)film * sizeof(double) > ctx->d_vaccum.bytes)
+    {
+        CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream)); /* the old film may still be written */
+        CLHIP_CHECK(ctx, ctx->d_vaccum.alloc(std::max<size_t>(3 * (size_t)film * sizeof(double), 16)));
+    }
+    CLHIP_CHECK(ctx, hipMemsetAsync(ctx->d_vaccum.p, 0, 3 * (size_t)film * sizeof(double), ctx->stream));
+    ctx->have_views = true;
+    if (nviews == 0)
+        return CLHIP_OK;
+    rc = upload_tables(ctx, ctx->vviews, batches, entries);
+    if (rc)
+        return rc;
+    return run_batches(ctx, (uint32_t)batches.size(), maxd, ctx->d_vaccum.as<double>(), true);
+}
+
+int clhip_view_sums(clhip_ctx* ctx, uint32_t view, double* rgb, size_t bytes)
+{
+    if (!ctx || !rgb)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_view_sums: bad arguments");
+    if (!ctx->have_views || view >= ctx->vviews.size())
+        return fail(ctx, CLHIP_E_INVALID, "view %u of %zu rendered", view, ctx->vviews.size());
+    const ViewRec& v = ctx->vviews[view];
+    if (bytes < (size_t)v.npix * 3 * sizeof(double))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_view_sums: buffer too small");
+    CLHIP_CHECK(ctx, hipMemcpyAsync(rgb, ctx->d_vaccum.as<double>() + 3 * (size_t)v.film_offset,
+                                    (size_t)v.npix * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return CLHIP_OK;
+}
+
+int clhip_view_readback(clhip_ctx* ctx, uint32_t view, float* rgba, size_t bytes)
+{
+    if (!ctx || !rgba)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_view_readback: bad arguments");
+    if (!ctx->have_views || view >= ctx->vviews.size())
+        return fail(ctx, CLHIP_E_INVALID, "view %u of %zu rendered", view, ctx->vviews.size());
+    const ViewRec& v = ctx->vviews[view];
+    if (bytes < (size_t)v.npix * 16)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_view_readback: buffer too small");
+    DevBuf out;
+    CLHIP_CHECK(ctx, out.alloc((size_t)v.npix * 16));
+    CLHIP_CHECK(ctx, clpt::launch_resolve(ctx->stream, ctx->d_vaccum.as<double>() + 3 * (size_t)v.film_offset, v.npix,
+                                          ctx->vspp[view], out.as<float4>()));
+    CLHIP_CHECK(ctx, hipMemcpyAsync(rgba, out.p, (size_t)v.npix * 16, hipMemcpyDeviceToHost, ctx->stream));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return CLHIP_OK;
+}
+
+int clhip_view_latency(clhip_ctx* ctx, double* ms, uint32_t nviews)
+{
+    if (!ctx || !ms)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_view_latency: bad arguments");
+    if (!ctx->have_views || nviews > ctx->vviews.size())
+        return fail(ctx, CLHIP_E_INVALID, "asked for %u of %zu views", nviews, ctx->vviews.size());
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->batch_events.empty())
+        return fail(ctx, CLHIP_E_STATE, "the last render was not a clhip_render_views");
+    for (uint32_t i = 0; i < nviews; ++i)
+    {
+        float t = 0.0f;
+        CLHIP_CHECK(ctx, hipEventElapsedTime(&t, ctx->ev_begin, ctx->batch_events[ctx->vlast_batch[i]]));
+        ms[i] = t;
+    }
     return CLHIP_OK;
 }
 
@@ -1004,6 +1452,9 @@ int clhip_counters(clhip_ctx* ctx, clhip_stats* out)
     }
     out->trace_launches  = ctx->trace_launches;
     out->shadow_launches = ctx->shadow_launches;
+    out->batches         = ctx->nbatches;
+    out->graph_replays   = ctx->graph_replays;
+    out->graph_captures  = ctx->graph_captures;
     for (auto& tl : ctx->timed)
     {
         float ms = 0.0f;
@@ -1048,10 +1499,10 @@ int clhip_camera_rays(clhip_ctx* ctx, const uint32_t* pixel_sample2, uint32_t n,
     if (!ctx->have_camera)
         return fail(ctx, CLHIP_E_STATE, "no camera");
     for (uint32_t i = 0; i < n; ++i)
-        if (pixel_sample2[2 * i] >= ctx->fr.npix)
+        if (pixel_sample2[2 * i] >= ctx->cam.npix)
             return fail(ctx, CLHIP_E_INVALID, "pixel %u out of range", pixel_sample2[2 * i]);
-    FrameView fr = ctx->fr;
-    fr.view      = ctx->params.view;
+    ViewRec fr = ctx->cam;
+    fr.lane    = ctx->params.view;
     Tmp ps, out;
     CLHIP_CHECK(ctx, ps.from_host(pixel_sample2, 2 * (size_t)n));
     CLHIP_CHECK(ctx, out.zeros((size_t)n * 32));
@@ -1161,7 +1612,7 @@ static int trace_common(clhip_ctx* ctx, const float* rays8, uint32_t n, int any,
     if (rc)
         return rc;
     const SceneView sc = scene_view(ctx);
-    const uint32_t  blocks = std::min<uint32_t>(ctx->blocks, std::max<uint32_t>(1, (n + 255) / 256));
+    const uint32_t  blocks = std::min<uint32_t>(ctx->num_cus * 8, std::max<uint32_t>(1, (n + 255) / 256));
     Tmp             r, o, p, g;
     CLHIP_CHECK(ctx, r.from_host(rays8, 8 * (size_t)n));
     CLHIP_CHECK(ctx, o.zeros(16 * (size_t)n));

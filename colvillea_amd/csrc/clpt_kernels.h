@@ -6,21 +6,21 @@
 
 namespace clpt
 {
-hipError_t launch_camera(hipStream_t s, const FrameView& fr, uint32_t sample_base, uint32_t npaths, const RayQueue& q,
-                         BatchCounters* bc, uint32_t max_blocks);
+hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const RayQueue& q, BatchCounters* bc, uint32_t blocks);
 hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, const uint32_t* count, uint32_t depth,
                         const HitBuf& hb, uint2* gstack, unsigned long long* ctr, uint32_t blocks);
-hipError_t launch_shade(hipStream_t s, const SceneView& sc, const FrameView& fr, uint32_t depth, uint32_t sample_base,
-                        const RayQueue& qin, const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq,
-                        double* accum, BatchCounters* bc, uint32_t blocks);
+hipError_t launch_shade(hipStream_t s, const SceneView& sc, const BatchTable& bt, uint32_t depth, const RayQueue& qin,
+                        const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, double* accum, BatchCounters* bc,
+                        uint32_t blocks);
 hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& sq, uint32_t depth, double* accum,
                          BatchCounters* bc, uint2* gstack, unsigned long long* ctr, uint32_t blocks);
-hipError_t launch_batch_end(hipStream_t s, const BatchCounters* bc, TotalCounters* tot);
+hipError_t launch_batch_end(hipStream_t s, const BatchCounters* bc, TotalCounters* tot, uint32_t* cursor);
+hipError_t launch_set_cursor(hipStream_t s, uint32_t* cursor, uint32_t value);
 hipError_t launch_resolve(hipStream_t s, const double* accum, uint32_t npix, uint32_t spp, float4* out);
 hipError_t launch_dome_tables(hipStream_t s, const float4* tex, uint32_t w, uint32_t h, float* pcond, float* cdfcond,
                               float* pv, float* cdfv, double* rowint);
 hipError_t launch_rng_batch(hipStream_t s, const uint4* seeds, uint32_t n, uint32_t ndraws, uint32_t* ou, float* of);
-hipError_t launch_camera_batch(hipStream_t s, const FrameView& fr, const uint2* ps, uint32_t n, float4* out);
+hipError_t launch_camera_batch(hipStream_t s, const ViewRec& fr, const uint2* ps, uint32_t n, float4* out);
 hipError_t launch_bsdf_batch(hipStream_t s, const SceneView& sc, const uint32_t* mtl, const float* wi3,
                              const float* wo3, const float* u3, uint32_t n, float* ef3, float* epdf, float* swi3,
                              float* sf3, float* spdf);

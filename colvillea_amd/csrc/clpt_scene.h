@@ -119,13 +119,36 @@ struct SceneView
     uint32_t           ninst;
 };
 
-/* Camera and frame parameters of one render. */
-struct FrameView
+/* One camera view: film size, path depth, RNG view lane, the view's offset
+ * in the float64 film, and the camera basis (core::Camera, camera.h:8-27). */
+struct alignas(16) ViewRec
 {
-    uint32_t width, height;
-    float    pos[3], d00[3], ddu[3], ddv[3];
-    uint32_t max_depth, rr_depth, view;
-    uint32_t npix;
+    uint32_t width, height, npix, max_depth;
+    uint32_t rr_depth, lane, film_offset, pad;
+    float    pos[4], d00[4], ddu[4], ddv[4];
+};
+
+/* A batch is a list of (view, first path, first sample) entries; path p of
+ * the batch belongs to the last entry with path_begin <= p, local index
+ * l = p - path_begin maps to pixel l % npix and sample sample_base + l / npix.
+ * The batch to run is read from *cursor, which k_batch_end advances, so a
+ * captured batch graph replays unchanged. */
+struct BatchEntry
+{
+    uint32_t view, path_begin, sample_base, pad;
+};
+
+struct BatchRec
+{
+    uint32_t entry_off, nentries, npaths, pad;
+};
+
+struct BatchTable
+{
+    const ViewRec*    views;
+    const BatchRec*   batches;
+    const BatchEntry* entries;
+    uint32_t*         cursor;
 };
 
 /* A ray queue: 3 float4 planes + count (device). */

@@ -393,38 +393,63 @@ __device__ __forceinline__ void surface_interaction(const SceneView& sc, const H
 
 /* ------------------------------------------------------------ kernels */
 
-__device__ __forceinline__ void camera_ray(const FrameView& fr, uint32_t px, uint32_t py, uint32_t sample, f3* o, f3* d)
+/* generateCameraRays (raygeneration.cu:83-94): seed (px, py, lane<<20 | s,
+ * 0), two draws of box-filter jitter, d = normalize(d00 + u ddu + v ddv). */
+__device__ __forceinline__ void camera_ray(const ViewRec& v, uint32_t px, uint32_t py, uint32_t sample, f3* o, f3* d)
 {
-    const uint32_t z  = (fr.view << 20) | sample;
+    const uint32_t z  = (v.lane << 20) | sample;
     const float    u0 = rng_draw(px, py, z, 1u), u1 = rng_draw(px, py, z, 2u);
-    const float    sx = ((float)px + u0) / (float)fr.width;
-    const float    sy = ((float)py + u1) / (float)fr.height;
-    const f3 d00 = ld3(fr.d00), ddu = ld3(fr.ddu), ddv = ld3(fr.ddv);
-    *o           = ld3(fr.pos);
+    const float    sx = ((float)px + u0) / (float)v.width;
+    const float    sy = ((float)py + u1) / (float)v.height;
+    const f3 d00 = ld3(v.d00), ddu = ld3(v.ddu), ddv = ld3(v.ddv);
+    *o           = ld3(v.pos);
     *d           = norm3(add3(add3(d00, sclL3(sx, ddu)), sclL3(sy, ddv)));
 }
 
-__global__ __launch_bounds__(256) void k_camera(FrameView fr, uint32_t sample_base, uint32_t npaths, RayQueue q,
-                                                BatchCounters* bc)
+/* Path p of batch b -> its view, pixel and sample index. */
+__device__ __forceinline__ void locate_path(const BatchTable& bt, const BatchRec& b, uint32_t p, uint32_t& view,
+                                            uint32_t& pixel, uint32_t& sample)
 {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t lo = 0, hi = b.nentries;
+    while (hi - lo > 1)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bt.entries[b.entry_off + mid].path_begin <= p)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const BatchEntry e = bt.entries[b.entry_off + lo];
+    const uint32_t   n = bt.views[e.view].npix;
+    const uint32_t   l = p - e.path_begin;
+    view               = e.view;
+    pixel              = l % n;
+    sample             = e.sample_base + l / n;
+}
+
+__global__ __launch_bounds__(256) void k_camera(BatchTable bt, RayQueue q, BatchCounters* bc)
+{
+    const uint32_t gid    = blockIdx.x * blockDim.x + threadIdx.x;
+    const BatchRec b      = bt.batches[*bt.cursor];
+    const uint32_t npaths = b.npaths;
     if (gid == 0)
     {
-        for (int b = 0; b < CLPT_MAXD + 2; ++b)
+        for (int k = 0; k < CLPT_MAXD + 2; ++k)
         {
-            bc->rays[b]    = 0u;
-            bc->hits[b]    = 0u;
-            bc->shadow[b]  = 0u;
-            bc->visible[b] = 0u;
+            bc->rays[k]    = 0u;
+            bc->hits[k]    = 0u;
+            bc->shadow[k]  = 0u;
+            bc->visible[k] = 0u;
         }
         bc->rays[1] = npaths;
     }
     for (uint32_t i = gid; i < npaths; i += gridDim.x * blockDim.x)
     {
-        const uint32_t pixel  = i % fr.npix;
-        const uint32_t sample = sample_base + i / fr.npix;
-        f3             o, d;
-        camera_ray(fr, pixel % fr.width, pixel / fr.width, sample, &o, &d);
+        uint32_t vi, pixel, sample;
+        locate_path(bt, b, i, vi, pixel, sample);
+        const ViewRec v = bt.views[vi];
+        f3            o, d;
+        camera_ray(v, pixel % v.width, pixel / v.width, sample, &o, &d);
         q.o[i] = make_float4(o.x, o.y, o.z, 0.0f);
         q.d[i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));
         q.T[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -470,23 +495,25 @@ __global__ __launch_bounds__(256) void k_trace(SceneView sc, RayQueue q, const u
     }
 }
 
-__global__ __launch_bounds__(256) void k_shade(SceneView sc, FrameView fr, uint32_t depth, uint32_t sample_base,
-                                               RayQueue qin, HitBuf hb, RayQueue qout, ShadowQueue sq, double* accum,
-                                               BatchCounters* bc)
+__global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint32_t depth, RayQueue qin, HitBuf hb,
+                                               RayQueue qout, ShadowQueue sq, double* accum, BatchCounters* bc)
 {
     const uint32_t n      = bc->rays[depth];
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool     dome   = sc.dome_index >= 0;
     const float    nl     = (float)sc.nemitters;
+    const BatchRec b      = bt.batches[*bt.cursor];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     {
         const float4   o4 = qin.o[i], d4 = qin.d[i], T4 = qin.T[i];
         const float4   h4 = hb.h[i];
-        const uint32_t path   = __float_as_uint(d4.w);
-        const uint32_t pixel  = path % fr.npix;
-        const uint32_t sample = sample_base + path / fr.npix;
-        const uint32_t px = pixel % fr.width, py = pixel / fr.width;
-        const uint32_t z  = (fr.view << 20) | sample;
+        const uint32_t path = __float_as_uint(d4.w);
+        uint32_t       vi, vpix, sample;
+        locate_path(bt, b, path, vi, vpix, sample);
+        const ViewRec& fr    = bt.views[vi];
+        const uint32_t pixel = fr.film_offset + vpix;
+        const uint32_t px = vpix % fr.width, py = vpix / fr.width;
+        const uint32_t z  = (fr.lane << 20) | sample;
         const f3       dir{d4.x, d4.y, d4.z};
         f3             T{T4.x, T4.y, T4.z};
         const bool     hit = h4.x >= 0.0f;
@@ -637,7 +664,7 @@ __global__ __launch_bounds__(256) void k_shadow(SceneView sc, ShadowQueue sq, ui
     }
 }
 
-__global__ void k_batch_end(const BatchCounters* bc, TotalCounters* tot)
+__global__ void k_batch_end(const BatchCounters* bc, TotalCounters* tot, uint32_t* cursor)
 {
     const int b = threadIdx.x;
     if (b < CLPT_MAXD + 2)
@@ -647,6 +674,16 @@ __global__ void k_batch_end(const BatchCounters* bc, TotalCounters* tot)
         tot->shadow[b] += bc->shadow[b];
         tot->visible[b] += bc->visible[b];
     }
+    if (b == 0)
+        *cursor += 1u;
+}
+
+/* Sets the batch cursor (a kernel rather than a memset, so that it may sit
+ * in a captured graph). */
+__global__ void k_set_cursor(uint32_t* cursor, uint32_t value)
+{
+    if (threadIdx.x == 0)
+        *cursor = value;
 }
 
 __global__ void k_resolve(const double* accum, uint32_t npix, uint32_t spp, float4* out)
@@ -805,7 +842,7 @@ __global__ void k_rng_batch(const uint4* seeds, uint32_t n, uint32_t ndraws, uin
     }
 }
 
-__global__ void k_camera_batch(FrameView fr, const uint2* ps, uint32_t n, float4* out)
+__global__ void k_camera_batch(ViewRec fr, const uint2* ps, uint32_t n, float4* out)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
@@ -930,10 +967,9 @@ static inline uint32_t grid_for(uint64_t items, uint32_t block, uint32_t max_blo
     return (uint32_t)g;
 }
 
-hipError_t launch_camera(hipStream_t s, const FrameView& fr, uint32_t sample_base, uint32_t npaths, const RayQueue& q,
-                         BatchCounters* bc, uint32_t max_blocks)
+hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const RayQueue& q, BatchCounters* bc, uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_camera, dim3(grid_for(npaths, 256, max_blocks)), dim3(256), 0, s, fr, sample_base, npaths, q, bc);
+    hipLaunchKernelGGL(k_camera, dim3(blocks), dim3(256), 0, s, bt, q, bc);
     return hipGetLastError();
 }
 
@@ -944,11 +980,11 @@ hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, c
     return hipGetLastError();
 }
 
-hipError_t launch_shade(hipStream_t s, const SceneView& sc, const FrameView& fr, uint32_t depth, uint32_t sample_base,
-                        const RayQueue& qin, const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq,
-                        double* accum, BatchCounters* bc, uint32_t blocks)
+hipError_t launch_shade(hipStream_t s, const SceneView& sc, const BatchTable& bt, uint32_t depth, const RayQueue& qin,
+                        const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, double* accum, BatchCounters* bc,
+                        uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, sc, fr, depth, sample_base, qin, hb, qout, sq, accum, bc);
+    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, sc, bt, depth, qin, hb, qout, sq, accum, bc);
     return hipGetLastError();
 }
 
@@ -959,9 +995,15 @@ hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& 
     return hipGetLastError();
 }
 
-hipError_t launch_batch_end(hipStream_t s, const BatchCounters* bc, TotalCounters* tot)
+hipError_t launch_batch_end(hipStream_t s, const BatchCounters* bc, TotalCounters* tot, uint32_t* cursor)
 {
-    hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, s, bc, tot);
+    hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, s, bc, tot, cursor);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_cursor(hipStream_t s, uint32_t* cursor, uint32_t value)
+{
+    hipLaunchKernelGGL(k_set_cursor, dim3(1), dim3(64), 0, s, cursor, value);
     return hipGetLastError();
 }
 
@@ -986,7 +1028,7 @@ hipError_t launch_rng_batch(hipStream_t s, const uint4* seeds, uint32_t n, uint3
     return hipGetLastError();
 }
 
-hipError_t launch_camera_batch(hipStream_t s, const FrameView& fr, const uint2* ps, uint32_t n, float4* out)
+hipError_t launch_camera_batch(hipStream_t s, const ViewRec& fr, const uint2* ps, uint32_t n, float4* out)
 {
     hipLaunchKernelGGL(k_camera_batch, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, fr, ps, n, out);
     return hipGetLastError();

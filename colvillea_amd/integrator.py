@@ -103,6 +103,19 @@ class Camera:
     m_camera_ddv: Sequence[float]
 
 
+@dataclass
+class View:
+    """One view of a batched render: film size, samples, depth, RNG lane
+    and camera."""
+    width: int
+    height: int
+    spp: int
+    max_depth: int
+    camera: Camera
+    rr_depth: int = 0
+    lane: int = 0
+
+
 def _f3(v) -> C.Array:
     return (C.c_float * 3)(*[float(x) for x in v])
 
@@ -122,6 +135,7 @@ class Integrator:
         self._type = IntegratorType(type)
         self._meshes: list = []
         self._keep: list = []  # host arrays referenced by the last BLAS build
+        self._views: list = []
         self.spp_per_render = 1
 
     # -- lifetime
@@ -280,12 +294,62 @@ class Integrator:
         self._check(self._lib.clhip_counters(self._ctx, C.byref(st)))
         d = {k: np.array(getattr(st, k)[:], np.uint64) for k in ("rays", "hits", "shadow", "visible")}
         for k in ("trace_nodes", "trace_tris", "shadow_nodes", "shadow_tris", "trace_launches", "shadow_launches",
-                  "trace_ms", "shade_ms", "shadow_ms", "render_ms"):
+                  "trace_ms", "shade_ms", "shadow_ms", "render_ms", "batches", "graph_replays", "graph_captures"):
             d[k] = getattr(st, k)
         return d
 
     def setTiming(self, on: bool = True):
         self._check(self._lib.clhip_set_timing(self._ctx, 1 if on else 0))
+
+    # -- launch shape and batched multi-view rendering (no reference counterpart)
+    LAUNCH_FIELDS = ("trace_blocks_per_cu", "shadow_blocks_per_cu", "shade_blocks_per_cu", "paths_per_batch",
+                     "graph", "order")
+
+    def setLaunch(self, **fields):
+        """Sets the bounce loop's launch shape; omitted or zero fields take
+        their defaults (see clhip_launch)."""
+        bad = set(fields) - set(self.LAUNCH_FIELDS)
+        if bad:
+            raise TypeError(f"unknown launch fields {sorted(bad)}")
+        l = capi.clhip_launch(*[int(fields.get(k, 0)) for k in self.LAUNCH_FIELDS])
+        self._check(self._lib.clhip_set_launch(self._ctx, C.byref(l)))
+
+    def getLaunch(self) -> dict:
+        l = capi.clhip_launch()
+        self._check(self._lib.clhip_get_launch(self._ctx, C.byref(l)))
+        return {k: getattr(l, k) for k in self.LAUNCH_FIELDS}
+
+    def renderViews(self, views: Sequence["View"]):
+        """Renders every view at its own size, spp and depth into per-view
+        films through one sequence of batched launches (asynchronous)."""
+        arr = (capi.clhip_view * max(1, len(views)))()
+        for i, v in enumerate(views):
+            cam = v.camera
+            arr[i] = capi.clhip_view(int(v.width), int(v.height), int(v.spp), int(v.max_depth), int(v.rr_depth),
+                                     int(v.lane), _f3(cam.m_camera_pos), _f3(cam.m_camera_d00),
+                                     _f3(cam.m_camera_ddu), _f3(cam.m_camera_ddv))
+        self._views = list(views)
+        self._check(self._lib.clhip_render_views(self._ctx, arr, len(views)))
+
+    def readbackView(self, i: int) -> np.ndarray:
+        v = self._views[i]
+        out = np.empty((v.height, v.width, 4), np.float32)
+        self._check(self._lib.clhip_view_readback(self._ctx, int(i), capi.fptr(out), out.nbytes))
+        return out
+
+    def readbackViewSums(self, i: int) -> np.ndarray:
+        v = self._views[i]
+        out = np.empty((v.height, v.width, 3), np.float64)
+        self._check(self._lib.clhip_view_sums(self._ctx, int(i), out.ctypes.data_as(C.POINTER(C.c_double)),
+                                              out.nbytes))
+        return out
+
+    def viewLatencies(self) -> np.ndarray:
+        """Per view: ms from the start of the last renderViews until the
+        batch holding its last path finished."""
+        out = np.empty(len(self._views), np.float64)
+        self._check(self._lib.clhip_view_latency(self._ctx, out.ctypes.data_as(C.POINTER(C.c_double)), len(out)))
+        return out
 
     # -- GL interop (integrator.h:66-71) has no counterpart without a display
     def registerFramebuffer(self, glTexture):

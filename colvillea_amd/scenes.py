@@ -14,7 +14,7 @@ from typing import List, Optional
 import numpy as np
 
 from .integrator import (Camera, DiffuseMtl, DirectionalEmitter, GlassMtl, HDRIDome, ImageTexture2D, MetalMtl,
-                         TriangleMesh)
+                         TriangleMesh, View)
 
 
 @dataclass
@@ -284,3 +284,25 @@ def make_c1(seed: int = 2, width: int = 1920, height: int = 1080, spp: int = 64,
     return SceneDesc("C1", [grid_mesh, sphere], [0] + [1] * n_spheres, list(range(n_spheres + 1)),
                      xf.astype(np.float32), materials, [dome], cam, width, height, spp, 8, 0,
                      info={"seed": seed, "triangles": int(ntri)})
+
+
+def make_c2_views(seed: int = 3, n_views: int = 256, side=(64, 512), spp=(4, 32), depth=(3, 10),
+                  rr_depth: int = 0) -> List[View]:
+    """Camera views of the 1080p scene (make_c1) for a batched render:
+    square films with side log-uniform in [64, 512], spp uniform in
+    {4..32}, max depth uniform in {3..10}; eyes on a seeded orbit around
+    the terrain looking at points near its centre.  View i uses RNG lane i."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for i in range(n_views):
+        s = int(round(math.exp(rng.uniform(math.log(side[0]), math.log(side[1])))))
+        n = int(rng.integers(spp[0], spp[1] + 1))
+        d = int(rng.integers(depth[0], depth[1] + 1))
+        az = rng.uniform(0.0, 2.0 * math.pi)
+        el = rng.uniform(0.15, 0.7)
+        dist = rng.uniform(8.0, 22.0)
+        eye = (dist * math.cos(el) * math.sin(az), dist * math.sin(el) + 1.0, dist * math.cos(el) * math.cos(az))
+        at = (rng.uniform(-6.0, 6.0), rng.uniform(0.0, 1.5), rng.uniform(-3.0, 3.0))
+        fov = rng.uniform(30.0, 60.0)
+        views.append(View(s, s, n, d, look_at(eye, at, (0, 1, 0), fov, s, s), rr_depth, i))
+    return views

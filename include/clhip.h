@@ -129,6 +129,9 @@ typedef struct
     double   shade_ms;                     /* summed shading kernel time */
     double   shadow_ms;                    /* summed any-hit kernel time */
     double   render_ms;                    /* wall time of the last clhip_render on the stream */
+    uint64_t batches;                      /* queue batches of the last render */
+    uint64_t graph_replays;                /* of which replayed from a captured hipGraph */
+    uint64_t graph_captures;               /* hipGraph captures made by the last render */
 } clhip_stats;
 
 /* ---- context: Integrator::createIntegrator (integrator.cpp:10-22) and
@@ -235,6 +238,47 @@ int clhip_occlusion_batch(clhip_ctx* ctx, const float* rays8, uint32_t n, uint8_
 /* World-to-object matrices the context computed for its instances
  * (num_instances * 12 floats). */
 int clhip_instance_inverses(clhip_ctx* ctx, float* out12, uint32_t n);
+
+/* ---- batched multi-view rendering.  No reference counterpart: the
+ *      reference renders one camera per Integrator::render (path.cpp:84-231).
+ *      Many views of the current scene go through one sequence of
+ *      persistent launches; paths of several views share each queue batch.
+ *      Callers: view servers, probe/lightmap bakers, multi-camera capture. ---- */
+typedef struct
+{
+    uint32_t width, height; /* film of this view */
+    uint32_t spp;           /* samples per pixel, 1..2^20 */
+    uint32_t max_depth;     /* 1..CLHIP_MAX_DEPTH */
+    uint32_t rr_depth;      /* Russian roulette from this depth (0 = off) */
+    uint32_t lane;          /* RNG view lane < 4096: z = (lane << 20) | sample */
+    float    pos[3], d00[3], ddu[3], ddv[3]; /* camera as in clhip_set_camera */
+} clhip_view;
+
+/* Launch shape of the bounce loop.  Zero in a field selects the default.
+ * Results do not depend on any field (up to float64 summation order). */
+typedef struct
+{
+    uint32_t trace_blocks_per_cu;  /* persistent closest-hit grid (256-thread blocks per CU) */
+    uint32_t shadow_blocks_per_cu; /* persistent any-hit grid */
+    uint32_t shade_blocks_per_cu;  /* camera and shading grids */
+    uint32_t paths_per_batch;      /* queue chunk of clhip_render_views (raised to the largest view) */
+    uint32_t graph;                /* 1: replay a captured hipGraph per batch; 2: plain launches */
+    uint32_t order;                /* clhip_render_views: 1 = given order, 2 = least work first */
+} clhip_launch;
+
+int clhip_set_launch(clhip_ctx* ctx, const clhip_launch* launch);
+int clhip_get_launch(const clhip_ctx* ctx, clhip_launch* out); /* effective values */
+
+/* Renders `spp` samples of every view into a film of its own (cleared on
+ * each call; the single-view film is untouched).  Asynchronous. */
+int clhip_render_views(clhip_ctx* ctx, const clhip_view* views, uint32_t num_views);
+/* View v of the last clhip_render_views: RGBA float32 = sums / spp, or the
+ * raw float64 RGB sums. */
+int clhip_view_readback(clhip_ctx* ctx, uint32_t view, float* rgba, size_t bytes);
+int clhip_view_sums(clhip_ctx* ctx, uint32_t view, double* rgb, size_t bytes);
+/* Milliseconds from the start of the last clhip_render_views until the
+ * batch holding view v's last path finished, per view (synchronises). */
+int clhip_view_latency(clhip_ctx* ctx, double* ms, uint32_t num_views);
 
 /* Measurement helper with no reference counterpart: `iters` launches of a
  * plain 16-byte streaming copy of `bytes` bytes; returns ms per copy.  Used

@@ -35,13 +35,30 @@ def test_exported_symbols_are_c_linkage():
     assert set(header_functions()) <= exported
 
 
-def test_struct_layouts_match_header():
-    # sizes of the POD mirrors (the header is plain C, natural alignment)
-    assert C.sizeof(capi.clhip_mesh) == 5 * 8 + 8
-    assert C.sizeof(capi.clhip_material) == 4 * 12
-    assert C.sizeof(capi.clhip_emitter) == 4 * 9
-    assert C.sizeof(capi.clhip_params) == 16
-    assert C.sizeof(capi.clhip_stats) == 4 * 33 * 8 + 6 * 8 + 4 * 8
+STRUCTS = ("clhip_mesh", "clhip_material", "clhip_emitter", "clhip_params", "clhip_stats", "clhip_view",
+           "clhip_launch")
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """sizeof and every field offset of the ctypes mirrors equal what a C
+    compiler makes of include/clhip.h."""
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "clhip.h"', "int main(void) {"]
+    for name in STRUCTS:
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for f, _ in getattr(capi, name)._fields_:
+            lines.append(f'printf("{name} {f} %zu\\n", offsetof({name}, {f}));')
+    lines += ["return 0; }"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for name in STRUCTS:
+        ct = getattr(capi, name)
+        assert got[(name, "size")] == C.sizeof(ct), name
+        for f, _ in ct._fields_:
+            assert got[(name, f)] == getattr(ct, f).offset, (name, f)
 
 
 def test_invalid_arguments_return_status():
