@@ -122,6 +122,31 @@ class clhip_launch(C.Structure):
     ]
 
 
+class clhip_guide(C.Structure):
+    _fields_ = [
+        ("enable", C.c_uint32),
+        ("bsdf_fraction", C.c_float),
+        ("train", C.c_uint32),
+        ("record_paths", C.c_uint32),
+        ("batch_size", C.c_uint32),
+        ("learning_rate", C.c_float),
+        ("max_steps", C.c_uint32),
+    ]
+
+
+class clhip_guide_stats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_uint64),
+        ("samples", C.c_uint64),
+        ("last_count", C.c_uint32),
+        ("last_loss", C.c_float),
+        ("train_ms", C.c_double),
+    ]
+
+
+GUIDE_PARAMS = 32768
+GUIDE_OUTPUTS = 40
+
 P = C.c_void_p
 u32, i32, f32p, u32p = C.c_uint32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)
 
@@ -164,6 +189,13 @@ SIGNATURES = {
     "clhip_view_readback": (C.c_int, [P, u32, f32p, C.c_size_t]),
     "clhip_view_sums": (C.c_int, [P, u32, C.POINTER(C.c_double), C.c_size_t]),
     "clhip_view_latency": (C.c_int, [P, C.POINTER(C.c_double), u32]),
+    "clhip_set_guide": (C.c_int, [P, C.POINTER(clhip_guide)]),
+    "clhip_guide_reset": (C.c_int, [P, C.c_uint64]),
+    "clhip_guide_get_weights": (C.c_int, [P, f32p, u32]),
+    "clhip_guide_set_weights": (C.c_int, [P, f32p, u32]),
+    "clhip_guide_stats_get": (C.c_int, [P, C.POINTER(clhip_guide_stats)]),
+    "clhip_guide_infer": (C.c_int, [P, f32p, u32, f32p]),
+    "clhip_guide_train": (C.c_int, [P, f32p, u32, f32p, f32p]),
 }
 
 

@@ -5,6 +5,7 @@
  */
 #include "../../include/clhip.h"
 #include "bvh_build.h"
+#include "clpt_guide.h"
 #include "clpt_kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -158,6 +159,20 @@ struct clhip_ctx
     std::vector<uint32_t> vspp, vlast_batch;
     std::vector<hipEvent_t> batch_events;
     bool                  have_views = false;
+
+    /* learned proposal */
+    float       world_lo[3] = {0.0f, 0.0f, 0.0f}, world_hi[3] = {1.0f, 1.0f, 1.0f};
+    clhip_guide guide{};
+    bool        guide_ready = false;
+    DevBuf      g_params, g_m1, g_m2, g_W, g_WT, g_img, g_gc;
+    DevBuf      g_X, g_act[4], g_out, g_grad[5], g_loss, g_partial, g_samples;
+    uint32_t    g_N = 0;
+    DevBuf      g_V, g_E, g_tag;
+    uint32_t    g_slots = 0;
+    size_t      g_tagcap = 0;
+    uint32_t    blocks_guided = 0;
+    uint64_t    g_issued = 0; /* training steps issued since the last reset */
+    std::vector<TimedLaunch> train_timed;
 
     /* timing */
     bool                      timing = false;
@@ -363,6 +378,146 @@ void recycle_batch_events(clhip_ctx* c)
     c->batch_events.clear();
 }
 
+/* ---- learned proposal: buffers and state ---- */
+
+bool guiding(const clhip_ctx* c) { return c->guide.enable != 0u && c->guide_ready; }
+
+int guide_reset_weights(clhip_ctx* c, uint64_t seed)
+{
+    if (!c->g_params.p)
+    {
+        CLHIP_CHECK(c, c->g_params.alloc(CLPT_G_PARAMS * sizeof(float)));
+        CLHIP_CHECK(c, c->g_m1.alloc(CLPT_G_PARAMS * sizeof(float)));
+        CLHIP_CHECK(c, c->g_m2.alloc(CLPT_G_PARAMS * sizeof(float)));
+        CLHIP_CHECK(c, c->g_W.alloc(CLPT_G_PARAMS * sizeof(__bf16)));
+        CLHIP_CHECK(c, c->g_WT.alloc(CLPT_G_PARAMS * sizeof(__bf16)));
+        CLHIP_CHECK(c, c->g_img.alloc(CLPT_G_IMG * sizeof(__bf16)));
+        CLHIP_CHECK(c, c->g_gc.alloc(sizeof(GuideCounters)));
+        CLHIP_CHECK(c, hipMemset(c->g_img.p, 0, CLPT_G_IMG * sizeof(__bf16)));
+    }
+    /* He-uniform weights from a SplitMix64 stream; the output layer is
+     * scaled by 0.1 and its unused columns (40..63) are zero */
+    std::vector<float> w(CLPT_G_PARAMS, 0.0f);
+    uint64_t           st = seed;
+    auto               next = [&st]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z          = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z          = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int l = 0; l < CLPT_G_LAYERS; ++l)
+    {
+        const uint32_t in = guide_layer_in(l), off = guide_layer_offset(l);
+        const double   a  = std::sqrt(6.0 / in) * (l == CLPT_G_LAYERS - 1 ? 0.1 : 1.0);
+        for (uint32_t k = 0; k < in; ++k)
+            for (uint32_t m = 0; m < (uint32_t)CLPT_G_HID; ++m)
+            {
+                const double u = next();
+                if (l < CLPT_G_LAYERS - 1 || m < (uint32_t)CLPT_G_NOUT)
+                    w[off + k * CLPT_G_HID + m] = (float)((2.0 * u - 1.0) * a);
+            }
+    }
+    CLHIP_CHECK(c, hipStreamSynchronize(c->stream));
+    CLHIP_CHECK(c, hipMemcpy(c->g_params.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    CLHIP_CHECK(c, hipMemset(c->g_m1.p, 0, CLPT_G_PARAMS * sizeof(float)));
+    CLHIP_CHECK(c, hipMemset(c->g_m2.p, 0, CLPT_G_PARAMS * sizeof(float)));
+    CLHIP_CHECK(c, hipMemset(c->g_gc.p, 0, sizeof(GuideCounters)));
+    CLHIP_CHECK(c, clpt::launch_guide_pack(c->stream, c->g_params.as<const float>(), c->g_W.as<__bf16>(),
+                                           c->g_WT.as<__bf16>(), c->g_img.as<__bf16>()));
+    CLHIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->guide_ready = true;
+    c->g_issued    = 0;
+    return CLHIP_OK;
+}
+
+int guide_train_buffers(clhip_ctx* c, uint32_t N)
+{
+    if (N <= c->g_N)
+        return CLHIP_OK;
+    CLHIP_CHECK(c, hipStreamSynchronize(c->stream));
+    CLHIP_CHECK(c, c->g_X.alloc((size_t)N * CLPT_G_IN * sizeof(__bf16)));
+    for (int l = 0; l < 4; ++l)
+        CLHIP_CHECK(c, c->g_act[l].alloc((size_t)N * CLPT_G_HID * sizeof(__bf16)));
+    for (int l = 0; l < 5; ++l)
+        CLHIP_CHECK(c, c->g_grad[l].alloc((size_t)N * CLPT_G_HID * sizeof(__bf16)));
+    CLHIP_CHECK(c, c->g_out.alloc((size_t)N * CLPT_G_HID * sizeof(float)));
+    CLHIP_CHECK(c, c->g_loss.alloc((size_t)(N / 256) * sizeof(float)));
+    CLHIP_CHECK(c, c->g_partial.alloc((size_t)(N / 1024) * CLPT_G_PARAMS * sizeof(float)));
+    CLHIP_CHECK(c, c->g_samples.alloc((size_t)N * sizeof(GuideSample)));
+    c->g_N = N;
+    return CLHIP_OK;
+}
+
+/* Recording buffers: training records for `slots` paths and a tag per
+ * shadow-queue entry. */
+int guide_record_buffers(clhip_ctx* c, uint32_t slots, size_t qcap)
+{
+    if (slots > c->g_slots)
+    {
+        CLHIP_CHECK(c, hipStreamSynchronize(c->stream));
+        CLHIP_CHECK(c, c->g_V.alloc((size_t)slots * (CLPT_MAXD + 1) * 4 * sizeof(float4)));
+        CLHIP_CHECK(c, c->g_E.alloc((size_t)slots * (CLPT_MAXD + 1) * sizeof(float4)));
+        c->g_slots = slots;
+    }
+    if (qcap > c->g_tagcap)
+    {
+        CLHIP_CHECK(c, hipStreamSynchronize(c->stream));
+        CLHIP_CHECK(c, c->g_tag.alloc(qcap * sizeof(uint32_t)));
+        c->g_tagcap = qcap;
+    }
+    return CLHIP_OK;
+}
+
+bool guide_recording(const clhip_ctx* c)
+{
+    return guiding(c) && c->guide.train && (c->guide.max_steps == 0u || c->g_issued < c->guide.max_steps);
+}
+
+GuideView guide_view(const clhip_ctx* c)
+{
+    GuideView g{};
+    if (!guiding(c))
+        return g;
+    g.img    = c->g_img.as<const __bf16>();
+    g.c_bsdf = c->guide.bsdf_fraction;
+    for (int k = 0; k < 3; ++k)
+    {
+        const float ext = c->world_hi[k] - c->world_lo[k];
+        g.lo[k]         = c->world_lo[k];
+        g.inv_ext[k]    = ext > 1e-20f ? 1.0f / ext : 1.0f;
+    }
+    if (guide_recording(c))
+    {
+        g.nslots = c->guide.record_paths;
+        g.V      = c->g_V.as<float4>();
+        g.E      = c->g_E.as<float4>();
+    }
+    return g;
+}
+
+clpt::GuideTrainBufs guide_bufs(clhip_ctx* c)
+{
+    clpt::GuideTrainBufs b{};
+    b.samples = c->g_samples.as<GuideSample>();
+    b.gc      = c->g_gc.as<GuideCounters>();
+    b.X       = c->g_X.as<__bf16>();
+    for (int l = 0; l < 4; ++l)
+        b.act[l] = c->g_act[l].as<__bf16>();
+    b.out = c->g_out.as<float>();
+    for (int l = 0; l < 5; ++l)
+        b.grad[l] = c->g_grad[l].as<__bf16>();
+    b.loss_part = c->g_loss.as<float>();
+    b.partial   = c->g_partial.as<float>();
+    b.params    = c->g_params.as<float>();
+    b.m1        = c->g_m1.as<float>();
+    b.m2        = c->g_m2.as<float>();
+    b.W         = c->g_W.as<__bf16>();
+    b.WT        = c->g_WT.as<__bf16>();
+    b.img       = c->g_img.as<__bf16>();
+    return b;
+}
+
 /* Grid sizes from the launch shape (zero = default). */
 void apply_launch(clhip_ctx* c)
 {
@@ -434,6 +589,11 @@ struct BatchArgs
     uint2*         gst;
     unsigned long long* ctr;
     uint32_t       max_depth, blocks_trace, blocks_shadow, blocks_shade;
+    GuideView      gv;
+    uint32_t       guided, blocks_guided;
+    GuideSample*   gsamples; /* gather target when recording */
+    GuideCounters* gc;
+    uint32_t       gcap;
 };
 
 BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
@@ -456,6 +616,17 @@ BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
     a.tot           = c->d_tot.as<TotalCounters>();
     a.gst           = c->d_gstack.as<uint2>();
     a.ctr           = c->d_ctr.as<unsigned long long>();
+    a.gv            = guide_view(c);
+    a.guided        = guiding(c) ? 1u : 0u;
+    a.blocks_guided = c->blocks_guided;
+    if (a.guided)
+        a.sq.tag = c->g_tag.as<uint32_t>();
+    if (a.gv.nslots)
+    {
+        a.gsamples = c->g_samples.as<GuideSample>();
+        a.gc       = c->g_gc.as<GuideCounters>();
+        a.gcap     = c->guide.batch_size;
+    }
     a.max_depth     = max_depth;
     a.blocks_trace  = c->blocks_trace;
     a.blocks_shadow = c->blocks_shadow;
@@ -469,7 +640,7 @@ BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
  * bracket every traversal and shading launch. */
 int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
 {
-    CLHIP_CHECK(c, clpt::launch_camera(c->stream, a.bt, a.q[0], a.bc, a.blocks_shade));
+    CLHIP_CHECK(c, clpt::launch_camera(c->stream, a.bt, a.gv, a.q[0], a.bc, a.blocks_shade));
     for (uint32_t depth = 1; depth <= a.max_depth; ++depth)
     {
         const RayQueue& qin  = a.q[(depth - 1) & 1];
@@ -489,12 +660,15 @@ int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
             if (kind == 0)
                 CLHIP_CHECK(c, clpt::launch_trace(c->stream, a.sc, qin, &a.bc->rays[depth], depth, a.hb, a.gst, a.ctr,
                                                   a.blocks_trace));
+            else if (kind == 1 && a.guided)
+                CLHIP_CHECK(c, clpt::launch_shade_guided(c->stream, a.sc, a.bt, a.gv, depth, qin, a.hb, qout, a.sq,
+                                                         a.accum, a.bc, a.blocks_guided));
             else if (kind == 1)
                 CLHIP_CHECK(c, clpt::launch_shade(c->stream, a.sc, a.bt, depth, qin, a.hb, qout, a.sq, a.accum, a.bc,
                                                   a.blocks_shade));
             else
                 CLHIP_CHECK(c, clpt::launch_shadow(c->stream, a.sc, a.sq, depth, a.accum, a.bc, a.gst, a.ctr,
-                                                   a.blocks_shadow));
+                                                   a.gv.nslots ? a.gv.E : nullptr, a.blocks_shadow));
             if (timed)
             {
                 CLHIP_CHECK(c, hipEventRecord(tl.b, c->stream));
@@ -502,7 +676,26 @@ int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
             }
         }
     }
+    if (a.gv.nslots)
+        CLHIP_CHECK(c, clpt::launch_guide_gather(c->stream, a.gv.V, a.gv.E, a.gv.nslots, a.max_depth, a.gsamples,
+                                                 a.gcap, a.gc));
     CLHIP_CHECK(c, clpt::launch_batch_end(c->stream, a.bc, a.tot, a.bt.cursor));
+    return CLHIP_OK;
+}
+
+/* One training step on the samples gathered by the last batch (timed with
+ * events for the statistics). */
+int issue_train(clhip_ctx* c)
+{
+    TimedLaunch tl{};
+    tl.a = pool_event(c);
+    tl.b = pool_event(c);
+    CLHIP_CHECK(c, hipEventRecord(tl.a, c->stream));
+    CLHIP_CHECK(c, clpt::launch_guide_train(c->stream, guide_bufs(c), c->guide.batch_size, c->guide.learning_rate,
+                                            nullptr));
+    CLHIP_CHECK(c, hipEventRecord(tl.b, c->stream));
+    c->train_timed.push_back(tl);
+    ++c->g_issued;
     return CLHIP_OK;
 }
 
@@ -583,9 +776,23 @@ int run_batches(clhip_ctx* c, uint32_t nbatches, uint32_t max_depth, double* acc
         CLHIP_CHECK(c, hipEventCreate(&c->ev_begin));
         CLHIP_CHECK(c, hipEventCreate(&c->ev_end));
     }
+    for (auto& t : c->train_timed)
+    {
+        c->event_pool.push_back(t.a);
+        c->event_pool.push_back(t.b);
+    }
+    c->train_timed.clear();
+    if (guiding(c))
+    {
+        int rc = guide_record_buffers(c, c->guide.train ? c->guide.record_paths : 0u, c->qcap);
+        if (!rc && c->guide.train)
+            rc = guide_train_buffers(c, c->guide.batch_size);
+        if (rc)
+            return rc;
+    }
     CLHIP_CHECK(c, clpt::launch_set_cursor(c->stream, c->d_cursor.as<uint32_t>(), 0u));
-    const BatchArgs a = batch_args(c, accum, max_depth);
-    hipGraphExec_t  x = nullptr;
+    BatchArgs      a = batch_args(c, accum, max_depth);
+    hipGraphExec_t x = nullptr;
     if (use_graph(c))
     {
         int rc = batch_graph(c, a, &x);
@@ -595,6 +802,23 @@ int run_batches(clhip_ctx* c, uint32_t nbatches, uint32_t max_depth, double* acc
     CLHIP_CHECK(c, hipEventRecord(c->ev_begin, c->stream));
     for (uint32_t b = 0; b < nbatches; ++b)
     {
+        /* the recording flag flips when training stops: re-derive the
+         * arguments (and graph) per batch while guiding */
+        if (guiding(c) && b > 0)
+        {
+            const BatchArgs a2 = batch_args(c, accum, max_depth);
+            if (std::memcmp(&a2, &a, sizeof(a)) != 0)
+            {
+                a = a2;
+                x = nullptr;
+                if (use_graph(c))
+                {
+                    int rc = batch_graph(c, a, &x);
+                    if (rc)
+                        return rc;
+                }
+            }
+        }
         if (x)
         {
             CLHIP_CHECK(c, hipGraphLaunch(x, c->stream));
@@ -603,6 +827,12 @@ int run_batches(clhip_ctx* c, uint32_t nbatches, uint32_t max_depth, double* acc
         else
         {
             int rc = issue_batch(c, a, c->timing);
+            if (rc)
+                return rc;
+        }
+        if (a.gv.nslots)
+        {
+            int rc = issue_train(c);
             if (rc)
                 return rc;
         }
@@ -698,6 +928,7 @@ int clhip_create(uint32_t width, uint32_t height, uint32_t integrator_type, clhi
         const uint32_t ot = env_u("CLPT_TRACE_BLOCKS_PER_CU"), os = env_u("CLPT_SHADOW_BLOCKS_PER_CU");
         c->def_trace  = ot ? ot : std::max(1u, rt);
         c->def_shadow = os ? os : std::max(1u, std::min(rs, 3u));
+        c->blocks_guided = c->num_cus * std::max(1u, clpt::resident_blocks_per_cu(3));
         apply_launch(c.get());
     }
     e          = hipStreamCreate(&c->stream);
@@ -718,6 +949,12 @@ int clhip_destroy(clhip_ctx* ctx)
         (void)hipStreamSynchronize(ctx->stream);
     recycle_events(ctx);
     recycle_batch_events(ctx);
+    for (auto& t : ctx->train_timed)
+    {
+        ctx->event_pool.push_back(t.a);
+        ctx->event_pool.push_back(t.b);
+    }
+    ctx->train_timed.clear();
     for (hipEvent_t e : ctx->event_pool)
         (void)hipEventDestroy(e);
     for (GraphSlot& g : ctx->graphs)
@@ -955,6 +1192,11 @@ int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uin
     ctx->inst_entity = std::move(ent);
     ctx->tlas_root   = r.root;
     ctx->tlas_depth  = r.max_depth;
+    for (int k = 0; k < 3; ++k)
+    {
+        ctx->world_lo[k] = num_instances ? r.lo[k] : 0.0f;
+        ctx->world_hi[k] = num_instances ? r.hi[k] : 1.0f;
+    }
     ctx->have_tlas   = true;
     ctx->inst_dirty  = true;
     return CLHIP_OK;
@@ -1703,4 +1945,157 @@ int clhip_instance_inverses(clhip_ctx* ctx, float* out12, uint32_t n)
     return CLHIP_OK;
 }
 
+/* ------------------------------------------------------------ learned proposal */
+
+int clhip_set_guide(clhip_ctx* ctx, const clhip_guide* g)
+{
+    if (!ctx || !g)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_set_guide: bad arguments");
+    clhip_guide v = *g;
+    if (v.bsdf_fraction == 0.0f)
+        v.bsdf_fraction = 0.5f;
+    if (!(v.bsdf_fraction > 0.0f && v.bsdf_fraction <= 1.0f))
+        return fail(ctx, CLHIP_E_INVALID, "bsdf_fraction %g outside (0, 1]: the BSDF must keep a positive share",
+                    (double)v.bsdf_fraction);
+    if (v.record_paths == 0)
+        v.record_paths = 32768u;
+    if (v.batch_size == 0)
+        v.batch_size = 65536u;
+    if (v.batch_size % 1024u != 0u || v.batch_size > (1u << 22))
+        return fail(ctx, CLHIP_E_INVALID, "batch_size %u must be a multiple of 1024 up to 2^22", v.batch_size);
+    if (v.record_paths > (1u << 22))
+        return fail(ctx, CLHIP_E_INVALID, "record_paths %u above 2^22", v.record_paths);
+    if (v.learning_rate == 0.0f)
+        v.learning_rate = 1e-3f;
+    if (!(v.learning_rate > 0.0f))
+        return fail(ctx, CLHIP_E_INVALID, "learning_rate must be positive");
+    ctx->guide = v;
+    if (v.enable && !ctx->guide_ready)
+        return guide_reset_weights(ctx, 1u);
+    return CLHIP_OK;
+}
+
+int clhip_guide_reset(clhip_ctx* ctx, uint64_t seed)
+{
+    if (!ctx)
+        return fail(nullptr, CLHIP_E_INVALID, "ctx is NULL");
+    return guide_reset_weights(ctx, seed);
+}
+
+int clhip_guide_get_weights(clhip_ctx* ctx, float* w, uint32_t n)
+{
+    if (!ctx || !w || n != CLPT_G_PARAMS)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_guide_get_weights: need %d floats", CLPT_G_PARAMS);
+    if (!ctx->guide_ready)
+        return fail(ctx, CLHIP_E_STATE, "no weights: call clhip_guide_reset or clhip_set_guide first");
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, hipMemcpy(w, ctx->g_params.p, CLPT_G_PARAMS * sizeof(float), hipMemcpyDeviceToHost));
+    return CLHIP_OK;
+}
+
+int clhip_guide_set_weights(clhip_ctx* ctx, const float* w, uint32_t n)
+{
+    if (!ctx || !w || n != CLPT_G_PARAMS)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_guide_set_weights: need %d floats", CLPT_G_PARAMS);
+    if (!ctx->guide_ready)
+    {
+        int rc = guide_reset_weights(ctx, 1u);
+        if (rc)
+            return rc;
+    }
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, hipMemcpy(ctx->g_params.p, w, CLPT_G_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+    CLHIP_CHECK(ctx, clpt::launch_guide_pack(ctx->stream, ctx->g_params.as<const float>(), ctx->g_W.as<__bf16>(),
+                                             ctx->g_WT.as<__bf16>(), ctx->g_img.as<__bf16>()));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return CLHIP_OK;
+}
+
+int clhip_guide_stats_get(clhip_ctx* ctx, clhip_guide_stats* out)
+{
+    if (!ctx || !out)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_guide_stats_get: bad arguments");
+    std::memset(out, 0, sizeof(*out));
+    if (!ctx->guide_ready)
+        return CLHIP_OK;
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    GuideCounters g;
+    CLHIP_CHECK(ctx, hipMemcpy(&g, ctx->g_gc.p, sizeof(g), hipMemcpyDeviceToHost));
+    out->steps      = g.steps;
+    out->samples    = g.samples_total;
+    out->last_count = g.trained;
+    out->last_loss  = g.loss;
+    for (auto& t : ctx->train_timed)
+    {
+        float ms = 0.0f;
+        CLHIP_CHECK(ctx, hipEventElapsedTime(&ms, t.a, t.b));
+        out->train_ms += ms;
+    }
+    return CLHIP_OK;
+}
+
+int clhip_guide_infer(clhip_ctx* ctx, const float* q9, uint32_t n, float* out40)
+{
+    if (!ctx || !q9 || !out40)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_guide_infer: bad arguments");
+    if (!ctx->guide_ready)
+        return fail(ctx, CLHIP_E_STATE, "no weights: call clhip_guide_reset or clhip_set_guide first");
+    if (n == 0)
+        return CLHIP_OK;
+    Tmp q, o;
+    CLHIP_CHECK(ctx, q.from_host(q9, 9 * (size_t)n));
+    CLHIP_CHECK(ctx, o.zeros((size_t)n * CLPT_G_NOUT * sizeof(float)));
+    const uint32_t blocks = std::min<uint32_t>(ctx->num_cus * 2u, (n + 255u) / 256u);
+    CLHIP_CHECK(ctx, clpt::launch_guide_infer(ctx->stream, ctx->g_img.as<const __bf16>(), q.b.as<const float>(), n,
+                                              o.b.as<float>(), blocks));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, o.to_host(out40, (size_t)n * CLPT_G_NOUT));
+    return CLHIP_OK;
+}
+
+int clhip_guide_train(clhip_ctx* ctx, const float* s13, uint32_t n, float* grad, float* loss)
+{
+    if (!ctx || (!s13 && n))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_guide_train: bad arguments");
+    if (!ctx->guide_ready)
+        return fail(ctx, CLHIP_E_STATE, "no weights: call clhip_guide_reset or clhip_set_guide first");
+    if (n > (1u << 22))
+        return fail(ctx, CLHIP_E_INVALID, "at most 2^22 samples per step");
+    const uint32_t N = std::max(1024u, (n + 1023u) / 1024u * 1024u);
+    int            rc = guide_train_buffers(ctx, N);
+    if (rc)
+        return rc;
+    std::vector<GuideSample> hs(n);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        const float* x = s13 + 13 * (size_t)i;
+        hs[i].a        = make_float4(x[0], x[1], x[2], x[3]);
+        hs[i].b        = make_float4(x[4], x[5], x[6], x[7]);
+        hs[i].c        = make_float4(x[8], x[9], x[10], x[11]);
+        hs[i].d        = make_float4(x[12], 0.0f, 0.0f, 0.0f);
+    }
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (n)
+        CLHIP_CHECK(ctx, hipMemcpy(ctx->g_samples.p, hs.data(), n * sizeof(GuideSample), hipMemcpyHostToDevice));
+    CLHIP_CHECK(ctx, hipMemcpy(ctx->g_gc.p, &n, sizeof(uint32_t), hipMemcpyHostToDevice)); /* GuideCounters::count */
+    Tmp g;
+    if (grad)
+        CLHIP_CHECK(ctx, g.zeros(CLPT_G_PARAMS * sizeof(float)));
+    CLHIP_CHECK(ctx, clpt::launch_guide_train(ctx->stream, guide_bufs(ctx), N, ctx->guide.learning_rate > 0.0f
+                                                                                 ? ctx->guide.learning_rate
+                                                                                 : 1e-3f,
+                                              grad ? g.b.as<float>() : nullptr));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (grad)
+        CLHIP_CHECK(ctx, g.to_host(grad, CLPT_G_PARAMS));
+    if (loss)
+    {
+        GuideCounters gc;
+        CLHIP_CHECK(ctx, hipMemcpy(&gc, ctx->g_gc.p, sizeof(gc), hipMemcpyDeviceToHost));
+        *loss = gc.loss;
+    }
+    return CLHIP_OK;
+}
+
 } // extern "C"
+

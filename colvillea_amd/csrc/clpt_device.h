@@ -75,6 +75,8 @@ CLPT_D float rng_draw(uint32_t px, uint32_t py, uint32_t z, uint32_t k) { return
 #define CLPT_EXTRA_DRAW(depth, slot) (0x80000000u + 8u * (uint32_t)(depth) + (uint32_t)(slot))
 #define CLPT_SLOT_RR 0u
 #define CLPT_SLOT_LOBE 1u
+#define CLPT_SLOT_GUIDE 2u   /* BSDF or learned proposal at a guided vertex */
+#define CLPT_SLOT_LOBESEL 3u /* lobe of the learned mixture */
 
 /* ------------------------------------------------------------ warps */
 

@@ -6,14 +6,19 @@
 
 namespace clpt
 {
-hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const RayQueue& q, BatchCounters* bc, uint32_t blocks);
+hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const GuideView& gv, const RayQueue& q, BatchCounters* bc,
+                         uint32_t blocks);
 hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, const uint32_t* count, uint32_t depth,
                         const HitBuf& hb, uint2* gstack, unsigned long long* ctr, uint32_t blocks);
 hipError_t launch_shade(hipStream_t s, const SceneView& sc, const BatchTable& bt, uint32_t depth, const RayQueue& qin,
                         const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, double* accum, BatchCounters* bc,
                         uint32_t blocks);
+/* Shading with the learned proposal mixed into the continuation (weights in LDS). */
+hipError_t launch_shade_guided(hipStream_t s, const SceneView& sc, const BatchTable& bt, const GuideView& gv,
+                               uint32_t depth, const RayQueue& qin, const HitBuf& hb, const RayQueue& qout,
+                               const ShadowQueue& sq, double* accum, BatchCounters* bc, uint32_t blocks);
 hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& sq, uint32_t depth, double* accum,
-                         BatchCounters* bc, uint2* gstack, unsigned long long* ctr, uint32_t blocks);
+                         BatchCounters* bc, uint2* gstack, unsigned long long* ctr, float4* guide_E, uint32_t blocks);
 hipError_t launch_batch_end(hipStream_t s, const BatchCounters* bc, TotalCounters* tot, uint32_t* cursor);
 hipError_t launch_set_cursor(hipStream_t s, uint32_t* cursor, uint32_t value);
 hipError_t launch_resolve(hipStream_t s, const double* accum, uint32_t npix, uint32_t spp, float4* out);
@@ -28,11 +33,39 @@ hipError_t launch_emitter_batch(hipStream_t s, const SceneView& sc, const float*
                                 float* val3, float* pdf, uint32_t* which);
 hipError_t launch_dome_eval_batch(hipStream_t s, const SceneView& sc, const float* dir3, uint32_t n, float* le3,
                                   float* pdf);
-/* Blocks of 256 threads resident per CU for k_trace (0), k_shadow (1), k_shade (2). */
+/* Blocks of 256 threads resident per CU for k_trace (0), k_shadow (1), k_shade (2),
+ * k_shade_guided (3). */
 uint32_t   resident_blocks_per_cu(int which);
 hipError_t launch_copy_probe(hipStream_t s, const float4* src, float4* dst, size_t n, uint32_t blocks);
 hipError_t launch_trace_batch(hipStream_t s, const SceneView& sc, const float4* rays, uint32_t n, int any, float4* out,
                               uint32_t* prim, uint2* gstack, uint32_t blocks);
+
+/* learned proposal (guide.hip) */
+struct GuideTrainBufs
+{
+    GuideSample*   samples;
+    GuideCounters* gc;
+    __bf16*        X;        /* N x 256 encoded inputs */
+    __bf16*        act[4];   /* N x 64 hidden activations */
+    float*         out;      /* N x 64 outputs */
+    __bf16*        grad[5];  /* N x 64 gradients w.r.t. each layer's output */
+    float*         loss_part;
+    float*         partial;  /* (N / 1024) x 32768 split weight gradients */
+    float*         params;
+    float*         m1;
+    float*         m2;
+    __bf16*        W;
+    __bf16*        WT;
+    __bf16*        img;
+};
+hipError_t launch_guide_infer(hipStream_t s, const __bf16* img, const float* q9, uint32_t n, float* out40,
+                              uint32_t blocks);
+hipError_t launch_guide_pack(hipStream_t s, const float* params, __bf16* W, __bf16* WT, __bf16* img);
+hipError_t launch_guide_gather(hipStream_t s, const float4* V, const float4* E, uint32_t nslots, uint32_t maxd,
+                               GuideSample* out, uint32_t cap, GuideCounters* gc);
+/* One Adam step on the samples in b.samples (count in b.gc, at most N, a
+ * multiple of 1024).  grad_out (optional) receives the summed gradient. */
+hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N, float lr, float* grad_out);
 } // namespace clpt
 
 #endif

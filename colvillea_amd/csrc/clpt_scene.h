@@ -172,6 +172,45 @@ struct ShadowQueue
     float4*   d; /* direction.xyz, Lo.g */
     float2*   x; /* Lo.b, pixel (bits) */
     uint32_t* count;
+    uint32_t* tag; /* training record of the shadow ray (guided shading only; else NULL) */
+};
+
+/* ---- learned directional proposal ---- */
+
+/* A training sample: query (9), sampled direction (3), weight. */
+struct GuideSample
+{
+    float4 a; /* q0..q3 */
+    float4 b; /* q4..q7 */
+    float4 c; /* q8, wi.xyz */
+    float4 d; /* weight, unused */
+};
+
+/* Device state of online training. */
+struct GuideCounters
+{
+    uint32_t           count;   /* samples gathered since the last step (may exceed the buffer) */
+    uint32_t           trained; /* samples in the last step */
+    uint32_t           steps;   /* Adam steps taken */
+    float              wsum;    /* weight sum of the last step */
+    float              loss;    /* weighted negative log-likelihood of the last step (per unit weight) */
+    uint32_t           pad;
+    unsigned long long samples_total;
+};
+
+/* What the guided shading kernels need.  Training records: per recorded
+ * path (slot) and vertex depth j, V holds 4 float4 (query q0..q8, the
+ * sampled direction, p_mix and R = f cos / T_next per channel; p_mix = 0
+ * marks no record) and E the contribution (rgb) that arrived at depth j
+ * (escape of ray j or the shadow ray of vertex j). */
+struct GuideView
+{
+    const __bf16* img; /* packed bf16 weights */
+    float         lo[3], inv_ext[3];
+    float         c_bsdf; /* probability of BSDF sampling at guided vertices */
+    uint32_t      nslots; /* recorded paths per batch (0 = no recording) */
+    float4*       V;
+    float4*       E;
 };
 
 /* Per-batch exact counters (device): queue counts per depth and traversal

@@ -1,0 +1,470 @@
+/*
+ * guide.hip -- kernels of the learned directional proposal: batched
+ * inference (the fused forward pass of clpt_guide.h), online training
+ * (encoding, matrix-core GEMMs, mixture loss, Adam) and the gather that
+ * turns the tracer's recorded path vertices into training samples.
+ *
+ * Training runs layer by layer over a batch of N samples kept in HBM:
+ *   forward   A_{l+1} = relu(A_l W_l)           (GEMM "nt", N x 64 tiles)
+ *   loss      G_5 = d(-sum_n w_n log p(wi_n)) / d out, scaled by 1 / sum w
+ *   backward  dW_l = A_l^T G_{l+1}              (GEMM "tn", split over N)
+ *             G_l  = (G_{l+1} W_l^T) . [A_l > 0]
+ *   Adam      on the float32 master weights, then the bfloat16 copies.
+ * Activations and gradients are stored as bfloat16, products accumulate in
+ * float32 on v_mfma_f32_32x32x16_bf16; split-N partial weight gradients
+ * are summed in a fixed order, so a step is deterministic.
+ */
+#include "clpt_guide.h"
+#include "clpt_kernels.h"
+
+namespace
+{
+
+__device__ __forceinline__ void load_image(const __bf16* __restrict__ img, __bf16* w)
+{
+    const uint4* src = reinterpret_cast<const uint4*>(img);
+    uint4*       dst = reinterpret_cast<uint4*>(w);
+    for (uint32_t k = threadIdx.x; k < CLPT_G_IMG / 8; k += blockDim.x)
+        dst[k] = src[k];
+    __syncthreads();
+}
+
+__device__ __forceinline__ float lum3(float r, float g, float b) { return 0.2126f * r + 0.7152f * g + 0.0722f * b; }
+
+} // namespace
+
+/* ------------------------------------------------------------ inference */
+
+__global__ __launch_bounds__(256) void k_guide_infer(const __bf16* img, const float* q9, uint32_t n, float* out40)
+{
+    __shared__ __bf16 w[CLPT_G_IMG];
+    load_image(img, w);
+    const uint32_t lane   = __lane_id();
+    const uint32_t wave   = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u)
+    {
+        const uint32_t i = base + lane;
+        float          q[9];
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+            q[d] = i < n ? q9[9 * (size_t)i + d] : 0.0f;
+        float out[CLPT_G_NOUT];
+        guide_forward_wave(w, q, out);
+        if (i < n)
+#pragma unroll
+            for (int k = 0; k < CLPT_G_NOUT; ++k)
+                out40[CLPT_G_NOUT * (size_t)i + k] = out[k];
+    }
+}
+
+/* ------------------------------------------------------------ training */
+
+__global__ __launch_bounds__(256) void k_guide_features(const GuideSample* s, const GuideCounters* gc, uint32_t N,
+                                                        __bf16* X)
+{
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N)
+        return;
+    const uint32_t cnt = min(gc->count, N);
+    float          v[9];
+    if (n < cnt)
+    {
+        const GuideSample g = s[n];
+        v[0] = g.a.x; v[1] = g.a.y; v[2] = g.a.z; v[3] = g.a.w;
+        v[4] = g.b.x; v[5] = g.b.y; v[6] = g.b.z; v[7] = g.b.w;
+        v[8] = g.c.x;
+    }
+    bf16x8* row = reinterpret_cast<bf16x8*>(X + (size_t)n * CLPT_G_IN);
+    float   o[8];
+#define CLPT_G_FEAT(G)                                                                                             \
+    {                                                                                                              \
+        if (n < cnt)                                                                                               \
+            guide_group<G>(v, o);                                                                                  \
+        else                                                                                                       \
+            for (int k = 0; k < 8; ++k)                                                                            \
+                o[k] = 0.0f;                                                                                       \
+        row[G] = guide_pack(o);                                                                                    \
+    }
+    CLPT_G_FEAT(0) CLPT_G_FEAT(1) CLPT_G_FEAT(2) CLPT_G_FEAT(3) CLPT_G_FEAT(4) CLPT_G_FEAT(5) CLPT_G_FEAT(6)
+    CLPT_G_FEAT(7) CLPT_G_FEAT(8) CLPT_G_FEAT(9) CLPT_G_FEAT(10) CLPT_G_FEAT(11) CLPT_G_FEAT(12) CLPT_G_FEAT(13)
+    CLPT_G_FEAT(14) CLPT_G_FEAT(15) CLPT_G_FEAT(16) CLPT_G_FEAT(17) CLPT_G_FEAT(18) CLPT_G_FEAT(19)
+    CLPT_G_FEAT(20) CLPT_G_FEAT(21) CLPT_G_FEAT(22) CLPT_G_FEAT(23) CLPT_G_FEAT(24) CLPT_G_FEAT(25)
+    CLPT_G_FEAT(26) CLPT_G_FEAT(27) CLPT_G_FEAT(28) CLPT_G_FEAT(29) CLPT_G_FEAT(30) CLPT_G_FEAT(31)
+#undef CLPT_G_FEAT
+}
+
+enum
+{
+    GEMM_RELU_BF16 = 0, /* C = bf16(max(y, 0)) */
+    GEMM_F32       = 1, /* C = y */
+    GEMM_MASK_BF16 = 2  /* C = bf16(mask > 0 ? y : 0) */
+};
+
+/* Y[n][j] = sum_k A[n][k] Bt[j][k]; A is N x K, Bt is J x K (both row-major
+ * bfloat16), one 32 x 32 tile of Y per wave. */
+template <int EPI>
+__global__ __launch_bounds__(256) void k_guide_gemm_nt(const __bf16* __restrict__ A, const __bf16* __restrict__ Bt,
+                                                       uint32_t N, uint32_t K, uint32_t J, void* C,
+                                                       const __bf16* __restrict__ mask)
+{
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t jt   = J / 32u;
+    const uint32_t tn = wave / jt, tj = wave % jt;
+    if (tn >= N / 32u)
+        return;
+    const uint32_t lane = __lane_id(), c = lane & 31u, h = lane >> 5;
+    const uint32_t n0 = 32u * tn, j0 = 32u * tj;
+    f32x16         acc = guide_zero16();
+    const __bf16*  pa  = A + (size_t)(n0 + c) * K + 8u * h;
+    const __bf16*  pb  = Bt + (size_t)(j0 + c) * K + 8u * h;
+    for (uint32_t k0 = 0; k0 < K; k0 += 16u)
+    {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(pa + k0);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(pb + k0);
+        acc            = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+    {
+        const size_t idx = (size_t)(n0 + 8u * (r / 4) + 4u * h + (r % 4)) * J + j0 + c;
+        const float  y   = acc[r];
+        if constexpr (EPI == GEMM_RELU_BF16)
+            static_cast<__bf16*>(C)[idx] = (__bf16)fmaxf(y, 0.0f);
+        else if constexpr (EPI == GEMM_F32)
+            static_cast<float*>(C)[idx] = y;
+        else
+            static_cast<__bf16*>(C)[idx] = (float)mask[idx] > 0.0f ? (__bf16)y : (__bf16)0.0f;
+    }
+}
+
+/* Partial weight gradient over one chunk of samples:
+ * P[chunk][k][m] = sum_{n in chunk} A[n][k] G[n][m]; A is N x K, G is N x 64.
+ * Written in the parameter layout (row k, column m) at param_off. */
+__global__ __launch_bounds__(256) void k_guide_gemm_tn(const __bf16* __restrict__ A, const __bf16* __restrict__ G,
+                                                       uint32_t K, uint32_t chunk, uint32_t nchunks, float* partial,
+                                                       uint32_t param_off)
+{
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t kt   = K / 32u;
+    const uint32_t tk = wave % kt, rem = wave / kt;
+    const uint32_t tm = rem % 2u, ch = rem / 2u;
+    if (ch >= nchunks)
+        return;
+    const uint32_t lane = __lane_id(), c = lane & 31u, h = lane >> 5;
+    const uint32_t k0 = 32u * tk, m0 = 32u * tm;
+    f32x16         acc = guide_zero16();
+    for (uint32_t n0 = ch * chunk; n0 < (ch + 1u) * chunk; n0 += 16u)
+    {
+        bf16x8 a, b;
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+        {
+            const size_t n = (size_t)(n0 + 8u * h + s);
+            a[s]           = A[n * K + k0 + c];
+            b[s]           = G[n * CLPT_G_HID + m0 + c];
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    float* P = partial + (size_t)ch * CLPT_G_PARAMS + param_off;
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+        P[(size_t)(k0 + 8u * (r / 4) + 4u * h + (r % 4)) * CLPT_G_HID + m0 + c] = acc[r];
+}
+
+/* Sum of the sample weights (one block, fixed order). */
+__global__ __launch_bounds__(1024) void k_guide_wsum(const GuideSample* s, GuideCounters* gc, uint32_t N)
+{
+    __shared__ double part[1024];
+    const uint32_t    cnt = min(gc->count, N);
+    double            acc = 0.0;
+    for (uint32_t n = threadIdx.x; n < cnt; n += blockDim.x)
+        acc += (double)s[n].d.x;
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t st = 512; st > 0; st >>= 1)
+    {
+        if (threadIdx.x < st)
+            part[threadIdx.x] += part[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        gc->wsum    = (float)part[0];
+        gc->trained = cnt;
+    }
+}
+
+/* Gradient of -w/W log p(wi) w.r.t. the 40 outputs of each sample (the
+ * vMF-mixture log-likelihood weighted by the sample's importance weight w
+ * and normalised by the batch weight sum W); rows beyond the sample count
+ * and output columns 40..63 are zero.  Per-block loss sums to loss_part. */
+__global__ __launch_bounds__(256) void k_guide_loss(const float* __restrict__ out, const GuideSample* s,
+                                                    const GuideCounters* gc, uint32_t N, __bf16* G5, float* loss_part)
+{
+    __shared__ float red[256];
+    const uint32_t   n   = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t   cnt = min(gc->count, N);
+    float            g[CLPT_G_HID];
+#pragma unroll
+    for (int k = 0; k < CLPT_G_HID; ++k)
+        g[k] = 0.0f;
+    float loss = 0.0f;
+    if (n < cnt && gc->wsum > 0.0f)
+    {
+        const GuideSample smp = s[n];
+        const float       wn  = smp.d.x / gc->wsum;
+        const f3          wi{smp.c.y, smp.c.z, smp.c.w};
+        float             o[CLPT_G_NOUT];
+#pragma unroll
+        for (int k = 0; k < CLPT_G_NOUT; ++k)
+            o[k] = out[(size_t)n * CLPT_G_HID + k];
+        VmfMix m;
+        vmf_mixture(o, m);
+        float v[CLPT_G_LOBES], p = 0.0f;
+#pragma unroll
+        for (int j = 0; j < CLPT_G_LOBES; ++j)
+        {
+            v[j] = m.norm[j] * clpt_expf(m.kappa[j] * (dot3(m.mu[j], wi) - 1.0f));
+            p += v[j];
+        }
+        loss = -wn * clpt_logf(fmaxf(p, 1e-30f));
+        const float ip = p > 0.0f ? 1.0f / p : 0.0f;
+#pragma unroll
+        for (int j = 0; j < CLPT_G_LOBES; ++j)
+        {
+            const float r   = v[j] * ip; /* responsibility of lobe j */
+            const float k   = m.kappa[j];
+            const float cmu = dot3(m.mu[j], wi);
+            g[5 * j]        = -wn * (r - m.w[j]);
+            const f3    raw{o[5 * j + 1], o[5 * j + 2], o[5 * j + 3]};
+            const float r2 = dot3(raw, raw);
+            if (r2 > 1e-12f)
+            {
+                const float s1 = -wn * r * k / sqrtf(r2);
+                g[5 * j + 1]   = s1 * (wi.x - cmu * m.mu[j].x);
+                g[5 * j + 2]   = s1 * (wi.y - cmu * m.mu[j].y);
+                g[5 * j + 3]   = s1 * (wi.z - cmu * m.mu[j].z);
+            }
+            const float sr = o[5 * j + 4];
+            if (sr > CLPT_G_KMIN && sr < CLPT_G_KMAX)
+            {
+                const float e  = clpt_expf(-2.0f * k);
+                const float dl = 1.0f / k - 2.0f * e / (1.0f - e) + (cmu - 1.0f);
+                g[5 * j + 4]   = -wn * r * dl * k;
+            }
+        }
+    }
+    if (n < N)
+    {
+        bf16x8* row = reinterpret_cast<bf16x8*>(G5 + (size_t)n * CLPT_G_HID);
+#pragma unroll
+        for (int q = 0; q < CLPT_G_HID / 8; ++q)
+            row[q] = guide_pack(g + 8 * q);
+    }
+    red[threadIdx.x] = loss;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1)
+    {
+        if (threadIdx.x < st)
+            red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        loss_part[blockIdx.x] = red[0];
+}
+
+/* bfloat16 copies of parameter p: W (row-major [in][out]), WT ([out][in])
+ * and the packed inference image. */
+__device__ __forceinline__ void guide_store_param(uint32_t p, float th, __bf16* W, __bf16* WT, __bf16* img)
+{
+    const uint32_t l   = p < (uint32_t)(CLPT_G_IN * CLPT_G_HID) ? 0u : 1u + (p - CLPT_G_IN * CLPT_G_HID) / (CLPT_G_HID * CLPT_G_HID);
+    const uint32_t off = guide_layer_offset((int)l), in = guide_layer_in((int)l);
+    const uint32_t k = (p - off) / CLPT_G_HID, m = (p - off) % CLPT_G_HID;
+    const __bf16   b = (__bf16)th;
+    W[p]                  = b;
+    WT[off + m * in + k]  = b;
+    if (l == 0)
+        img[m * CLPT_G_S0 + k] = b;
+    else
+        img[guide_img_offset((int)l) + m * CLPT_G_S1 + guide_perm(k)] = b;
+}
+
+__global__ void k_guide_pack(const float* params, __bf16* W, __bf16* WT, __bf16* img)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < CLPT_G_PARAMS)
+        guide_store_param(p, params[p], W, WT, img);
+}
+
+/* Adam on the float32 master weights (gradient = fixed-order sum of the
+ * split-N partials), then the bfloat16 copies.  No update when the batch
+ * held no samples. */
+__global__ void k_guide_adam(float* params, float* m1, float* m2, const float* partial, uint32_t nchunks,
+                             GuideCounters* gc, float lr, float beta1, float beta2, float eps, __bf16* W, __bf16* WT,
+                             __bf16* img, float* grad_out)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= CLPT_G_PARAMS || gc->trained == 0u)
+        return;
+    float g = 0.0f;
+    for (uint32_t c = 0; c < nchunks; ++c)
+        g += partial[(size_t)c * CLPT_G_PARAMS + p];
+    if (grad_out)
+        grad_out[p] = g;
+    const double t  = (double)(gc->steps + 1u);
+    const float  a  = beta1 * m1[p] + (1.0f - beta1) * g;
+    const float  b  = beta2 * m2[p] + (1.0f - beta2) * g * g;
+    m1[p]           = a;
+    m2[p]           = b;
+    const float  ah = (float)((double)a / (1.0 - pow((double)beta1, t)));
+    const float  bh = (float)((double)b / (1.0 - pow((double)beta2, t)));
+    const float  th = params[p] - lr * ah / (sqrtf(bh) + eps);
+    params[p]       = th;
+    guide_store_param(p, th, W, WT, img);
+}
+
+/* Close a step: loss of the batch, step counter, empty sample buffer. */
+__global__ __launch_bounds__(256) void k_guide_finish(GuideCounters* gc, const float* loss_part, uint32_t nparts)
+{
+    __shared__ float red[256];
+    float            acc = 0.0f;
+    for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x)
+        acc += loss_part[k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1)
+    {
+        if (threadIdx.x < st)
+            red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        if (gc->trained)
+        {
+            gc->loss = red[0];
+            gc->steps += 1u;
+            gc->samples_total += gc->trained;
+        }
+        gc->count = 0u;
+    }
+}
+
+/* ------------------------------------------------------------ gather */
+
+/* One thread per recorded path: walk its vertices from the deepest up,
+ * keeping the sum S of the contributions that arrived after each vertex.
+ * Vertex k yields the sample (query, sampled direction, weight) with weight
+ * lum(S * R) / p, R = f cos / T_next per channel and p the density the
+ * direction was drawn with: the single-sample estimate of
+ * lum(L_i f cos) / p of the radiance that came back along it. */
+__global__ __launch_bounds__(256) void k_guide_gather(const float4* V, const float4* E, uint32_t nslots,
+                                                      uint32_t maxd, GuideSample* out, uint32_t cap, GuideCounters* gc)
+{
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool     live = slot < nslots;
+    float          sr = 0.0f, sg = 0.0f, sb = 0.0f;
+    for (uint32_t j = maxd; j >= 1u; --j)
+    {
+        const size_t rec  = (size_t)(live ? slot : 0u) * (CLPT_MAXD + 1) + j;
+        float4       v3   = live ? V[4 * rec + 3] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float        w    = 0.0f;
+        if (v3.x > 0.0f)
+            w = lum3(sr * v3.y, sg * v3.z, sb * v3.w) / v3.x;
+        const bool   want = w > 0.0f && w < 3.0e38f;
+        const unsigned long long mask = __ballot(want);
+        if (mask)
+        {
+            const uint32_t leader = (uint32_t)(__ffsll((long long)mask) - 1);
+            uint32_t       base   = 0;
+            if (__lane_id() == leader)
+                base = atomicAdd(&gc->count, (uint32_t)__popcll(mask));
+            base = __shfl(base, (int)leader);
+            const uint32_t idx = base + (uint32_t)__popcll(mask & ((1ull << __lane_id()) - 1ull));
+            if (want && idx < cap)
+            {
+                GuideSample g;
+                g.a      = V[4 * rec + 0];
+                g.b      = V[4 * rec + 1];
+                g.c      = V[4 * rec + 2];
+                g.d      = make_float4(w, 0.0f, 0.0f, 0.0f);
+                out[idx] = g;
+            }
+        }
+        if (live)
+        {
+            const float4 e = E[rec];
+            sr += e.x;
+            sg += e.y;
+            sb += e.z;
+        }
+    }
+}
+
+/* ------------------------------------------------------------ launchers */
+
+namespace clpt
+{
+hipError_t launch_guide_infer(hipStream_t s, const __bf16* img, const float* q9, uint32_t n, float* out40,
+                              uint32_t blocks)
+{
+    hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(256), 0, s, img, q9, n, out40);
+    return hipGetLastError();
+}
+
+hipError_t launch_guide_pack(hipStream_t s, const float* params, __bf16* W, __bf16* WT, __bf16* img)
+{
+    hipLaunchKernelGGL(k_guide_pack, dim3(CLPT_G_PARAMS / 256), dim3(256), 0, s, params, W, WT, img);
+    return hipGetLastError();
+}
+
+hipError_t launch_guide_gather(hipStream_t s, const float4* V, const float4* E, uint32_t nslots, uint32_t maxd,
+                               GuideSample* out, uint32_t cap, GuideCounters* gc)
+{
+    if (nslots == 0 || maxd == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_guide_gather, dim3((nslots + 255) / 256), dim3(256), 0, s, V, E, nslots, maxd, out, cap, gc);
+    return hipGetLastError();
+}
+
+hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N, float lr, float* grad_out)
+{
+    if (N % 1024u != 0u || N == 0u)
+        return hipErrorInvalidValue;
+    const uint32_t nt = N / 32u;
+    hipLaunchKernelGGL(k_guide_wsum, dim3(1), dim3(1024), 0, s, b.samples, b.gc, N);
+    hipLaunchKernelGGL(k_guide_features, dim3(N / 256), dim3(256), 0, s, b.samples, b.gc, N, b.X);
+    /* forward: waves = tiles (N/32) x (64/32) */
+    const __bf16* in = b.X;
+    for (int l = 0; l < 4; ++l)
+    {
+        hipLaunchKernelGGL(k_guide_gemm_nt<GEMM_RELU_BF16>, dim3(nt * 2 / 4), dim3(256), 0, s, in,
+                           b.WT + guide_layer_offset(l), N, guide_layer_in(l), (uint32_t)CLPT_G_HID,
+                           (void*)b.act[l], (const __bf16*)nullptr);
+        in = b.act[l];
+    }
+    hipLaunchKernelGGL(k_guide_gemm_nt<GEMM_F32>, dim3(nt * 2 / 4), dim3(256), 0, s, in, b.WT + guide_layer_offset(4),
+                       N, (uint32_t)CLPT_G_HID, (uint32_t)CLPT_G_HID, (void*)b.out, (const __bf16*)nullptr);
+    hipLaunchKernelGGL(k_guide_loss, dim3(N / 256), dim3(256), 0, s, b.out, b.samples, b.gc, N, b.grad[4],
+                       b.loss_part);
+    /* backward: G[l] is the gradient w.r.t. the input of layer l+1 (N x 64) */
+    const uint32_t chunk = 1024u, nch = N / chunk;
+    for (int l = 4; l >= 0; --l)
+    {
+        const __bf16*  A  = l == 0 ? b.X : b.act[l - 1];
+        const uint32_t K  = guide_layer_in(l);
+        const uint32_t wv = (K / 32u) * 2u * nch;
+        hipLaunchKernelGGL(k_guide_gemm_tn, dim3((wv + 3) / 4), dim3(256), 0, s, A, b.grad[l], K, chunk, nch,
+                           b.partial, guide_layer_offset(l));
+        if (l > 0)
+            hipLaunchKernelGGL(k_guide_gemm_nt<GEMM_MASK_BF16>, dim3(nt * 2 / 4), dim3(256), 0, s, b.grad[l],
+                               b.W + guide_layer_offset(l), N, (uint32_t)CLPT_G_HID, (uint32_t)CLPT_G_HID,
+                               (void*)b.grad[l - 1], b.act[l - 1]);
+    }
+    hipLaunchKernelGGL(k_guide_adam, dim3(CLPT_G_PARAMS / 256), dim3(256), 0, s, b.params, b.m1, b.m2, b.partial,
+                       nch, b.gc, lr, 0.9f, 0.999f, 1e-8f, b.W, b.WT, b.img, grad_out);
+    hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(256), 0, s, b.gc, b.loss_part, N / 256);
+    return hipGetLastError();
+}
+} // namespace clpt

@@ -17,6 +17,7 @@
  * replacing one atomicAdd per item (soa.h:39-49).
  */
 #include "clpt_device.h"
+#include "clpt_guide.h"
 #include "clpt_kernels.h"
 
 
@@ -427,11 +428,33 @@ __device__ __forceinline__ void locate_path(const BatchTable& bt, const BatchRec
     sample             = e.sample_base + l / n;
 }
 
-__global__ __launch_bounds__(256) void k_camera(BatchTable bt, RayQueue q, BatchCounters* bc)
+/* Paths of a batch that record training vertices: one per block of
+ * tstride consecutive paths, at an offset hashed from the block and the
+ * batch (jittered-stratified over the batch's pixels and samples). */
+__device__ __forceinline__ uint32_t guide_tstride(const GuideView& gv, uint32_t npaths)
+{
+    return gv.nslots ? max(1u, (npaths + gv.nslots - 1u) / gv.nslots) : 0u;
+}
+
+__device__ __forceinline__ bool guide_records(uint32_t path, uint32_t tstr, uint32_t batch)
+{
+    if (!tstr)
+        return false;
+    uint32_t h = (path / tstr) * 0x9E3779B1u ^ (batch * 0x85EBCA77u + 0x165667B1u);
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return path % tstr == h % tstr;
+}
+
+__global__ __launch_bounds__(256) void k_camera(BatchTable bt, GuideView gv, RayQueue q, BatchCounters* bc)
 {
     const uint32_t gid    = blockIdx.x * blockDim.x + threadIdx.x;
     const BatchRec b      = bt.batches[*bt.cursor];
     const uint32_t npaths = b.npaths;
+    const uint32_t tstr   = guide_tstride(gv, npaths);
     if (gid == 0)
     {
         for (int k = 0; k < CLPT_MAXD + 2; ++k)
@@ -453,6 +476,16 @@ __global__ __launch_bounds__(256) void k_camera(BatchTable bt, RayQueue q, Batch
         q.o[i] = make_float4(o.x, o.y, o.z, 0.0f);
         q.d[i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));
         q.T[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        if (guide_records(i, tstr, *bt.cursor))
+        {
+            const size_t  rec = (size_t)(i / tstr) * (CLPT_MAXD + 1);
+            const float4 z   = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            for (uint32_t j = 1; j <= v.max_depth; ++j)
+            {
+                gv.V[4 * (rec + j) + 3] = z;
+                gv.E[rec + j]           = z;
+            }
+        }
     }
 }
 
@@ -620,8 +653,222 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
     }
 }
 
+/* Shading with the learned proposal (guided vertices: diffuse and
+ * conductor surfaces).  The continuation direction is drawn from the
+ * one-sample mixture c_bsdf * p_bsdf + (1 - c_bsdf) * p_guide; its density
+ * p_mix replaces the BSDF pdf everywhere the plain kernel uses it (path
+ * throughput f cos / p_mix, the escape MIS weight and the next-event MIS
+ * weight against the HDRI), so with c_bsdf > 0 the estimator stays
+ * unbiased.  The network runs once per wave of 64 rays (all lanes take part;
+ * its weights sit in LDS).  Paths selected for training record their
+ * guided vertices and the contributions that came back. */
+__global__ __launch_bounds__(256) void k_shade_guided(SceneView sc, BatchTable bt, GuideView gv, uint32_t depth,
+                                                      RayQueue qin, HitBuf hb, RayQueue qout, ShadowQueue sq,
+                                                      double* accum, BatchCounters* bc)
+{
+    __shared__ __bf16 wlds[CLPT_G_IMG];
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(gv.img);
+        uint4*       dst = reinterpret_cast<uint4*>(wlds);
+        for (uint32_t k = threadIdx.x; k < CLPT_G_IMG / 8; k += blockDim.x)
+            dst[k] = src[k];
+        __syncthreads();
+    }
+    const uint32_t n      = bc->rays[depth];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const bool     dome   = sc.dome_index >= 0;
+    const float    nl     = (float)sc.nemitters;
+    const BatchRec b      = bt.batches[*bt.cursor];
+    const uint32_t tstr   = guide_tstride(gv, b.npaths);
+    const float    cb     = gv.c_bsdf;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride)
+    {
+        const uint32_t i      = base + lane_id();
+        const bool     active = i < n;
+        float4         o4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d4 = o4, T4 = o4, h4 = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t       prim = 0u;
+        if (active)
+        {
+            o4   = qin.o[i];
+            d4   = qin.d[i];
+            T4   = qin.T[i];
+            h4   = hb.h[i];
+            prim = hb.prim[i];
+        }
+        const uint32_t path = __float_as_uint(d4.w);
+        uint32_t       vi = 0, vpix = 0, sample = 0;
+        if (active)
+            locate_path(bt, b, path, vi, vpix, sample);
+        const ViewRec& fr     = bt.views[vi];
+        const uint32_t pixel  = fr.film_offset + vpix;
+        const uint32_t px = vpix % fr.width, py = vpix / fr.width;
+        const uint32_t z      = (fr.lane << 20) | sample;
+        const f3       dir{d4.x, d4.y, d4.z};
+        f3             T{T4.x, T4.y, T4.z};
+        const bool     hit    = active && h4.x >= 0.0f;
+        const bool     vertex = hit && depth < fr.max_depth;
+        const bool     train  = active && guide_records(path, tstr, *bt.cursor);
+        const size_t   rec    = train ? (size_t)(path / tstr) * (CLPT_MAXD + 1) + depth : 0;
+        Surface        si{};
+        MaterialRec    mt{};
+        if (vertex)
+        {
+            HitRec hr;
+            hr.t    = h4.x;
+            hr.b1   = h4.y;
+            hr.b2   = h4.z;
+            hr.inst = __float_as_uint(h4.w);
+            hr.prim = prim;
+            surface_interaction(sc, hr, si);
+            mt = sc.mtl[si.material];
+        }
+        const bool guided = vertex && (mt.type == 0u || mt.type == 1u);
+        const f3   wo     = neg3(dir);
+        float      q[9];
+        {
+            GuideBox box;
+            for (int k = 0; k < 3; ++k)
+            {
+                box.lo[k]      = gv.lo[k];
+                box.inv_ext[k] = gv.inv_ext[k];
+            }
+            guide_query(box, si.p, si.f.n, wo, q);
+        }
+        VmfMix mix;
+        if (__ballot(guided) != 0ull) /* wave-uniform */
+        {
+            float out[CLPT_G_NOUT];
+            guide_forward_wave(wlds, q, out);
+            vmf_mixture(out, mix);
+        }
+        bool  want_shadow = false, want_next = false;
+        f3    Li{0.0f, 0.0f, 0.0f}, ldir{0.0f, 0.0f, 0.0f}, ndir{0.0f, 0.0f, 0.0f};
+        float npdf = 0.0f;
+        if (active && !hit)
+        {
+            if (dome)
+            {
+                const f3 le = dome_le(sc, dir);
+                f3       cc;
+                if (depth == 1)
+                    cc = le;
+                else
+                {
+                    const float lp = dome_pdf(sc, dir) / nl;
+                    const float w  = o4.w / (o4.w + lp);
+                    cc             = mul3(T, scl3(le, w));
+                }
+                accum_add(accum, pixel, cc);
+                if (train)
+                    gv.E[rec] = make_float4(cc.x, cc.y, cc.z, 0.0f);
+            }
+        }
+        else if (vertex)
+        {
+            const f3       woL = to_local(si.f, wo);
+            const uint32_t k0  = 2u + 4u * (depth - 1u);
+            {
+                const float lu = rng_draw(px, py, z, k0 + 1u), lv = rng_draw(px, py, z, k0 + 2u);
+                float       lpdf  = 0.0f;
+                uint32_t    which = 0;
+                const f3    value = sample_emitters(sc, lu, lv, &ldir, &lpdf, &which);
+                if (nonzero3(value) && lpdf > 0.0f)
+                {
+                    const f3 wiL = to_local(si.f, ldir);
+                    const f3 f   = bsdf_eval(mt, wiL, woL);
+                    if (nonzero3(f))
+                    {
+                        float weight = 1.0f;
+                        if (sc.emitters[which].type == 1u)
+                        {
+                            float ps = bsdf_pdf(mt, wiL, woL);
+                            if (guided)
+                                ps = cb * ps + (1.0f - cb) * vmf_mixture_pdf(mix, ldir);
+                            const float lp = lpdf / nl;
+                            weight         = lp / (lp + ps);
+                        }
+                        const float cosv = fabsf(wiL.z);
+                        Li          = f3{T.x * value.x * f.x * cosv * weight / lpdf, T.y * value.y * f.y * cosv * weight / lpdf,
+                                T.z * value.z * f.z * cosv * weight / lpdf};
+                        want_shadow = true;
+                    }
+                }
+            }
+            const float bu = rng_draw(px, py, z, k0 + 3u), bv = rng_draw(px, py, z, k0 + 4u);
+            const float ul = rng_draw(px, py, z, CLPT_EXTRA_DRAW(depth, CLPT_SLOT_LOBE));
+            f3          wiL{0.0f, 0.0f, 0.0f}, f{0.0f, 0.0f, 0.0f};
+            float       pdf = 0.0f;
+            if (!guided)
+                f = bsdf_sample(mt, woL, bu, bv, ul, &wiL, &pdf);
+            else
+            {
+                const float ut = rng_draw(px, py, z, CLPT_EXTRA_DRAW(depth, CLPT_SLOT_GUIDE));
+                f3          wi;
+                float       bp = 0.0f;
+                if (ut < cb)
+                {
+                    f  = bsdf_sample(mt, woL, bu, bv, ul, &wiL, &bp);
+                    wi = to_world(si.f, wiL);
+                }
+                else
+                {
+                    const float us = rng_draw(px, py, z, CLPT_EXTRA_DRAW(depth, CLPT_SLOT_LOBESEL));
+                    wi             = vmf_mixture_sample(mix, us, bu, bv);
+                    wiL            = to_local(si.f, wi);
+                    f              = bsdf_eval(mt, wiL, woL);
+                    bp             = bsdf_pdf(mt, wiL, woL);
+                }
+                pdf = cb * bp + (1.0f - cb) * vmf_mixture_pdf(mix, wi);
+            }
+            if (nonzero3(f) && pdf > 0.0f)
+            {
+                const float cosv = fabsf(wiL.z);
+                const f3    fc{f.x * cosv, f.y * cosv, f.z * cosv};
+                T         = f3{T.x * (fc.x / pdf), T.y * (fc.y / pdf), T.z * (fc.z / pdf)};
+                ndir      = to_world(si.f, wiL);
+                npdf      = pdf;
+                want_next = true;
+                if (fr.rr_depth > 0u && depth >= fr.rr_depth)
+                {
+                    const float qq = fminf(1.0f, fmaxf(T.x, fmaxf(T.y, T.z)));
+                    const float ur = rng_draw(px, py, z, CLPT_EXTRA_DRAW(depth, CLPT_SLOT_RR));
+                    if (!(ur < qq))
+                        want_next = false;
+                    else
+                        T = f3{T.x / qq, T.y / qq, T.z / qq};
+                }
+                if (train && guided && want_next)
+                {
+                    gv.V[4 * rec + 0] = make_float4(q[0], q[1], q[2], q[3]);
+                    gv.V[4 * rec + 1] = make_float4(q[4], q[5], q[6], q[7]);
+                    gv.V[4 * rec + 2] = make_float4(q[8], ndir.x, ndir.y, ndir.z);
+                    gv.V[4 * rec + 3] = make_float4(pdf, T.x > 0.0f ? fc.x / T.x : 0.0f, T.y > 0.0f ? fc.y / T.y : 0.0f,
+                                                    T.z > 0.0f ? fc.z / T.z : 0.0f);
+                }
+            }
+        }
+        wave_count(&bc->hits[depth], hit);
+        const uint32_t ss = wave_push(&bc->shadow[depth], want_shadow);
+        if (want_shadow)
+        {
+            sq.o[ss]   = make_float4(si.p.x, si.p.y, si.p.z, Li.x);
+            sq.d[ss]   = make_float4(ldir.x, ldir.y, ldir.z, Li.y);
+            sq.x[ss]   = make_float2(Li.z, __uint_as_float(pixel));
+            sq.tag[ss] = train ? (uint32_t)rec : 0xffffffffu;
+        }
+        const uint32_t ns = wave_push(&bc->rays[depth + 1], want_next);
+        if (want_next)
+        {
+            qout.o[ns] = make_float4(si.p.x, si.p.y, si.p.z, npdf);
+            qout.d[ns] = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(path));
+            qout.T[ns] = make_float4(T.x, T.y, T.z, 0.0f);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_shadow(SceneView sc, ShadowQueue sq, uint32_t depth, double* accum,
-                                                BatchCounters* bc, uint2* gstack, unsigned long long* ctr)
+                                                BatchCounters* bc, uint2* gstack, unsigned long long* ctr,
+                                                float4* guide_E)
 {
     CLPT_DECLARE_STACK(st, gstack);
     uint32_t       nn = 0, nt = 0, nvis = 0;
@@ -644,6 +891,12 @@ __global__ __launch_bounds__(256) void k_shadow(SceneView sc, ShadowQueue sq, ui
                 const float2 x = sq.x[i];
                 accum_add(accum, __float_as_uint(x.y), f3{o.w, d.w, x.x});
                 ++nvis;
+                if (sq.tag)
+                {
+                    const uint32_t tg = sq.tag[i];
+                    if (tg != 0xffffffffu)
+                        guide_E[tg] = make_float4(o.w, d.w, x.x, 0.0f);
+                }
             }
             i += st.gstride;
             active = i < n;
@@ -967,9 +1220,10 @@ static inline uint32_t grid_for(uint64_t items, uint32_t block, uint32_t max_blo
     return (uint32_t)g;
 }
 
-hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const RayQueue& q, BatchCounters* bc, uint32_t blocks)
+hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const GuideView& gv, const RayQueue& q, BatchCounters* bc,
+                         uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_camera, dim3(blocks), dim3(256), 0, s, bt, q, bc);
+    hipLaunchKernelGGL(k_camera, dim3(blocks), dim3(256), 0, s, bt, gv, q, bc);
     return hipGetLastError();
 }
 
@@ -988,10 +1242,18 @@ hipError_t launch_shade(hipStream_t s, const SceneView& sc, const BatchTable& bt
     return hipGetLastError();
 }
 
-hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& sq, uint32_t depth, double* accum,
-                         BatchCounters* bc, uint2* gstack, unsigned long long* ctr, uint32_t blocks)
+hipError_t launch_shade_guided(hipStream_t s, const SceneView& sc, const BatchTable& bt, const GuideView& gv,
+                               uint32_t depth, const RayQueue& qin, const HitBuf& hb, const RayQueue& qout,
+                               const ShadowQueue& sq, double* accum, BatchCounters* bc, uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(256), 0, s, sc, sq, depth, accum, bc, gstack, ctr);
+    hipLaunchKernelGGL(k_shade_guided, dim3(blocks), dim3(256), 0, s, sc, bt, gv, depth, qin, hb, qout, sq, accum, bc);
+    return hipGetLastError();
+}
+
+hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& sq, uint32_t depth, double* accum,
+                         BatchCounters* bc, uint2* gstack, unsigned long long* ctr, float4* guide_E, uint32_t blocks)
+{
+    hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(256), 0, s, sc, sq, depth, accum, bc, gstack, ctr, guide_E);
     return hipGetLastError();
 }
 
@@ -1065,8 +1327,10 @@ uint32_t resident_blocks_per_cu(int which)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace, 256, 0);
     else if (which == 1)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shadow, 256, 0);
-    else
+    else if (which == 2)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shade, 256, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shade_guided, 256, 0);
     return (e == hipSuccess && n > 0) ? (uint32_t)n : 4u;
 }
 

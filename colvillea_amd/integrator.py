@@ -351,6 +351,54 @@ class Integrator:
         self._check(self._lib.clhip_view_latency(self._ctx, out.ctypes.data_as(C.POINTER(C.c_double)), len(out)))
         return out
 
+    # -- learned directional proposal (no reference counterpart)
+    GUIDE_FIELDS = ("enable", "bsdf_fraction", "train", "record_paths", "batch_size", "learning_rate", "max_steps")
+
+    def setGuide(self, enable: bool = True, **fields):
+        """Turns the learned proposal on or off; other fields as in
+        clhip_guide (zero = default)."""
+        bad = set(fields) - set(self.GUIDE_FIELDS)
+        if bad:
+            raise TypeError(f"unknown guide fields {sorted(bad)}")
+        g = capi.clhip_guide()
+        g.enable = 1 if enable else 0
+        for k, v in fields.items():
+            setattr(g, k, v)
+        self._check(self._lib.clhip_set_guide(self._ctx, C.byref(g)))
+
+    def guideReset(self, seed: int = 1):
+        self._check(self._lib.clhip_guide_reset(self._ctx, int(seed)))
+
+    def guideWeights(self) -> np.ndarray:
+        w = np.empty(capi.GUIDE_PARAMS, np.float32)
+        self._check(self._lib.clhip_guide_get_weights(self._ctx, capi.fptr(w), len(w)))
+        return w
+
+    def setGuideWeights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(self._lib.clhip_guide_set_weights(self._ctx, capi.fptr(w), len(w)))
+
+    def guideStats(self) -> dict:
+        st = capi.clhip_guide_stats()
+        self._check(self._lib.clhip_guide_stats_get(self._ctx, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in capi.clhip_guide_stats._fields_}
+
+    def guideInfer(self, q) -> np.ndarray:
+        """Network outputs (n x 40) for queries (n x 9)."""
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, 9)
+        out = np.empty((q.shape[0], capi.GUIDE_OUTPUTS), np.float32)
+        self._check(self._lib.clhip_guide_infer(self._ctx, capi.fptr(q), q.shape[0], capi.fptr(out)))
+        return out
+
+    def guideTrain(self, samples, want_grad: bool = False):
+        """One Adam step on samples (n x 13: query, direction, weight);
+        returns (loss, gradient or None)."""
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
+        g = np.empty(capi.GUIDE_PARAMS, np.float32) if want_grad else None
+        loss = C.c_float()
+        self._check(self._lib.clhip_guide_train(self._ctx, capi.fptr(s), s.shape[0], capi.fptr(g), C.byref(loss)))
+        return loss.value, g
+
     # -- GL interop (integrator.h:66-71) has no counterpart without a display
     def registerFramebuffer(self, glTexture):
         raise NotImplementedError("GL interop is not available on MI355X; use readbackFramebuffer()")

@@ -306,3 +306,106 @@ def make_c2_views(seed: int = 3, n_views: int = 256, side=(64, 512), spp=(4, 32)
         fov = rng.uniform(30.0, 60.0)
         views.append(View(s, s, n, d, look_at(eye, at, (0, 1, 0), fov, s, s), rr_depth, i))
     return views
+
+
+def grid_rect(origin, u, v, nu: int, nv: int, normal, hole=None) -> TriangleMesh:
+    """Rectangle origin + s u + t v (s, t in [0, 1]) as nu x nv cells of two
+    triangles each, wound and shaded with `normal`; cells whose centre lies
+    in hole = (s0, s1, t0, t1) are left out."""
+    origin, u, v = (np.asarray(a, np.float64) for a in (origin, u, v))
+    s = np.linspace(0.0, 1.0, nu + 1)
+    t = np.linspace(0.0, 1.0, nv + 1)
+    S, T = np.meshgrid(s, t, indexing="ij")
+    verts = (origin + S[..., None] * u + T[..., None] * v).reshape(-1, 3)
+    idx = []
+    n = np.asarray(normal, np.float64)
+    flip = np.dot(np.cross(u, v), n) < 0
+    for i in range(nu):
+        for j in range(nv):
+            if hole is not None:
+                cs, ct = (i + 0.5) / nu, (j + 0.5) / nv
+                if hole[0] <= cs <= hole[1] and hole[2] <= ct <= hole[3]:
+                    continue
+            a, b, c, d = i * (nv + 1) + j, (i + 1) * (nv + 1) + j, (i + 1) * (nv + 1) + j + 1, i * (nv + 1) + j + 1
+            idx += [(a, c, b), (a, d, c)] if flip else [(a, b, c), (a, c, d)]
+    nrm = np.tile(n.astype(np.float32), (verts.shape[0], 1))
+    return TriangleMesh(verts.astype(np.float32), np.array(idx, np.uint32), normals=nrm)
+
+
+def merge_meshes(meshes) -> TriangleMesh:
+    vs, ids, ns, off = [], [], [], 0
+    for m in meshes:
+        vs.append(m.vertices)
+        ids.append(m.triangles + off)
+        ns.append(m.normals)
+        off += m.vertices.shape[0]
+    return TriangleMesh(np.concatenate(vs), np.concatenate(ids).astype(np.uint32), normals=np.concatenate(ns))
+
+
+def rotate_hdri_to(img: np.ndarray, target_phi: float) -> np.ndarray:
+    """Rotate an HDRI about the vertical axis (a whole-texel column shift)
+    so that its brightest texel's azimuth moves to target_phi, where a
+    direction is (sin th sin ph, cos th, sin th cos ph)."""
+    h, w = img.shape[:2]
+    lum = img[..., :3].sum(-1)
+    y, x = np.unravel_index(np.argmax(lum), lum.shape)
+    d = hdri_directions(w, h).reshape(h, w, 3)[y, x]
+    phi = math.atan2(d[0], d[2])
+    shift = int(round((target_phi - phi) / (2 * math.pi) * w))
+    # the dome's u grows with -phi or +phi depending on the mapping: pick the
+    # shift direction that lands the peak on target_phi
+    for sgn in (1, -1):
+        r = np.roll(img, sgn * shift, axis=1)
+        yy, xx = np.unravel_index(np.argmax(r[..., :3].sum(-1)), lum.shape)
+        dd = hdri_directions(w, h).reshape(h, w, 3)[yy, xx]
+        err = abs((math.atan2(dd[0], dd[2]) - target_phi + math.pi) % (2 * math.pi) - math.pi)
+        if err <= 2 * math.pi / w * 1.5:
+            return np.ascontiguousarray(r)
+    raise RuntimeError("could not rotate the HDRI peak to the requested azimuth")
+
+
+def make_c3(seed: int = 4, width: int = 1920, height: int = 1080, spp: int = 256, cell: float = 1.0 / 16.0,
+            n_props: int = 50, prop_subdiv: int = 3, hdri_w: int = 2048, hdri_h: int = 1024, hdri_seed: int = 2,
+            max_depth: int = 10) -> SceneDesc:
+    """A closed 10 x 10 x 4 room (walls, floor and ceiling as grids of
+    `cell`-sized squares, normals facing in) with a 1 x 1 window in the wall
+    z = -5, 50 icosphere props (1/3 diffuse, GGX conductor, GGX dielectric)
+    and no light but the 1080p scene's HDRI (log-normal, hot spot at 1e4 x
+    the mean), rotated about the vertical so its hot spot shines in through
+    the window.  1920x1080, 256 spp, depth 10."""
+    rng = np.random.default_rng(seed)
+    nx, ny = int(round(10.0 / cell)), int(round(4.0 / cell))
+    hole = (0.45, 0.55, 0.5, 0.75)  # x in [-0.5, 0.5], y in [2, 3]
+    walls = [
+        grid_rect((-5, 0, -5), (10, 0, 0), (0, 0, 10), nx, nx, (0, 1, 0)),      # floor
+        grid_rect((-5, 4, -5), (10, 0, 0), (0, 0, 10), nx, nx, (0, -1, 0)),     # ceiling
+        grid_rect((-5, 0, -5), (10, 0, 0), (0, 4, 0), nx, ny, (0, 0, 1), hole),  # window wall
+        grid_rect((-5, 0, 5), (10, 0, 0), (0, 4, 0), nx, ny, (0, 0, -1)),
+        grid_rect((-5, 0, -5), (0, 0, 10), (0, 4, 0), nx, ny, (1, 0, 0)),
+        grid_rect((5, 0, -5), (0, 0, 10), (0, 4, 0), nx, ny, (-1, 0, 0)),
+    ]
+    room = merge_meshes(walls)
+    sphere = icosphere(prop_subdiv)
+    r = rng.uniform(0.2, 0.6, n_props)
+    cx = rng.uniform(-4.3, 4.3, n_props)
+    cz = rng.uniform(-4.3, 4.3, n_props)
+    cy = r * rng.uniform(1.0, 3.0, n_props)
+    materials = [DiffuseMtl((0.7, 0.68, 0.62))]
+    for i in range(n_props):
+        kind = i % 3
+        if kind == 0:
+            materials.append(DiffuseMtl(tuple(rng.uniform(0.2, 0.8, 3))))
+        elif kind == 1:
+            eta, k = CONDUCTOR_TABLE[int(rng.integers(len(CONDUCTOR_TABLE)))]
+            materials.append(MetalMtl((1.0, 1.0, 1.0), float(np.sqrt(rng.uniform(0.05, 0.5))), eta, k))
+        else:
+            materials.append(GlassMtl(float(np.sqrt(rng.uniform(0.01, 0.3))), 1.5))
+    xf = np.stack([IDENTITY] + [translate_scale((x, y, z), rr) for x, y, z, rr in zip(cx, cy, cz, r)])
+    hrng = np.random.default_rng(hdri_seed)
+    hdri = rotate_hdri_to(lognormal_hdri(hdri_w, hdri_h, 1.5, hrng), math.pi)  # peak towards -z
+    dome = HDRIDome(ImageTexture2D(hdri))
+    cam = look_at((0.0, 1.7, 4.6), (0.0, 1.0, -5.0), (0, 1, 0), 65.0, width, height)
+    ntri = room.triangles.shape[0] + n_props * sphere.triangles.shape[0]
+    return SceneDesc("C3", [room, sphere], [0] + [1] * n_props, list(range(n_props + 1)), xf.astype(np.float32),
+                     materials, [dome], cam, width, height, spp, max_depth, 0,
+                     info={"seed": seed, "triangles": int(ntri)})

@@ -280,6 +280,50 @@ int clhip_view_sums(clhip_ctx* ctx, uint32_t view, double* rgb, size_t bytes);
  * batch holding view v's last path finished, per view (synchronises). */
 int clhip_view_latency(clhip_ctx* ctx, double* ms, uint32_t num_views);
 
+/* ---- learned directional proposal (path guiding).  No reference
+ *      counterpart.  A 256-64-64-64-64-64 MLP (bfloat16 on the matrix
+ *      cores) maps (position in the scene box, normal, outgoing direction)
+ *      to a mixture of 8 von Mises-Fisher lobes; at diffuse and conductor
+ *      vertices the continuation is drawn from
+ *      bsdf_fraction * p_bsdf + (1 - bsdf_fraction) * p_mlp and weighted by
+ *      that mixture density (one-sample MIS), which also enters the MIS
+ *      weights against HDRI sampling.  Training is online: recorded paths
+ *      give (query, direction, lum(L f cos) / p) samples, and one Adam step
+ *      on the negative log-likelihood follows every batch. ---- */
+typedef struct
+{
+    uint32_t enable;        /* 0: plain tracer */
+    float    bsdf_fraction; /* (0, 1]: probability of BSDF sampling at guided vertices */
+    uint32_t train;         /* 1: one training step after every batch */
+    uint32_t record_paths;  /* paths per batch that record training vertices (0 = 32768) */
+    uint32_t batch_size;    /* samples per training step, multiple of 1024 (0 = 65536) */
+    float    learning_rate; /* Adam (0 = 1e-3) */
+    uint32_t max_steps;     /* stop training after this many steps (0 = never) */
+} clhip_guide;
+
+typedef struct
+{
+    uint64_t steps;         /* Adam steps taken since the last reset */
+    uint64_t samples;       /* samples trained on since the last reset */
+    uint32_t last_count;    /* samples in the last step */
+    float    last_loss;     /* weighted negative log-likelihood of the last step (per unit weight) */
+    double   train_ms;      /* summed device time of the training steps of the last render */
+} clhip_guide_stats;
+
+int clhip_set_guide(clhip_ctx* ctx, const clhip_guide* guide);
+/* Fresh weights (He-uniform from `seed`, output layer scaled by 0.1) and
+ * Adam state. */
+int clhip_guide_reset(clhip_ctx* ctx, uint64_t seed);
+/* The float32 master weights (32768 values, layer l row-major [in][out]). */
+int clhip_guide_get_weights(clhip_ctx* ctx, float* weights, uint32_t n);
+int clhip_guide_set_weights(clhip_ctx* ctx, const float* weights, uint32_t n);
+int clhip_guide_stats_get(clhip_ctx* ctx, clhip_guide_stats* out);
+/* Network outputs (40 per query) for n queries of 9 floats. */
+int clhip_guide_infer(clhip_ctx* ctx, const float* q9, uint32_t n, float* out40);
+/* One Adam step on n samples of 13 floats (query 9, direction 3, weight);
+ * grad (32768 floats, optional) receives the summed gradient. */
+int clhip_guide_train(clhip_ctx* ctx, const float* samples13, uint32_t n, float* grad, float* loss);
+
 /* Measurement helper with no reference counterpart: `iters` launches of a
  * plain 16-byte streaming copy of `bytes` bytes; returns ms per copy.  Used
  * as the known-traffic kernel for HBM byte counters and the copy ceiling. */

@@ -1,0 +1,213 @@
+"""Float64 CPU restatement of the learned directional proposal.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ (and nothing shipped).  The
+reference renderer has no learned guiding, so nothing here restates a
+reference file; the definitions follow the renderer's own specification
+(colvillea_amd/csrc/clpt_guide.h, guide.hip):
+
+  encoding   9 inputs -> 256 features: group g < 27 holds, for input g // 3
+             and frequencies f = 4 (g % 3) + 0..3, the pairs
+             sin(2^f pi v), cos(2^f pi v); group 27 the raw inputs 0..7;
+             group 28 raw input 8, then 1, then zeros; groups 29..31 zeros.
+  network    256 -> 64 -> 64 -> 64 -> 64 -> 64, ReLU between, no biases;
+             weights W_l[in][out], outputs 0..39 used.
+  mixture    lobe j: logit out[5j], mean out[5j+1..3] (normalised),
+             kappa = exp(clamp(out[5j+4], -4, 8)); weights softmax(logits);
+             vMF_j(w) = kappa / (2 pi (1 - e^{-2 kappa})) e^{kappa (mu.w - 1)}.
+  loss       L = -sum_n (w_n / sum w) log p(wi_n).
+  Adam       beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected.
+
+Pins (tests/test_guide_oracle.py): central finite differences of the loss
+gradient; vMF normalisation by quadrature; the mixture integrates to 1;
+chi-square of mixture sampling against the pdf; Adam's first step in closed
+form; encoding values at known arguments; bfloat16 rounding of known bit
+patterns.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+IN, HID, NOUT, LOBES = 256, 64, 40, 8
+PARAMS = IN * HID + 4 * HID * HID
+KMIN, KMAX = -4.0, 8.0
+
+
+def bf16(x):
+    """Round to bfloat16 (nearest, ties to even) via float32, as float64."""
+    f = np.asarray(x, np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def split_params(p):
+    """Flat parameter vector -> [W0 (256x64), W1..W4 (64x64)]."""
+    p = np.asarray(p, np.float64)
+    Ws = [p[:IN * HID].reshape(IN, HID)]
+    off = IN * HID
+    for _ in range(4):
+        Ws.append(p[off:off + HID * HID].reshape(HID, HID))
+        off += HID * HID
+    return Ws
+
+
+def join_params(Ws):
+    return np.concatenate([w.reshape(-1) for w in Ws])
+
+
+def encode(q):
+    q = np.asarray(q, np.float64)
+    N = q.shape[0]
+    X = np.zeros((N, IN))
+    for g in range(27):
+        d, fb = divmod(g, 3)
+        for k in range(4):
+            a = np.pi * (2.0 ** (4 * fb + k)) * q[:, d]
+            X[:, 8 * g + 2 * k] = np.sin(a)
+            X[:, 8 * g + 2 * k + 1] = np.cos(a)
+    X[:, 216:224] = q[:, :8]
+    X[:, 224] = q[:, 8]
+    X[:, 225] = 1.0
+    return X
+
+
+def forward(Ws, X, bf=False):
+    """Outputs (N x 64) and the per-layer inputs (post-ReLU, as used)."""
+    r = bf16 if bf else (lambda a: a)
+    A = [r(X)]
+    h = A[0]
+    for l in range(5):
+        z = h @ r(Ws[l])
+        if l < 4:
+            h = r(np.maximum(z, 0.0))
+            A.append(h)
+        else:
+            return z, A
+
+
+def mixture(out):
+    out = np.asarray(out, np.float64)[:, :NOUT].reshape(-1, LOBES, 5)
+    a = out[:, :, 0]
+    a = a - a.max(axis=1, keepdims=True)
+    w = np.exp(a)
+    w /= w.sum(axis=1, keepdims=True)
+    raw = out[:, :, 1:4]
+    nr = np.linalg.norm(raw, axis=2, keepdims=True)
+    mu = np.where(nr ** 2 > 1e-12, raw / np.maximum(nr, 1e-300), np.array([0.0, 0.0, 1.0]))
+    kappa = np.exp(np.clip(out[:, :, 4], KMIN, KMAX))
+    norm = w * kappa / (2.0 * np.pi * (-np.expm1(-2.0 * kappa)))
+    return w, mu, kappa, norm
+
+
+def vmf_pdf(mu, kappa, d):
+    return kappa / (2.0 * np.pi * (-np.expm1(-2.0 * kappa))) * np.exp(kappa * (np.sum(mu * d, -1) - 1.0))
+
+
+def mix_pdf(out, d):
+    """Mixture density of each row's mixture at direction d (N x 3)."""
+    w, mu, kappa, norm = mixture(out)
+    cosv = np.einsum("njk,nk->nj", mu, np.asarray(d, np.float64))
+    return np.sum(norm * np.exp(kappa * (cosv - 1.0)), axis=1)
+
+
+def frame(n):
+    """Orthonormal t, b around unit n (any fixed completion)."""
+    n = np.asarray(n, np.float64)
+    a = np.where(np.abs(n[..., :1]) > 0.9, np.array([0.0, 1.0, 0.0]), np.array([1.0, 0.0, 0.0]))
+    t = np.cross(n, a)
+    t /= np.linalg.norm(t, axis=-1, keepdims=True)
+    b = np.cross(n, t)
+    return t, b
+
+
+def mix_sample(out, usel, u1, u2):
+    """Lobe by usel over the weights, then the Jakob (2012) inversion."""
+    w, mu, kappa, _ = mixture(out)
+    c = np.cumsum(w, axis=1)
+    j = np.minimum((usel[:, None] >= c[:, :-1]).sum(axis=1), LOBES - 1)
+    idx = np.arange(len(j))
+    m, k = mu[idx, j], kappa[idx, j]
+    ct = np.clip(1.0 + np.log(u1 + (1.0 - u1) * np.exp(-2.0 * k)) / k, -1.0, 1.0)
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    t, b = frame(m)
+    ph = 2.0 * np.pi * u2
+    return st[:, None] * (np.cos(ph)[:, None] * t + np.sin(ph)[:, None] * b) + ct[:, None] * m
+
+
+def loss_and_grad(params, q, wi, weight, bf=False):
+    """L = -sum_n w_n / W log p(wi_n) and dL/dparams (flat), float64 (bf:
+    the same with the GPU's bfloat16 rounding points mirrored)."""
+    Ws = split_params(params)
+    r = bf16 if bf else (lambda a: a)
+    X = encode(q)
+    out, A = forward(Ws, X, bf)
+    wn = np.asarray(weight, np.float64) / np.sum(weight)
+    wi = np.asarray(wi, np.float64)
+    N = out.shape[0]
+    o = out[:, :NOUT].reshape(N, LOBES, 5)
+    w, mu, kappa, norm = mixture(out)
+    cosv = np.einsum("njk,nk->nj", mu, wi)
+    v = norm * np.exp(kappa * (cosv - 1.0))
+    p = v.sum(axis=1)
+    loss = float(-np.sum(wn * np.log(np.maximum(p, 1e-300))))
+    rr = v / p[:, None]
+    G = np.zeros((N, LOBES, 5))
+    G[:, :, 0] = -wn[:, None] * (rr - w)
+    raw = o[:, :, 1:4]
+    nr = np.linalg.norm(raw, axis=2)
+    ok = nr ** 2 > 1e-12
+    s1 = np.where(ok, -wn[:, None] * rr * kappa / np.maximum(nr, 1e-300), 0.0)
+    G[:, :, 1:4] = s1[:, :, None] * (wi[:, None, :] - cosv[:, :, None] * mu)
+    sr = o[:, :, 4]
+    inside = (sr > KMIN) & (sr < KMAX)
+    e = np.exp(-2.0 * kappa)
+    dl = 1.0 / kappa - 2.0 * e / (-np.expm1(-2.0 * kappa)) + (cosv - 1.0)
+    G[:, :, 4] = np.where(inside, -wn[:, None] * rr * dl * kappa, 0.0)
+    g = np.zeros((N, HID))
+    g[:, :NOUT] = G.reshape(N, NOUT)
+    g = r(g)
+    grads = [None] * 5
+    for l in range(4, -1, -1):
+        grads[l] = A[l].T @ g
+        if l > 0:
+            g = r((g @ r(Ws[l]).T) * (A[l] > 0))
+    return loss, join_params(grads)
+
+
+def adam(params, m, v, t, g, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    mh = m / (1 - b1 ** t)
+    vh = v / (1 - b2 ** t)
+    return params - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def init_params(seed=1):
+    """He-uniform weights from a SplitMix64 stream (the library's
+    clhip_guide_reset draws the same stream in the same order)."""
+    st = np.uint64(seed)
+    out = np.zeros(PARAMS)
+    vals = []
+    M = (1 << 64) - 1
+    s = int(seed)
+    n = IN * HID + 4 * HID * HID
+    for _ in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & M
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        vals.append((z >> 11) * (1.0 / 9007199254740992.0))
+    u = np.array(vals)
+    off = 0
+    for l in range(5):
+        fan = IN if l == 0 else HID
+        a = np.sqrt(6.0 / fan) * (0.1 if l == 4 else 1.0)
+        blk = (2.0 * u[off:off + fan * HID] - 1.0) * a
+        blk = blk.reshape(fan, HID)
+        if l == 4:
+            blk[:, NOUT:] = 0.0
+        out[off:off + fan * HID] = blk.reshape(-1)
+        off += fan * HID
+    del st
+    return out.astype(np.float32).astype(np.float64)

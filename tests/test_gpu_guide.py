@@ -1,0 +1,209 @@
+"""The learned directional proposal on the GPU against the float64 oracle
+(oracle/guide_ref.py), and its effect on rendering.
+
+  - weights: the library's initial weights equal the oracle's SplitMix64
+    He-uniform draw bit for bit (float32);
+  - inference: the fused MFMA forward pass equals the oracle's forward pass
+    with the same bfloat16 rounding points to float32-accumulation accuracy,
+    and the float64 network to bfloat16 accuracy;
+  - training: the summed gradient of one step equals the oracle's analytic
+    gradient (relative error <= 1e-2 against the bfloat16-mirrored oracle,
+    <= 6e-2 against pure float64), and the Adam update equals the oracle's
+    Adam applied to that gradient;
+  - learning: on samples from a known direction distribution the loss falls
+    to within 0.25 nats of that distribution's entropy;
+  - rendering: guided and plain estimators have the same mean, and after
+    online training the guided tracer's error at equal samples is below the
+    plain tracer's (the gain over the method without the network).
+"""
+import numpy as np
+import pytest
+
+import guide_ref as G
+from colvillea_amd import Integrator, IntegratorType, scenes, upload_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_queries(rng, n):
+    q = np.empty((n, 9))
+    q[:, :3] = rng.uniform(0, 1, (n, 3))
+    for s in (3, 6):
+        v = rng.normal(size=(n, 3))
+        q[:, s:s + 3] = v / np.linalg.norm(v, axis=1, keepdims=True)
+    return q.astype(np.float32).astype(np.float64)
+
+
+def rand_dirs(rng, n):
+    v = rng.normal(size=(n, 3))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32).astype(np.float64)
+
+
+@pytest.fixture(scope="module")
+def integ():
+    it = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, 16, 16)
+    yield it
+    it.close()
+
+
+def test_initial_weights_match_oracle(integ):
+    integ.guideReset(7)
+    w = integ.guideWeights()
+    assert np.array_equal(w, G.init_params(7).astype(np.float32))
+
+
+def test_inference_matches_oracle(integ):
+    rng = np.random.default_rng(1)
+    integ.guideReset(3)
+    p = integ.guideWeights().astype(np.float64)
+    # larger output layer than the initial 0.1 scale, so all outputs matter
+    Ws = G.split_params(p)
+    Ws[4][:, :40] *= 10.0
+    p = G.join_params(Ws).astype(np.float32)
+    integ.setGuideWeights(p)
+    q = rand_queries(rng, 1000)  # not a multiple of 64: idle lanes in the last wave
+    got = integ.guideInfer(q).astype(np.float64)
+    ref_bf, _ = G.forward(G.split_params(p.astype(np.float64)), G.encode(q), bf=True)
+    ref_64, _ = G.forward(G.split_params(p.astype(np.float64)), G.encode(q), bf=False)
+    ref_bf, ref_64 = ref_bf[:, :40], ref_64[:, :40]
+    scale = np.sqrt(np.mean(ref_64 ** 2))
+    e_bf = np.abs(got - ref_bf) / scale
+    e_64 = np.abs(got - ref_64) / scale
+    # same rounding points: float32 accumulation order and the encoding's
+    # float32 sin/cos (a rare bf16 rounding flip) are the only differences
+    assert np.median(e_bf) < 1e-3 and e_bf.max() < 3e-2, (np.median(e_bf), e_bf.max())
+    # against the float64 network: bfloat16 weights and activations
+    assert np.sqrt(np.mean(e_64 ** 2)) < 2e-2, np.sqrt(np.mean(e_64 ** 2))
+
+
+def test_training_gradient_and_adam_match_oracle(integ):
+    rng = np.random.default_rng(2)
+    integ.guideReset(5)
+    integ.setGuide(False, learning_rate=1e-3)
+    p0 = integ.guideWeights().astype(np.float64)
+    Ws = G.split_params(p0)
+    Ws[4][:, :40] *= 10.0
+    p0 = G.join_params(Ws).astype(np.float32).astype(np.float64)
+    integ.setGuideWeights(p0)
+    n = 3000
+    q = rand_queries(rng, n)
+    wi = rand_dirs(rng, n)
+    w = rng.uniform(0.05, 3.0, n).astype(np.float32).astype(np.float64)
+    samples = np.concatenate([q, wi, w[:, None]], 1)
+    loss, g = integ.guideTrain(samples, want_grad=True)
+    g = g.astype(np.float64)
+    l_bf, g_bf = G.loss_and_grad(p0, q, wi, w, bf=True)
+    l_64, g_64 = G.loss_and_grad(p0, q, wi, w, bf=False)
+    assert abs(loss - l_bf) <= 1e-3 * abs(l_bf), (loss, l_bf)
+    offs = [0, 256 * 64] + [256 * 64 + k * 4096 for k in range(1, 5)]
+    for l in range(5):
+        a, b = offs[l], offs[l + 1]
+        rb = np.linalg.norm(g[a:b] - g_bf[a:b]) / np.linalg.norm(g_bf[a:b])
+        r6 = np.linalg.norm(g[a:b] - g_64[a:b]) / np.linalg.norm(g_64[a:b])
+        assert rb < 1e-2, (l, rb)
+        assert r6 < 6e-2, (l, r6)
+    p1 = integ.guideWeights().astype(np.float64)
+    ref1, _, _ = G.adam(p0, np.zeros_like(p0), np.zeros_like(p0), 1, g, lr=1e-3)
+    assert np.allclose(p1, ref1, rtol=1e-6, atol=1e-9)
+    st = integ.guideStats()
+    assert st["steps"] == 1 and st["last_count"] == n
+
+
+def test_training_learns_a_known_distribution(integ):
+    """Directions drawn from one vMF lobe whose mean depends on the query's
+    normal: the loss approaches the lobe's entropy."""
+    rng = np.random.default_rng(4)
+    integ.guideReset(9)
+    integ.setGuide(False, learning_rate=3e-3)
+    kappa = 20.0
+    # entropy of vMF(kappa) = -E[log p]
+    h = -(np.log(kappa / (2 * np.pi * (1 - np.exp(-2 * kappa)))) + kappa * (1 / np.tanh(kappa) - 1 / kappa - 1))
+    losses = []
+    for step in range(300):
+        q = rand_queries(rng, 8192)
+        mu = q[:, 3:6]
+        one = np.zeros((8192, 64))
+        one[:, 1:4] = mu
+        one[:, 4] = np.log(kappa)
+        one[:, 5::5][:, :7] = -30.0  # other lobes off
+        d = G.mix_sample(one, rng.uniform(size=8192), rng.uniform(size=8192), rng.uniform(size=8192))
+        s = np.concatenate([q, d, np.ones((8192, 1))], 1)
+        loss, _ = integ.guideTrain(s)
+        losses.append(loss)
+    integ.setGuide(False, learning_rate=1e-3)
+    first, last = np.mean(losses[:5]), np.mean(losses[-20:])
+    assert first > h + 1.0
+    assert last < h + 0.25, (first, last, h)
+
+
+def room(width=96, height=64):
+    return scenes.make_c3(width=width, height=height, spp=64, cell=0.5, n_props=8, prop_subdiv=1, hdri_w=256,
+                          hdri_h=128, max_depth=6)
+
+
+def chunk_means(it, chunks, spp):
+    it.resetIterationIndex()
+    means, prev = [], None
+    for _ in range(chunks):
+        it.render(spp)
+        s = it.readbackSums()
+        cur = s if prev is None else s - prev
+        prev = s
+        means.append(cur.mean() / spp)
+    return np.array(means), prev / (chunks * spp)
+
+
+def relmse(img, ref):
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+@pytest.fixture(scope="module")
+def room_ref():
+    desc = room()
+    it = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
+    upload_scene(it, desc)
+    it.render(4096)
+    ref = it.readbackSums() / 4096
+    yield desc, it, ref
+    it.close()
+
+
+def test_guided_estimator_is_unbiased(room_ref):
+    desc, it, ref = room_ref
+    it.guideReset(13)
+    it.setGuide(True, bsdf_fraction=0.5, train=0)
+    try:
+        gm, gimg = chunk_means(it, 8, 32)
+    finally:
+        it.setGuide(False)
+    se = gm.std(ddof=1) / np.sqrt(len(gm))
+    assert abs(gm.mean() - ref.mean()) < 4 * se + 1e-3 * ref.mean(), (gm.mean(), ref.mean(), se)
+    assert relmse(gimg, ref) < 0.5
+
+
+def test_trained_guide_beats_plain_at_equal_samples(room_ref):
+    desc, it, ref = room_ref
+    it.guideReset(17)
+    it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0, samples_per_batch=1)
+    try:
+        # train online for 96 batches (one sample per pixel each)
+        it.setGuide(True, bsdf_fraction=0.5, train=1, record_paths=desc.width * desc.height, batch_size=32768,
+                    learning_rate=3e-3)
+        it.resetIterationIndex()
+        it.render(96)
+        st = it.guideStats()
+        assert st["steps"] == 96 and np.isfinite(st["last_loss"]) and st["samples"] > 96 * 1000
+        # then compare at equal samples with frozen weights (different seeds: view lane 1)
+        it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=1, samples_per_batch=1)
+        it.setGuide(True, bsdf_fraction=0.5, train=0)
+        it.render(64)
+        guided = it.readbackSums() / 64
+        it.setGuide(False)
+        it.render(64)
+        plain = it.readbackSums() / 64
+    finally:
+        it.setGuide(False)
+        it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0)
+    eg, ep = relmse(guided, ref), relmse(plain, ref)
+    print(f"relMSE at 64 spp: guided {eg:.4g}, plain {ep:.4g}, ratio {eg / ep:.3f}")
+    assert eg < 0.8 * ep, (eg, ep)
