@@ -110,6 +110,30 @@ def batched_views(integ, args) -> dict:
             "autotune": None if tune is None else {"trials": len(tune["trials"]), "best_sample_ms": tune["ms"]}}
 
 
+def guided_room(args) -> dict:
+    """Time to equal error, guided (online training included) vs plain, on
+    the closed room lit through a window (1920x1080, depth 10)."""
+    from colvillea_amd import guided_eval, scenes
+
+    t = time.perf_counter()
+    desc = scenes.make_c3(spp=args.guided_spp)
+    log(f"room scene: {desc.num_triangles()} triangles in {time.perf_counter() - t:.1f} s")
+    pts = [p for p in (8, 16, 32, 64, 128, 256, 512, 1024) if p < args.guided_spp] + [args.guided_spp]
+    guide = {"bsdf_fraction": 0.5, "train": 1, "record_paths": 32768, "batch_size": 65536, "learning_rate": 1e-3}
+    r = guided_eval.time_to_equal_error(desc, args.guided_ref_spp, pts, guide, log=log)
+    nsamp = desc.width * desc.height * args.guided_spp
+    return {"workload": f"closed 10x10x4 room lit through a 1x1 window by the 1080p scene's HDRI, "
+                        f"{desc.num_triangles()} triangles, {desc.width}x{desc.height}, {args.guided_spp} spp, "
+                        f"depth {desc.max_depth}, seed 4",
+            "t_plain_s": round(r["t_plain"], 3), "t_guided_s": None if r["t_guided"] is None else round(r["t_guided"], 3),
+            "time_ratio": None if r["ratio"] is None else round(r["ratio"], 3), "relmse_plain_final": r["eps"],
+            "plain_Msamples_per_s": round(nsamp / r["t_plain"] / 1e6, 2),
+            "guided_Msamples_per_s": round(nsamp / r["guided"][-1]["seconds"] / 1e6, 2),
+            "relmse_guided_final": r["guided"][-1]["relmse"], "reference_spp": r["ref_spp"],
+            "train_steps": r["train_steps"], "samples_trained": r["samples_trained"], "guide": r["guide"],
+            "curves": {"plain": r["plain"], "guided": r["guided"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +150,9 @@ def main():
     ap.add_argument("--no-c2", action="store_true", help="skip the batched 256-view render")
     ap.add_argument("--c2-steps", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain room evaluation")
+    ap.add_argument("--guided-ref-spp", type=int, default=1024)
+    ap.add_argument("--guided-spp", type=int, default=256)
     ap.add_argument("--autotune-log", default=None, help="write the autotuner's trials here (JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -199,6 +226,9 @@ def main():
     roofline["copy_ceiling_GBs"] = round(copy_gbs, 1)
     roofline["frac_of_copy_ceiling"] = round(achieved / copy_gbs, 4)
     log(f"copy probe: {args.copy_mib} MiB in {copy_ms.value:.3f} ms -> {copy_gbs:.0f} GB/s (read + write)")
+    integ.close()
+    integ = None
+    guided = None if args.no_guided else guided_room(args)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(desc, args.cpu_seconds, args.cpu_threads)
@@ -210,8 +240,9 @@ def main():
                                   f"{desc.num_triangles()} triangles (fBm grid + {len(desc.instance_mesh) - 1} "
                                   f"instanced icospheres), 2048x1024 log-normal HDRI, seed {desc.info['seed']}",
                       "width": desc.width, "height": desc.height, "spp": desc.spp, "max_depth": desc.max_depth},
-           "roofline": roofline, "cpu_baseline": cpu, "batched": batched}
-    integ.close()
+           "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "guided": guided}
+    if integ is not None:
+        integ.close()
     print(json.dumps(out), flush=True)
 
 

@@ -131,6 +131,7 @@ class clhip_guide(C.Structure):
         ("batch_size", C.c_uint32),
         ("learning_rate", C.c_float),
         ("max_steps", C.c_uint32),
+        ("precision", C.c_uint32),
     ]
 
 
@@ -195,6 +196,7 @@ SIGNATURES = {
     "clhip_guide_set_weights": (C.c_int, [P, f32p, u32]),
     "clhip_guide_stats_get": (C.c_int, [P, C.POINTER(clhip_guide_stats)]),
     "clhip_guide_infer": (C.c_int, [P, f32p, u32, f32p]),
+    "clhip_guide_fp8_scales": (C.c_int, [P, f32p]),
     "clhip_guide_train": (C.c_int, [P, f32p, u32, f32p, f32p]),
 }
 

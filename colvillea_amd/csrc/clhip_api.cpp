@@ -164,7 +164,7 @@ struct clhip_ctx
     float       world_lo[3] = {0.0f, 0.0f, 0.0f}, world_hi[3] = {1.0f, 1.0f, 1.0f};
     clhip_guide guide{};
     bool        guide_ready = false;
-    DevBuf      g_params, g_m1, g_m2, g_W, g_WT, g_img, g_gc;
+    DevBuf      g_params, g_m1, g_m2, g_W, g_WT, g_img, g_gc, g_img8, g_scl;
     DevBuf      g_X, g_act[4], g_out, g_grad[5], g_loss, g_partial, g_samples;
     uint32_t    g_N = 0;
     DevBuf      g_V, g_E, g_tag;
@@ -393,7 +393,10 @@ int guide_reset_weights(clhip_ctx* c, uint64_t seed)
         CLHIP_CHECK(c, c->g_WT.alloc(CLPT_G_PARAMS * sizeof(__bf16)));
         CLHIP_CHECK(c, c->g_img.alloc(CLPT_G_IMG * sizeof(__bf16)));
         CLHIP_CHECK(c, c->g_gc.alloc(sizeof(GuideCounters)));
+        CLHIP_CHECK(c, c->g_img8.alloc(CLPT_G_IMG));
+        CLHIP_CHECK(c, c->g_scl.alloc(16 * sizeof(float)));
         CLHIP_CHECK(c, hipMemset(c->g_img.p, 0, CLPT_G_IMG * sizeof(__bf16)));
+        CLHIP_CHECK(c, hipMemset(c->g_img8.p, 0, CLPT_G_IMG));
     }
     /* He-uniform weights from a SplitMix64 stream; the output layer is
      * scaled by 0.1 and its unused columns (40..63) are zero */
@@ -425,6 +428,8 @@ int guide_reset_weights(clhip_ctx* c, uint64_t seed)
     CLHIP_CHECK(c, hipMemset(c->g_gc.p, 0, sizeof(GuideCounters)));
     CLHIP_CHECK(c, clpt::launch_guide_pack(c->stream, c->g_params.as<const float>(), c->g_W.as<__bf16>(),
                                            c->g_WT.as<__bf16>(), c->g_img.as<__bf16>()));
+    CLHIP_CHECK(c, clpt::launch_guide_pack_fp8(c->stream, c->g_params.as<const float>(), c->g_gc.as<GuideCounters>(),
+                                               c->g_img8.as<uint8_t>(), c->g_scl.as<float>()));
     CLHIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->guide_ready = true;
     c->g_issued    = 0;
@@ -480,6 +485,8 @@ GuideView guide_view(const clhip_ctx* c)
     if (!guiding(c))
         return g;
     g.img    = c->g_img.as<const __bf16>();
+    g.img8   = c->guide.precision == 1u ? c->g_img8.as<const uint8_t>() : nullptr;
+    g.scl    = c->g_scl.as<const float>();
     g.c_bsdf = c->guide.bsdf_fraction;
     for (int k = 0; k < 3; ++k)
     {
@@ -515,6 +522,8 @@ clpt::GuideTrainBufs guide_bufs(clhip_ctx* c)
     b.W         = c->g_W.as<__bf16>();
     b.WT        = c->g_WT.as<__bf16>();
     b.img       = c->g_img.as<__bf16>();
+    b.img8      = c->g_img8.as<uint8_t>();
+    b.scl       = c->g_scl.as<float>();
     return b;
 }
 
@@ -1967,6 +1976,8 @@ int clhip_set_guide(clhip_ctx* ctx, const clhip_guide* g)
         return fail(ctx, CLHIP_E_INVALID, "record_paths %u above 2^22", v.record_paths);
     if (v.learning_rate == 0.0f)
         v.learning_rate = 1e-3f;
+    if (v.precision > 1u)
+        return fail(ctx, CLHIP_E_INVALID, "precision %u: 0 (bfloat16) or 1 (FP8)", v.precision);
     if (!(v.learning_rate > 0.0f))
         return fail(ctx, CLHIP_E_INVALID, "learning_rate must be positive");
     ctx->guide = v;
@@ -2007,7 +2018,21 @@ int clhip_guide_set_weights(clhip_ctx* ctx, const float* w, uint32_t n)
     CLHIP_CHECK(ctx, hipMemcpy(ctx->g_params.p, w, CLPT_G_PARAMS * sizeof(float), hipMemcpyHostToDevice));
     CLHIP_CHECK(ctx, clpt::launch_guide_pack(ctx->stream, ctx->g_params.as<const float>(), ctx->g_W.as<__bf16>(),
                                              ctx->g_WT.as<__bf16>(), ctx->g_img.as<__bf16>()));
+    CLHIP_CHECK(ctx, clpt::launch_guide_pack_fp8(ctx->stream, ctx->g_params.as<const float>(),
+                                                 ctx->g_gc.as<GuideCounters>(), ctx->g_img8.as<uint8_t>(),
+                                                 ctx->g_scl.as<float>()));
     CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return CLHIP_OK;
+}
+
+int clhip_guide_fp8_scales(clhip_ctx* ctx, float* scl)
+{
+    if (!ctx || !scl)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_guide_fp8_scales: bad arguments");
+    if (!ctx->guide_ready)
+        return fail(ctx, CLHIP_E_STATE, "no weights: call clhip_guide_reset or clhip_set_guide first");
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, hipMemcpy(scl, ctx->g_scl.p, 10 * sizeof(float), hipMemcpyDeviceToHost));
     return CLHIP_OK;
 }
 
@@ -2046,8 +2071,10 @@ int clhip_guide_infer(clhip_ctx* ctx, const float* q9, uint32_t n, float* out40)
     CLHIP_CHECK(ctx, q.from_host(q9, 9 * (size_t)n));
     CLHIP_CHECK(ctx, o.zeros((size_t)n * CLPT_G_NOUT * sizeof(float)));
     const uint32_t blocks = std::min<uint32_t>(ctx->num_cus * 2u, (n + 255u) / 256u);
-    CLHIP_CHECK(ctx, clpt::launch_guide_infer(ctx->stream, ctx->g_img.as<const __bf16>(), q.b.as<const float>(), n,
-                                              o.b.as<float>(), blocks));
+    CLHIP_CHECK(ctx, clpt::launch_guide_infer(ctx->stream, ctx->g_img.as<const __bf16>(),
+                                              ctx->guide.precision == 1u ? ctx->g_img8.as<const uint8_t>() : nullptr,
+                                              ctx->g_scl.as<const float>(), q.b.as<const float>(), n, o.b.as<float>(),
+                                              blocks));
     CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     CLHIP_CHECK(ctx, o.to_host(out40, (size_t)n * CLPT_G_NOUT));
     return CLHIP_OK;

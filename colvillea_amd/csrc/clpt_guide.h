@@ -127,9 +127,8 @@ CLPT_D bf16x8 guide_pack(const float o[8])
 /* Group 2t+h of query v, h = lane half (0 or 1): one argument reduction per
  * lane where both candidate groups are sin/cos blocks. */
 template <int T>
-CLPT_D bf16x8 guide_group_sel(const float v[9], bool h)
+CLPT_D void guide_group_sel(const float v[9], bool h, float o[8])
 {
-    float o[8];
     constexpr int G0 = 2 * T, G1 = 2 * T + 1;
     if constexpr (G1 < 27)
     {
@@ -146,7 +145,6 @@ CLPT_D bf16x8 guide_group_sel(const float v[9], bool h)
         for (int k = 0; k < 8; ++k)
             o[k] = h ? b[k] : a[k];
     }
-    return guide_pack(o);
 }
 
 /* ------------------------------------------------------------ vMF mixture */
@@ -228,6 +226,65 @@ CLPT_D f3 vmf_mixture_sample(const VmfMix& m, float usel, float u1, float u2)
 
 /* ------------------------------------------------------------ fused forward */
 
+/* FP8 (OCP e4m3) inference, the precision ablation: weights of layer l are
+ * stored as e4m3(W * ws_l), the input of layer l enters as e4m3(x * as_l)
+ * (both saturated at +-448), and the float32 accumulator is scaled back by
+ * 1 / (as_l ws_l).  scl[2l] = as_l, scl[2l+1] = 1 / (as_l ws_l).  The
+ * instruction is v_mfma_f32_32x32x16_fp8_fp8, lane layout as for bf16 with
+ * 8 bytes per lane. */
+#define CLPT_G_FP8_MAX 448.0f
+
+CLPT_D long guide_pack_fp8(const float o[8], float scale)
+{
+    float x[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        x[k] = fminf(fmaxf(o[k] * scale, -CLPT_G_FP8_MAX), CLPT_G_FP8_MAX);
+    int lo = __builtin_amdgcn_cvt_pk_fp8_f32(x[0], x[1], 0, false);
+    lo     = __builtin_amdgcn_cvt_pk_fp8_f32(x[2], x[3], lo, true);
+    int hi = __builtin_amdgcn_cvt_pk_fp8_f32(x[4], x[5], 0, false);
+    hi     = __builtin_amdgcn_cvt_pk_fp8_f32(x[6], x[7], hi, true);
+    return (long)(uint32_t)lo | ((long)(uint32_t)hi << 32);
+}
+
+template <bool FP8>
+struct GuideFrag
+{
+    typedef bf16x8 type;
+};
+template <>
+struct GuideFrag<true>
+{
+    typedef long type;
+};
+
+template <bool FP8>
+CLPT_D typename GuideFrag<FP8>::type guide_load_a(const void* W, uint32_t off)
+{
+    if constexpr (FP8)
+        return *(const long*)((const uint8_t*)W + off);
+    else
+        return *(const bf16x8*)((const __bf16*)W + off);
+}
+
+template <bool FP8>
+CLPT_D typename GuideFrag<FP8>::type guide_pack_b(const float o[8], float scale)
+{
+    if constexpr (FP8)
+        return guide_pack_fp8(o, scale);
+    else
+        return guide_pack(o);
+}
+
+template <bool FP8>
+CLPT_D f32x16 guide_mfma(typename GuideFrag<FP8>::type a, typename GuideFrag<FP8>::type b, f32x16 c)
+{
+    if constexpr (FP8)
+        return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a, b, c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
 CLPT_D f32x16 guide_zero16()
 {
     f32x16 z;
@@ -237,10 +294,12 @@ CLPT_D f32x16 guide_zero16()
     return z;
 }
 
-template <int L>
-CLPT_D void guide_hidden_layer(const __bf16* W, f32x16 acc[2][2], uint32_t c, uint32_t h, bool relu_in)
+/* Layer L >= 1: input = relu(previous accumulator) (times `in_scale`, the
+ * previous layer's 1/(as ws) folded with this layer's as for FP8). */
+template <bool FP8, int L>
+CLPT_D void guide_hidden_layer(const void* W, f32x16 acc[2][2], uint32_t c, uint32_t h, float in_scale)
 {
-    bf16x8 b[2][4];
+    typename GuideFrag<FP8>::type b[2][4];
 #pragma unroll
     for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
@@ -249,13 +308,10 @@ CLPT_D void guide_hidden_layer(const __bf16* W, f32x16 acc[2][2], uint32_t c, ui
             float o[8];
 #pragma unroll
             for (int s = 0; s < 8; ++s)
-            {
-                const float x = acc[t / 2][ct][8 * (t % 2) + s];
-                o[s]          = relu_in ? fmaxf(x, 0.0f) : x;
-            }
-            b[ct][t] = guide_pack(o);
+                o[s] = fmaxf(acc[t / 2][ct][8 * (t % 2) + s], 0.0f);
+            b[ct][t] = guide_pack_b<FP8>(o, in_scale);
         }
-    const __bf16* Wl = W + guide_img_offset(L);
+    const uint32_t base = guide_img_offset(L);
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
@@ -266,16 +322,18 @@ CLPT_D void guide_hidden_layer(const __bf16* W, f32x16 acc[2][2], uint32_t c, ui
 #pragma unroll
         for (int rt = 0; rt < 2; ++rt)
         {
-            const bf16x8 a = *(const bf16x8*)(Wl + (32 * rt + c) * CLPT_G_S1 + 16 * t + 8 * h);
-            acc[rt][0]     = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[0][t], acc[rt][0], 0, 0, 0);
-            acc[rt][1]     = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[1][t], acc[rt][1], 0, 0, 0);
+            const auto a = guide_load_a<FP8>(W, base + (32 * rt + c) * CLPT_G_S1 + 16 * t + 8 * h);
+            acc[rt][0]   = guide_mfma<FP8>(a, b[0][t], acc[rt][0]);
+            acc[rt][1]   = guide_mfma<FP8>(a, b[1][t], acc[rt][1]);
         }
 }
 
 /* Forward pass for the 64 queries of a wave (one per lane; every lane of
  * the wave must call it, with any query for idle lanes).  W is the packed
- * image (LDS or global).  On return out[] holds this lane's 40 outputs. */
-CLPT_D void guide_forward_wave(const __bf16* W, const float q[9], float out[CLPT_G_NOUT])
+ * image (LDS or global): bf16 elements, or e4m3 bytes with the scales scl
+ * (FP8).  On return out[] holds this lane's 40 outputs. */
+template <bool FP8>
+CLPT_D void guide_forward_wave_t(const void* W, const float* scl, const float q[9], float out[CLPT_G_NOUT])
 {
     const uint32_t lane = __lane_id();
     const uint32_t c = lane & 31u, h = lane >> 5;
@@ -286,6 +344,10 @@ CLPT_D void guide_forward_wave(const __bf16* W, const float q[9], float out[CLPT
         qa[d] = __shfl(q[d], (int)c);
         qb[d] = __shfl(q[d], (int)(c + 32));
     }
+    float sc[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
+        sc[k] = FP8 ? scl[k] : 1.0f;
     f32x16 acc[2][2];
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt)
@@ -295,22 +357,25 @@ CLPT_D void guide_forward_wave(const __bf16* W, const float q[9], float out[CLPT
     const bool hb = h != 0;
 #define CLPT_G_L0_STEP(T)                                                                                          \
     {                                                                                                              \
-        const bf16x8 b0 = guide_group_sel<T>(qa, hb), b1 = guide_group_sel<T>(qb, hb);                            \
+        float o0[8], o1[8];                                                                                        \
+        guide_group_sel<T>(qa, hb, o0);                                                                            \
+        guide_group_sel<T>(qb, hb, o1);                                                                            \
+        const auto b0 = guide_pack_b<FP8>(o0, sc[0]), b1 = guide_pack_b<FP8>(o1, sc[0]);                           \
         _Pragma("unroll") for (int rt = 0; rt < 2; ++rt)                                                           \
         {                                                                                                          \
-            const bf16x8 a = *(const bf16x8*)(W + (32 * rt + c) * CLPT_G_S0 + 16 * (T) + 8 * h);                  \
-            acc[rt][0]     = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc[rt][0], 0, 0, 0);                  \
-            acc[rt][1]     = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc[rt][1], 0, 0, 0);                  \
+            const auto a = guide_load_a<FP8>(W, (32 * rt + c) * CLPT_G_S0 + 16 * (T) + 8 * h);                    \
+            acc[rt][0]   = guide_mfma<FP8>(a, b0, acc[rt][0]);                                                     \
+            acc[rt][1]   = guide_mfma<FP8>(a, b1, acc[rt][1]);                                                     \
         }                                                                                                          \
     }
     CLPT_G_L0_STEP(0) CLPT_G_L0_STEP(1) CLPT_G_L0_STEP(2) CLPT_G_L0_STEP(3) CLPT_G_L0_STEP(4) CLPT_G_L0_STEP(5)
     CLPT_G_L0_STEP(6) CLPT_G_L0_STEP(7) CLPT_G_L0_STEP(8) CLPT_G_L0_STEP(9) CLPT_G_L0_STEP(10) CLPT_G_L0_STEP(11)
     CLPT_G_L0_STEP(12) CLPT_G_L0_STEP(13) CLPT_G_L0_STEP(14) CLPT_G_L0_STEP(15)
 #undef CLPT_G_L0_STEP
-    guide_hidden_layer<1>(W, acc, c, h, true);
-    guide_hidden_layer<2>(W, acc, c, h, true);
-    guide_hidden_layer<3>(W, acc, c, h, true);
-    guide_hidden_layer<4>(W, acc, c, h, true);
+    guide_hidden_layer<FP8, 1>(W, acc, c, h, sc[1] * sc[2]);
+    guide_hidden_layer<FP8, 2>(W, acc, c, h, sc[3] * sc[4]);
+    guide_hidden_layer<FP8, 3>(W, acc, c, h, sc[5] * sc[6]);
+    guide_hidden_layer<FP8, 4>(W, acc, c, h, sc[7] * sc[8]);
     /* lane l takes column tile h (query l): its own rows of that tile and
      * the partner lane's (l ^ 32) rows of the same tile */
 #pragma unroll
@@ -321,13 +386,18 @@ CLPT_D void guide_forward_wave(const __bf16* W, const float q[9], float out[CLPT
             const int row0 = 32 * rt + 8 * (r / 4) + r % 4; /* half 0; half 1 is row0 + 4 */
             if (row0 >= CLPT_G_NOUT)
                 continue;
-            const float own  = hb ? acc[rt][1][r] : acc[rt][0][r];
-            const float send = hb ? acc[rt][0][r] : acc[rt][1][r];
+            const float own  = (hb ? acc[rt][1][r] : acc[rt][0][r]) * sc[9];
+            const float send = (hb ? acc[rt][0][r] : acc[rt][1][r]) * sc[9];
             const float recv = __shfl_xor(send, 32);
             out[row0]        = hb ? recv : own;
             if (row0 + 4 < CLPT_G_NOUT)
                 out[row0 + 4] = hb ? own : recv;
         }
+}
+
+CLPT_D void guide_forward_wave(const __bf16* W, const float q[9], float out[CLPT_G_NOUT])
+{
+    guide_forward_wave_t<false>(W, nullptr, q, out);
 }
 
 /* The query of a surface point: position in the scene box, normal, wo. */

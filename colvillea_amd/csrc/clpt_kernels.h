@@ -57,9 +57,14 @@ struct GuideTrainBufs
     __bf16*        W;
     __bf16*        WT;
     __bf16*        img;
+    uint8_t*       img8; /* FP8 image refreshed after every step when not NULL */
+    float*         scl;
 };
-hipError_t launch_guide_infer(hipStream_t s, const __bf16* img, const float* q9, uint32_t n, float* out40,
-                              uint32_t blocks);
+/* bf16 inference, or FP8 when img8 is not NULL (scales scl) */
+hipError_t launch_guide_infer(hipStream_t s, const __bf16* img, const uint8_t* img8, const float* scl, const float* q9,
+                              uint32_t n, float* out40, uint32_t blocks);
+hipError_t launch_guide_pack_fp8(hipStream_t s, const float* params, const GuideCounters* gc, uint8_t* img8,
+                                 float* scl);
 hipError_t launch_guide_pack(hipStream_t s, const float* params, __bf16* W, __bf16* WT, __bf16* img);
 hipError_t launch_guide_gather(hipStream_t s, const float4* V, const float4* E, uint32_t nslots, uint32_t maxd,
                                GuideSample* out, uint32_t cap, GuideCounters* gc);

@@ -196,6 +196,7 @@ struct GuideCounters
     float              loss;    /* weighted negative log-likelihood of the last step (per unit weight) */
     uint32_t           pad;
     unsigned long long samples_total;
+    uint32_t           amax[4]; /* max hidden activation of each layer in the last step (float bits) */
 };
 
 /* What the guided shading kernels need.  Training records: per recorded
@@ -205,7 +206,9 @@ struct GuideCounters
  * (escape of ray j or the shadow ray of vertex j). */
 struct GuideView
 {
-    const __bf16* img; /* packed bf16 weights */
+    const __bf16*  img;  /* packed bf16 weights */
+    const uint8_t* img8; /* packed e4m3 weights (FP8 inference), else NULL */
+    const float*   scl;  /* FP8 scales (10 floats) */
     float         lo[3], inv_ext[3];
     float         c_bsdf; /* probability of BSDF sampling at guided vertices */
     uint32_t      nslots; /* recorded paths per batch (0 = no recording) */

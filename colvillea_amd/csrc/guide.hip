@@ -20,25 +20,26 @@
 namespace
 {
 
-__device__ __forceinline__ void load_image(const __bf16* __restrict__ img, __bf16* w)
-{
-    const uint4* src = reinterpret_cast<const uint4*>(img);
-    uint4*       dst = reinterpret_cast<uint4*>(w);
-    for (uint32_t k = threadIdx.x; k < CLPT_G_IMG / 8; k += blockDim.x)
-        dst[k] = src[k];
-    __syncthreads();
-}
-
 __device__ __forceinline__ float lum3(float r, float g, float b) { return 0.2126f * r + 0.7152f * g + 0.0722f * b; }
 
 } // namespace
 
 /* ------------------------------------------------------------ inference */
 
-__global__ __launch_bounds__(256) void k_guide_infer(const __bf16* img, const float* q9, uint32_t n, float* out40)
+template <bool FP8>
+__global__ __launch_bounds__(256) void k_guide_infer(const void* img, const float* scl, const float* q9, uint32_t n,
+                                                     float* out40)
 {
-    __shared__ __bf16 w[CLPT_G_IMG];
-    load_image(img, w);
+    __shared__ uint4 w[CLPT_G_IMG * (FP8 ? 1 : 2) / 16];
+    __shared__ float s[10];
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(img);
+        for (uint32_t k = threadIdx.x; k < CLPT_G_IMG * (FP8 ? 1 : 2) / 16; k += blockDim.x)
+            w[k] = src[k];
+        if (threadIdx.x < 10)
+            s[threadIdx.x] = FP8 ? scl[threadIdx.x] : 1.0f;
+        __syncthreads();
+    }
     const uint32_t lane   = __lane_id();
     const uint32_t wave   = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -50,7 +51,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(const __bf16* img, const fl
         for (int d = 0; d < 9; ++d)
             q[d] = i < n ? q9[9 * (size_t)i + d] : 0.0f;
         float out[CLPT_G_NOUT];
-        guide_forward_wave(w, q, out);
+        guide_forward_wave_t<FP8>(w, s, q, out);
         if (i < n)
 #pragma unroll
             for (int k = 0; k < CLPT_G_NOUT; ++k)
@@ -106,7 +107,7 @@ enum
 template <int EPI>
 __global__ __launch_bounds__(256) void k_guide_gemm_nt(const __bf16* __restrict__ A, const __bf16* __restrict__ Bt,
                                                        uint32_t N, uint32_t K, uint32_t J, void* C,
-                                                       const __bf16* __restrict__ mask)
+                                                       const __bf16* __restrict__ mask, uint32_t* amax)
 {
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t jt   = J / 32u;
@@ -124,11 +125,13 @@ __global__ __launch_bounds__(256) void k_guide_gemm_nt(const __bf16* __restrict_
         const bf16x8 b = *reinterpret_cast<const bf16x8*>(pb + k0);
         acc            = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
+    float mx = 0.0f;
 #pragma unroll
     for (int r = 0; r < 16; ++r)
     {
         const size_t idx = (size_t)(n0 + 8u * (r / 4) + 4u * h + (r % 4)) * J + j0 + c;
         const float  y   = acc[r];
+        mx               = fmaxf(mx, y);
         if constexpr (EPI == GEMM_RELU_BF16)
             static_cast<__bf16*>(C)[idx] = (__bf16)fmaxf(y, 0.0f);
         else if constexpr (EPI == GEMM_F32)
@@ -136,6 +139,14 @@ __global__ __launch_bounds__(256) void k_guide_gemm_nt(const __bf16* __restrict_
         else
             static_cast<__bf16*>(C)[idx] = (float)mask[idx] > 0.0f ? (__bf16)y : (__bf16)0.0f;
     }
+    if constexpr (EPI == GEMM_RELU_BF16)
+        if (amax)
+        {
+            for (int off = 32; off > 0; off >>= 1)
+                mx = fmaxf(mx, __shfl_xor(mx, off));
+            if (__lane_id() == 0)
+                atomicMax(amax, __float_as_uint(mx)); /* non-negative floats order as their bits */
+        }
 }
 
 /* Partial weight gradient over one chunk of samples:
@@ -192,6 +203,8 @@ __global__ __launch_bounds__(1024) void k_guide_wsum(const GuideSample* s, Guide
     {
         gc->wsum    = (float)part[0];
         gc->trained = cnt;
+        for (int l = 0; l < 4; ++l)
+            gc->amax[l] = 0u;
     }
 }
 
@@ -295,6 +308,50 @@ __global__ void k_guide_pack(const float* params, __bf16* W, __bf16* WT, __bf16*
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < CLPT_G_PARAMS)
         guide_store_param(p, params[p], W, WT, img);
+}
+
+/* e4m3 image of the weights, one block per layer: ws_l = 448 / max|W_l|,
+ * as_0 = 448 (|features| <= 1), as_l = 448 / (1.25 max activation of the
+ * last training step) for the hidden inputs (2.0 before any training). */
+__global__ __launch_bounds__(256) void k_guide_pack_fp8(const float* params, const GuideCounters* gc, uint8_t* img8,
+                                                        float* scl)
+{
+    __shared__ float red[256];
+    const int      l   = blockIdx.x;
+    const uint32_t off = guide_layer_offset(l), in = guide_layer_in(l), cnt = in * CLPT_G_HID;
+    float          mx  = 0.0f;
+    for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x)
+        mx = fmaxf(mx, fabsf(params[off + k]));
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1)
+    {
+        if (threadIdx.x < st)
+            red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+    }
+    const float ws = red[0] > 0.0f ? CLPT_G_FP8_MAX / red[0] : 1.0f;
+    float       as = CLPT_G_FP8_MAX;
+    if (l > 0)
+    {
+        const float am = __uint_as_float(gc->amax[l - 1]);
+        as             = CLPT_G_FP8_MAX / (am > 0.0f ? 1.25f * am : 2.0f);
+    }
+    for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x)
+    {
+        const uint32_t i = k / CLPT_G_HID, m = k % CLPT_G_HID;
+        const float    x = fminf(fmaxf(params[off + k] * ws, -CLPT_G_FP8_MAX), CLPT_G_FP8_MAX);
+        const uint8_t  b = (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xff);
+        if (l == 0)
+            img8[m * CLPT_G_S0 + i] = b;
+        else
+            img8[guide_img_offset(l) + m * CLPT_G_S1 + guide_perm(i)] = b;
+    }
+    if (threadIdx.x == 0)
+    {
+        scl[2 * l]     = as;
+        scl[2 * l + 1] = 1.0f / (as * ws);
+    }
 }
 
 /* Adam on the float32 master weights (gradient = fixed-order sum of the
@@ -406,10 +463,20 @@ __global__ __launch_bounds__(256) void k_guide_gather(const float4* V, const flo
 
 namespace clpt
 {
-hipError_t launch_guide_infer(hipStream_t s, const __bf16* img, const float* q9, uint32_t n, float* out40,
-                              uint32_t blocks)
+hipError_t launch_guide_infer(hipStream_t s, const __bf16* img, const uint8_t* img8, const float* scl, const float* q9,
+                              uint32_t n, float* out40, uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(256), 0, s, img, q9, n, out40);
+    if (img8)
+        hipLaunchKernelGGL(k_guide_infer<true>, dim3(blocks), dim3(256), 0, s, (const void*)img8, scl, q9, n, out40);
+    else
+        hipLaunchKernelGGL(k_guide_infer<false>, dim3(blocks), dim3(256), 0, s, (const void*)img, scl, q9, n, out40);
+    return hipGetLastError();
+}
+
+hipError_t launch_guide_pack_fp8(hipStream_t s, const float* params, const GuideCounters* gc, uint8_t* img8,
+                                 float* scl)
+{
+    hipLaunchKernelGGL(k_guide_pack_fp8, dim3(CLPT_G_LAYERS), dim3(256), 0, s, params, gc, img8, scl);
     return hipGetLastError();
 }
 
@@ -441,11 +508,12 @@ hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N
     {
         hipLaunchKernelGGL(k_guide_gemm_nt<GEMM_RELU_BF16>, dim3(nt * 2 / 4), dim3(256), 0, s, in,
                            b.WT + guide_layer_offset(l), N, guide_layer_in(l), (uint32_t)CLPT_G_HID,
-                           (void*)b.act[l], (const __bf16*)nullptr);
+                           (void*)b.act[l], (const __bf16*)nullptr, &b.gc->amax[l]);
         in = b.act[l];
     }
     hipLaunchKernelGGL(k_guide_gemm_nt<GEMM_F32>, dim3(nt * 2 / 4), dim3(256), 0, s, in, b.WT + guide_layer_offset(4),
-                       N, (uint32_t)CLPT_G_HID, (uint32_t)CLPT_G_HID, (void*)b.out, (const __bf16*)nullptr);
+                       N, (uint32_t)CLPT_G_HID, (uint32_t)CLPT_G_HID, (void*)b.out, (const __bf16*)nullptr,
+                       (uint32_t*)nullptr);
     hipLaunchKernelGGL(k_guide_loss, dim3(N / 256), dim3(256), 0, s, b.out, b.samples, b.gc, N, b.grad[4],
                        b.loss_part);
     /* backward: G[l] is the gradient w.r.t. the input of layer l+1 (N x 64) */
@@ -460,11 +528,13 @@ hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N
         if (l > 0)
             hipLaunchKernelGGL(k_guide_gemm_nt<GEMM_MASK_BF16>, dim3(nt * 2 / 4), dim3(256), 0, s, b.grad[l],
                                b.W + guide_layer_offset(l), N, (uint32_t)CLPT_G_HID, (uint32_t)CLPT_G_HID,
-                               (void*)b.grad[l - 1], b.act[l - 1]);
+                               (void*)b.grad[l - 1], b.act[l - 1], (uint32_t*)nullptr);
     }
     hipLaunchKernelGGL(k_guide_adam, dim3(CLPT_G_PARAMS / 256), dim3(256), 0, s, b.params, b.m1, b.m2, b.partial,
                        nch, b.gc, lr, 0.9f, 0.999f, 1e-8f, b.W, b.WT, b.img, grad_out);
     hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(256), 0, s, b.gc, b.loss_part, N / 256);
+    if (b.img8)
+        hipLaunchKernelGGL(k_guide_pack_fp8, dim3(CLPT_G_LAYERS), dim3(256), 0, s, b.params, b.gc, b.img8, b.scl);
     return hipGetLastError();
 }
 } // namespace clpt

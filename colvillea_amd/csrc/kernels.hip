@@ -662,16 +662,22 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
  * unbiased.  The network runs once per wave of 64 rays (all lanes take part;
  * its weights sit in LDS).  Paths selected for training record their
  * guided vertices and the contributions that came back. */
-__global__ __launch_bounds__(256) void k_shade_guided(SceneView sc, BatchTable bt, GuideView gv, uint32_t depth,
-                                                      RayQueue qin, HitBuf hb, RayQueue qout, ShadowQueue sq,
-                                                      double* accum, BatchCounters* bc)
+/* At least 2 waves per SIMD: the LDS image allows 2 blocks per CU, and the
+ * register budget is held to match (accumulators in VGPRs, no spills).
+ * FP8 selects the e4m3 inference ablation. */
+template <bool FP8>
+__global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTable bt, GuideView gv, uint32_t depth,
+                                                         RayQueue qin, HitBuf hb, RayQueue qout, ShadowQueue sq,
+                                                         double* accum, BatchCounters* bc)
 {
-    __shared__ __bf16 wlds[CLPT_G_IMG];
+    __shared__ uint4 wlds[CLPT_G_IMG * (FP8 ? 1 : 2) / 16];
+    __shared__ float wscl[10];
     {
-        const uint4* src = reinterpret_cast<const uint4*>(gv.img);
-        uint4*       dst = reinterpret_cast<uint4*>(wlds);
-        for (uint32_t k = threadIdx.x; k < CLPT_G_IMG / 8; k += blockDim.x)
-            dst[k] = src[k];
+        const uint4* src = reinterpret_cast<const uint4*>(FP8 ? (const void*)gv.img8 : (const void*)gv.img);
+        for (uint32_t k = threadIdx.x; k < CLPT_G_IMG * (FP8 ? 1 : 2) / 16; k += blockDim.x)
+            wlds[k] = src[k];
+        if (threadIdx.x < 10)
+            wscl[threadIdx.x] = FP8 ? gv.scl[threadIdx.x] : 1.0f;
         __syncthreads();
     }
     const uint32_t n      = bc->rays[depth];
@@ -738,7 +744,7 @@ __global__ __launch_bounds__(256) void k_shade_guided(SceneView sc, BatchTable b
         if (__ballot(guided) != 0ull) /* wave-uniform */
         {
             float out[CLPT_G_NOUT];
-            guide_forward_wave(wlds, q, out);
+            guide_forward_wave_t<FP8>(wlds, wscl, q, out);
             vmf_mixture(out, mix);
         }
         bool  want_shadow = false, want_next = false;
@@ -1246,7 +1252,12 @@ hipError_t launch_shade_guided(hipStream_t s, const SceneView& sc, const BatchTa
                                uint32_t depth, const RayQueue& qin, const HitBuf& hb, const RayQueue& qout,
                                const ShadowQueue& sq, double* accum, BatchCounters* bc, uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_shade_guided, dim3(blocks), dim3(256), 0, s, sc, bt, gv, depth, qin, hb, qout, sq, accum, bc);
+    if (gv.img8)
+        hipLaunchKernelGGL(k_shade_guided<true>, dim3(blocks), dim3(256), 0, s, sc, bt, gv, depth, qin, hb, qout, sq,
+                           accum, bc);
+    else
+        hipLaunchKernelGGL(k_shade_guided<false>, dim3(blocks), dim3(256), 0, s, sc, bt, gv, depth, qin, hb, qout, sq,
+                           accum, bc);
     return hipGetLastError();
 }
 
@@ -1330,7 +1341,7 @@ uint32_t resident_blocks_per_cu(int which)
     else if (which == 2)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shade, 256, 0);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shade_guided, 256, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shade_guided<false>, 256, 0);
     return (e == hipSuccess && n > 0) ? (uint32_t)n : 4u;
 }
 

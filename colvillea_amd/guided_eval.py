@@ -1,0 +1,101 @@
+"""Time to equal error: the guided tracer (online training included) against
+the plain tracer on one scene.
+
+Both tracers render the same view in chunks; after each chunk the
+accumulated image is compared with a reference.  The error is the relative
+MSE  mean over pixels and channels of (I - R)^2 / (R^2 + 1e-2), corrected
+for the reference's own noise: the reference is the mean of two independent
+halves R1, R2 (different RNG view lanes), whose per-pixel variance of the
+mean is estimated by ((R1 - R2) / 2)^2 and subtracted in the same relative
+form.  The reference comes from the plain tracer (its agreement with the
+float64 oracle is tested separately at small sizes).
+
+Time is wall-clock around render + synchronise of each chunk (readbacks
+excluded), so the guided tracer's time includes its network inference and
+every training step.  T_plain(eps) is the plain tracer's time to its final
+error eps; T_guided(eps) is where the guided tracer's error curve crosses
+eps (log-log interpolation between chunks).
+"""
+from __future__ import annotations
+
+import math
+import time
+from typing import List, Optional
+
+import numpy as np
+
+from .integrator import Integrator, IntegratorType
+from .scenes import SceneDesc
+from .integrator import upload_scene
+
+EPS_DEN = 1e-2
+
+
+def _render_curve(it: Integrator, spp_points: List[int], ref, ref_var) -> list:
+    it.resetIterationIndex()
+    it.sync()
+    done, t_acc, curve = 0, 0.0, []
+    for s in spp_points:
+        t = time.perf_counter()
+        it.render(s - done)
+        it.sync()
+        t_acc += time.perf_counter() - t
+        done = s
+        img = it.readbackSums() / s
+        den = ref ** 2 + EPS_DEN
+        e = float(np.mean((img - ref) ** 2 / den) - np.mean(ref_var / den))
+        curve.append({"spp": s, "seconds": t_acc, "relmse": e})
+    return curve
+
+
+def _crossing(curve: list, eps: float) -> Optional[float]:
+    prev = None
+    for p in curve:
+        if p["relmse"] <= eps:
+            if prev is None or prev["relmse"] <= 0 or p["relmse"] <= 0:
+                return p["seconds"]
+            # log-log interpolation of error against time
+            x0, x1 = math.log(prev["seconds"]), math.log(p["seconds"])
+            y0, y1 = math.log(prev["relmse"]), math.log(p["relmse"])
+            y = math.log(eps)
+            return math.exp(x0 + (y - y0) * (x1 - x0) / (y1 - y0)) if y1 != y0 else p["seconds"]
+        prev = p
+    return None
+
+
+def time_to_equal_error(desc: SceneDesc, ref_spp: int, spp_points: List[int], guide: dict,
+                        it: Optional[Integrator] = None, log=print) -> dict:
+    own = it is None
+    if own:
+        it = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
+        upload_scene(it, desc)
+    try:
+        t0 = time.perf_counter()
+        halves = []
+        for lane in (2, 3):
+            it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=lane)
+            it.render(ref_spp // 2)
+            halves.append(it.readbackSums() / (ref_spp // 2))
+        ref = 0.5 * (halves[0] + halves[1])
+        ref_var = ((halves[0] - halves[1]) * 0.5) ** 2
+        log(f"reference: {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+        it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0, samples_per_batch=1)
+        it.setGuide(False)
+        plain = _render_curve(it, spp_points, ref, ref_var)
+        log(f"plain: {plain[-1]}")
+        it.guideReset(guide.get("seed", 1))
+        it.setGuide(True, **{k: v for k, v in guide.items() if k != "seed"})
+        guided = _render_curve(it, spp_points, ref, ref_var)
+        gst = it.guideStats()
+        log(f"guided: {guided[-1]}, {gst['steps']} training steps, {gst['train_ms']:.1f} ms training in the last call")
+        it.setGuide(False)
+        eps = plain[-1]["relmse"]
+        tg = _crossing(guided, eps)
+        tp = plain[-1]["seconds"]
+        return {"plain": plain, "guided": guided, "eps": eps, "t_plain": tp, "t_guided": tg,
+                "ratio": (tg / tp) if tg is not None else None, "ref_spp": ref_spp,
+                "guide": dict(guide), "train_steps": int(gst["steps"]), "samples_trained": int(gst["samples"])}
+    finally:
+        it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=desc.view)
+        if own:
+            it.close()

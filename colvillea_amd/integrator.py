@@ -352,7 +352,8 @@ class Integrator:
         return out
 
     # -- learned directional proposal (no reference counterpart)
-    GUIDE_FIELDS = ("enable", "bsdf_fraction", "train", "record_paths", "batch_size", "learning_rate", "max_steps")
+    GUIDE_FIELDS = ("enable", "bsdf_fraction", "train", "record_paths", "batch_size", "learning_rate", "max_steps",
+                    "precision")
 
     def setGuide(self, enable: bool = True, **fields):
         """Turns the learned proposal on or off; other fields as in
@@ -382,6 +383,11 @@ class Integrator:
         st = capi.clhip_guide_stats()
         self._check(self._lib.clhip_guide_stats_get(self._ctx, C.byref(st)))
         return {k: getattr(st, k) for k, _ in capi.clhip_guide_stats._fields_}
+
+    def guideFp8Scales(self) -> np.ndarray:
+        s = np.empty(10, np.float32)
+        self._check(self._lib.clhip_guide_fp8_scales(self._ctx, capi.fptr(s)))
+        return s
 
     def guideInfer(self, q) -> np.ndarray:
         """Network outputs (n x 40) for queries (n x 9)."""

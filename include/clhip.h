@@ -299,6 +299,7 @@ typedef struct
     uint32_t batch_size;    /* samples per training step, multiple of 1024 (0 = 65536) */
     float    learning_rate; /* Adam (0 = 1e-3) */
     uint32_t max_steps;     /* stop training after this many steps (0 = never) */
+    uint32_t precision;     /* inference: 0 = bfloat16, 1 = FP8 e4m3 with per-layer scales (ablation) */
 } clhip_guide;
 
 typedef struct
@@ -318,7 +319,11 @@ int clhip_guide_reset(clhip_ctx* ctx, uint64_t seed);
 int clhip_guide_get_weights(clhip_ctx* ctx, float* weights, uint32_t n);
 int clhip_guide_set_weights(clhip_ctx* ctx, const float* weights, uint32_t n);
 int clhip_guide_stats_get(clhip_ctx* ctx, clhip_guide_stats* out);
-/* Network outputs (40 per query) for n queries of 9 floats. */
+/* The FP8 scales (10 floats: per layer the activation scale and the
+ * accumulator multiplier) of the current e4m3 image. */
+int clhip_guide_fp8_scales(clhip_ctx* ctx, float* scales10);
+/* Network outputs (40 per query) for n queries of 9 floats, at the
+ * inference precision of clhip_set_guide. */
 int clhip_guide_infer(clhip_ctx* ctx, const float* q9, uint32_t n, float* out40);
 /* One Adam step on n samples of 13 floats (query 9, direction 3, weight);
  * grad (32768 floats, optional) receives the summed gradient. */

@@ -40,6 +40,35 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def e4m3(x):
+    """Round to OCP FP8 e4m3 (nearest, ties to even; saturated at +-448;
+    subnormal step 2^-9), as float64."""
+    x = np.clip(np.asarray(x, np.float64), -448.0, 448.0)
+    a = np.abs(x)
+    _, e = np.frexp(a)
+    step = np.where(a >= 2.0 ** -6, np.ldexp(1.0, e - 4), 2.0 ** -9)
+    return np.sign(x) * (np.round(a / step) * step)
+
+
+def forward_fp8(Ws, X, scl):
+    """FP8 inference as the renderer's ablation computes it: the input of
+    layer l enters as e4m3(x * as_l) and the weights as e4m3(W * ws_l) with
+    ws_l = 448 / max|W_l| (float32), the accumulator is scaled by scl[2l+1]
+    (= 1 / (as_l ws_l)); hidden inputs are relu(acc) * (scl[2l-1] scl[2l])."""
+    f32 = np.float32
+    scl = np.asarray(scl, np.float32)
+    h = e4m3(np.asarray(X, np.float32).astype(np.float64) * float(scl[0]))
+    acc = None
+    for l in range(5):
+        W = np.asarray(Ws[l], np.float32)
+        ws = f32(448.0) / f32(np.abs(W).max())
+        Wq = e4m3(W.astype(np.float64) * float(ws)).astype(np.float64)
+        if l > 0:
+            h = e4m3(np.maximum(acc, 0.0) * float(f32(scl[2 * l - 1]) * f32(scl[2 * l])))
+        acc = h @ Wq
+    return acc * float(scl[9])
+
+
 def split_params(p):
     """Flat parameter vector -> [W0 (256x64), W1..W4 (64x64)]."""
     p = np.asarray(p, np.float64)

@@ -76,6 +76,37 @@ def test_inference_matches_oracle(integ):
     assert np.sqrt(np.mean(e_64 ** 2)) < 2e-2, np.sqrt(np.mean(e_64 ** 2))
 
 
+def test_fp8_inference_matches_oracle(integ):
+    """The e4m3 ablation against the oracle's FP8 emulation with the
+    device's scales, and its accuracy cost against bf16."""
+    rng = np.random.default_rng(11)
+    integ.guideReset(3)
+    p = integ.guideWeights().astype(np.float64)
+    Ws = G.split_params(p)
+    Ws[4][:, :40] *= 10.0
+    p = G.join_params(Ws).astype(np.float32)
+    integ.setGuideWeights(p)
+    integ.setGuide(False, precision=1)
+    try:
+        q = rand_queries(rng, 1000)
+        got = integ.guideInfer(q).astype(np.float64)
+        scl = integ.guideFp8Scales()
+    finally:
+        integ.setGuide(False, precision=0)
+    Wd = G.split_params(p.astype(np.float64))
+    ref8 = G.forward_fp8(Wd, G.encode(q), scl)[:, :40]
+    ref64, _ = G.forward(Wd, G.encode(q), bf=False)
+    ref64 = ref64[:, :40]
+    scale = np.sqrt(np.mean(ref64 ** 2))
+    e8 = np.abs(got - ref8) / scale
+    # rare e4m3 rounding flips (float32 vs float64 products before rounding)
+    assert np.median(e8) < 1e-3 and np.mean(e8) < 1e-2, (np.median(e8), np.mean(e8))
+    # the ablation's cost: FP8 is a few percent off the float64 network
+    rms8 = float(np.sqrt(np.mean(((got - ref64) / scale) ** 2)))
+    print(f"FP8 inference rms error vs float64: {rms8:.4f}")
+    assert rms8 < 0.2
+
+
 def test_training_gradient_and_adam_match_oracle(integ):
     rng = np.random.default_rng(2)
     integ.guideReset(5)

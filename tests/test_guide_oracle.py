@@ -32,6 +32,32 @@ def test_bf16_rounding_known_patterns():
     assert G.bf16(3.14159265) == 3.140625
 
 
+def test_e4m3_rounding_known_values():
+    assert G.e4m3(1.0) == 1.0 and G.e4m3(448.0) == 448.0 and G.e4m3(1e6) == 448.0
+    assert G.e4m3(1.0625) == 1.0                      # tie (step 1/8) -> even
+    assert G.e4m3(1.1875) == 1.25                     # tie -> even (mantissa 2)
+    assert G.e4m3(2.0 ** -9) == 2.0 ** -9             # smallest subnormal
+    assert G.e4m3(2.0 ** -10) == 0.0                  # tie with 0 -> even (0)
+    assert G.e4m3(-3.3) == -3.25
+    # brute force against the format: zero, 7 subnormals m 2^-9, normals
+    # (1 + m/8) 2^(e-7) for e = 1..15 except e = 15, m = 7 (NaN): 127 values
+    vals = [m * 2.0 ** -9 for m in range(8)]
+    vals += [(1 + m / 8) * 2.0 ** (e - 7) for e in range(1, 16) for m in range(8) if not (e == 15 and m == 7)]
+    vals = np.array(sorted(vals))
+    assert len(vals) == 127 and vals[-1] == 448.0
+    assert np.all(G.e4m3(vals) == vals)
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(0, 448, 20000), rng.uniform(0, 0.02, 5000), (vals[1:] + vals[:-1]) / 2])
+    got = G.e4m3(x)
+    d = np.abs(x[:, None] - vals[None, :])
+    best = d.min(axis=1)
+    # nearest value; at a tie, the one with an even mantissa (even index in the sorted list)
+    for xi, gi, bi, row in zip(x, got, best, d):
+        cand = np.nonzero(row == bi)[0]
+        want = vals[cand[0]] if len(cand) == 1 else vals[[c for c in cand if c % 2 == 0][0]]
+        assert gi == want, (xi, gi, want)
+
+
 def test_encoding_known_values():
     q = np.zeros((1, 9))
     q[0, :] = [0.25, 0.5, 0.0, 1.0, -0.5, 0.125, 0.75, -1.0, 0.3]
