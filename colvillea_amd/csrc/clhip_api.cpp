@@ -538,6 +538,21 @@ void apply_launch(clhip_ctx* c)
 
 bool use_graph(const clhip_ctx* c) { return !c->timing && c->launch.graph != 2u; }
 
+/* WavefrontDirectLightingIntegrator (direct.cpp:84-201,
+ * raygeneration.cu:139-282): at the first hit one light sample and one BSDF
+ * sample, MIS-weighted against each other, the BSDF ray counting only if it
+ * escapes to the dome.  That is the path tracer stopped after the second
+ * segment without Russian roulette: the same kernels, RNG draws and MIS
+ * weights (with the reference's defects corrected as in the path tracer). */
+void direct_lighting_depth(const clhip_ctx* c, ViewRec& v)
+{
+    if (c->itype == CLHIP_INTEGRATOR_DIRECT_LIGHTING)
+    {
+        v.max_depth = 2u;
+        v.rr_depth  = 0u;
+    }
+}
+
 /* Copies the batch tables to their device arrays, in stream order: the
  * previous render may still read the old tables, so the copies are queued
  * behind it from pinned staging that stays untouched until they ran. */
@@ -1391,8 +1406,6 @@ int clhip_render(clhip_ctx* ctx, uint32_t spp)
 {
     if (!ctx)
         return fail(nullptr, CLHIP_E_INVALID, "ctx is NULL");
-    if (ctx->itype != CLHIP_INTEGRATOR_PATH_TRACING)
-        return fail(ctx, CLHIP_E_UNSUPPORTED, "integrator type %u is not available yet", ctx->itype);
     if (!ctx->have_camera)
         return fail(ctx, CLHIP_E_STATE, "no camera: call clhip_set_camera first");
     int rc = sync_instances(ctx);
@@ -1420,6 +1433,7 @@ int clhip_render(clhip_ctx* ctx, uint32_t spp)
     ViewRec v     = ctx->cam;
     v.max_depth   = ctx->params.max_depth;
     v.rr_depth    = ctx->params.rr_depth;
+    direct_lighting_depth(ctx, v);
     v.lane        = ctx->params.view;
     v.film_offset = 0;
     std::vector<ViewRec>    views{v};
@@ -1475,8 +1489,6 @@ int clhip_render_views(clhip_ctx* ctx, const clhip_view* views, uint32_t nviews)
 {
     if (!ctx || (!views && nviews))
         return fail(ctx, CLHIP_E_INVALID, "clhip_render_views: bad arguments");
-    if (ctx->itype != CLHIP_INTEGRATOR_PATH_TRACING)
-        return fail(ctx, CLHIP_E_UNSUPPORTED, "integrator type %u is not available yet", ctx->itype);
     int rc = sync_instances(ctx);
     if (rc)
         return rc;
@@ -1494,7 +1506,7 @@ int clhip_render_views(clhip_ctx* ctx, const clhip_view* views, uint32_t nviews)
         if (w.lane >= (1u << 12))
             return fail(ctx, CLHIP_E_INVALID, "view %u: lane %u outside [0, 4096)", i, w.lane);
         film += (uint64_t)w.width * w.height;
-        maxd   = std::max(maxd, w.max_depth);
+        maxd   = std::max(maxd, ctx->itype == CLHIP_INTEGRATOR_DIRECT_LIGHTING ? 2u : w.max_depth);
         maxpix = std::max(maxpix, w.width * w.height);
     }
     if (film > (1ull << 30))
@@ -1515,6 +1527,7 @@ int clhip_render_views(clhip_ctx* ctx, const clhip_view* views, uint32_t nviews)
         v.rr_depth    = w.rr_depth;
         v.lane        = w.lane;
         v.film_offset = off;
+        direct_lighting_depth(ctx, v);
         for (int k = 0; k < 3; ++k)
         {
             v.pos[k] = w.pos[k];
