@@ -62,14 +62,17 @@ def build(verbose: bool = False) -> str:
 
 
 HOST_EXAMPLE = os.path.join(HERE, "host", "render_example")
+SCENE_EXAMPLE = os.path.join(HERE, "host", "scene_example")
 
 
 def build_host_example() -> str:
-    """The C++ host example over host/hip_integrator.h (plain g++, linked
-    against the library with an rpath to colvillea_amd/lib)."""
-    src = os.path.join(HERE, "host", "render_example.cpp")
-    _run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{REPO}", src, f"-L{OUT_DIR}", "-lclpt_hip",
-          "-Wl,-rpath,$ORIGIN/../lib", "-o", HOST_EXAMPLE])
+    """The C++ host examples over host/hip_integrator.h and host/hip_scene.h
+    (plain g++, linked against the library with an rpath to
+    colvillea_amd/lib)."""
+    for name, out in (("render_example.cpp", HOST_EXAMPLE), ("scene_example.cpp", SCENE_EXAMPLE)):
+        src = os.path.join(HERE, "host", name)
+        _run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{REPO}", src, f"-L{OUT_DIR}", "-lclpt_hip",
+              "-Wl,-rpath,$ORIGIN/../lib", "-o", out])
     return HOST_EXAMPLE
 
 

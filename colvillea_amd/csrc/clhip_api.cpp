@@ -1332,6 +1332,13 @@ int clhip_set_camera(clhip_ctx* ctx, const float pos[3], const float d00[3], con
 {
     if (!ctx || !pos || !d00 || !ddu || !ddv)
         return fail(ctx, CLHIP_E_INVALID, "clhip_set_camera: bad arguments");
+    /* an unchanged camera keeps the progressive film (path.h:73-80) */
+    bool same = ctx->have_camera;
+    for (int k = 0; k < 3 && same; ++k)
+        same = ctx->cam.pos[k] == pos[k] && ctx->cam.d00[k] == d00[k] && ctx->cam.ddu[k] == ddu[k] &&
+               ctx->cam.ddv[k] == ddv[k];
+    if (same)
+        return CLHIP_OK;
     for (int k = 0; k < 3; ++k)
     {
         ctx->cam.pos[k] = pos[k];

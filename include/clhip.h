@@ -170,7 +170,8 @@ int clhip_build_entities(clhip_ctx* ctx, const uint32_t* material_index, uint32_
 int clhip_build_emitters(clhip_ctx* ctx, const clhip_emitter* emitters, uint32_t num_emitters, int32_t dome_index);
 
 /* Integrator::updateCamera (integrator.h:80; core::Camera, librender/
- * nodebase/camera.h:8-27).  Resets the progressive accumulation. */
+ * nodebase/camera.h:8-27).  Resets the progressive accumulation when the
+ * camera differs from the current one (path.h:73-80). */
 int clhip_set_camera(clhip_ctx* ctx, const float pos[3], const float d00[3], const float ddu[3], const float ddv[3]);
 int clhip_set_params(clhip_ctx* ctx, const clhip_params* params);
 

@@ -1,6 +1,6 @@
-"""The C++ host mirror of core::Integrator (colvillea_amd/host/
-hip_integrator.h) compiles, links against libclpt_hip.so and drives a render
-in the reference's RenderEngine call order."""
+"""The C++ host mirrors of core::Integrator (colvillea_amd/host/
+hip_integrator.h) and of Scene/RenderEngine (host/hip_scene.h) compile, link
+against libclpt_hip.so and drive renders in the reference's call order."""
 import os
 import subprocess
 
@@ -36,3 +36,32 @@ def test_host_example_renders():
     assert rays[0] == 64 * 48
     assert all(0.05 < m < 5.0 for m in mean)
     assert rays[1] > 0 and rays[1] <= rays[0]
+
+
+def _run_scene(*args):
+    exe = hipbuild.SCENE_EXAMPLE
+    if not os.path.exists(exe):
+        hipbuild.build_host_example()
+    return subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=120)
+
+
+def test_scene_edit_actions_and_compile_outputs():
+    """Scene bookkeeping (scene.h:25-92, scene.cpp:186-345) without a device."""
+    hipbuild.build_host_example()
+    r = _run_scene("--selftest")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "selftest ok" in r.stdout
+
+
+def test_scene_engine_reports_missing_device():
+    r = _run_scene(4)
+    if r.returncode == 0:
+        pytest.skip("a HIP device is present")
+    assert r.returncode == 2 and "no HIP device" in r.stderr
+
+
+@pytest.mark.gpu
+def test_scene_engine_renders_progressively_and_recompiles_on_edits():
+    r = _run_scene(6)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("frames 6 mean")
