@@ -330,6 +330,22 @@ int clhip_guide_infer(clhip_ctx* ctx, const float* q9, uint32_t n, float* out40)
  * grad (32768 floats, optional) receives the summed gradient. */
 int clhip_guide_train(clhip_ctx* ctx, const float* samples13, uint32_t n, float* grad, float* loss);
 
+/* ---- image files (ImageUtils, src/delegate/imageutil.cpp:13-139, without
+ *      stb or FreeImage).  Images are RGBA float32, row 0 at the top.
+ *      Loaders: call with rgba = NULL to get width and height, then with a
+ *      buffer of at least 16 * w * h bytes.  Errors: clhip_image_last_error. */
+const char* clhip_image_last_error(void);
+/* Radiance RGBE .hdr: flat or run-length scanlines, "-Y H +X W"; value =
+ * c * 2^(e-136), alpha 1 (the reference loader's conventions). */
+int clhip_image_load_hdr(const char* path, float* rgba, size_t bytes, uint32_t* w, uint32_t* h);
+int clhip_image_save_hdr(const char* path, const float* rgba, uint32_t w, uint32_t h);
+/* PFM: little-endian float32 RGB, rows bottom to top. */
+int clhip_image_load_pfm(const char* path, float* rgba, size_t bytes, uint32_t* w, uint32_t* h);
+int clhip_image_save_pfm(const char* path, const float* rgba, uint32_t w, uint32_t h);
+/* OpenEXR: uncompressed scanline FLOAT RGBA (the reference saves EXR via
+ * FreeImage, imageutil.cpp:31-79). */
+int clhip_image_save_exr(const char* path, const float* rgba, uint32_t w, uint32_t h);
+
 /* Measurement helper with no reference counterpart: `iters` launches of a
  * plain 16-byte streaming copy of `bytes` bytes; returns ms per copy.  Used
  * as the known-traffic kernel for HBM byte counters and the copy ceiling. */
