@@ -196,6 +196,8 @@ SIGNATURES = {
     "clhip_image_load_pfm": (C.c_int, [C.c_char_p, f32p, C.c_size_t, u32p, u32p]),
     "clhip_image_save_pfm": (C.c_int, [C.c_char_p, f32p, u32, u32]),
     "clhip_image_save_exr": (C.c_int, [C.c_char_p, f32p, u32, u32]),
+    "clhip_mesh_last_error": (C.c_char_p, []),
+    "clhip_mesh_load_obj": (C.c_int, [C.c_char_p, f32p, f32p, f32p, f32p, u32p, u32p, u32p, u32p]),
     "clhip_set_guide": (C.c_int, [P, C.POINTER(clhip_guide)]),
     "clhip_guide_reset": (C.c_int, [P, C.c_uint64]),
     "clhip_guide_get_weights": (C.c_int, [P, f32p, u32]),

@@ -23,7 +23,7 @@ OBJ_DIR = os.path.join(REPO, "_build", "clpt")
 LIB = os.path.join(OUT_DIR, "libclpt_hip.so")
 
 ARCH = os.environ.get("CLPT_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["kernels.hip", "guide.hip", "clhip_api.cpp", "bvh_build.cpp", "image_io.cpp"]
+SOURCES = ["kernels.hip", "guide.hip", "clhip_api.cpp", "bvh_build.cpp", "image_io.cpp", "mesh_io.cpp"]
 CFLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-munsafe-fp-atomics",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 

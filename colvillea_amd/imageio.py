@@ -1,6 +1,7 @@
-"""Image files through the library (no stb / FreeImage): Radiance .hdr
-read/write, PFM read/write, OpenEXR write.  Arrays are (h, w, 4) float32
-RGBA with row 0 at the top; RGB input gets alpha 1."""
+"""Files through the library (no stb / FreeImage / assimp): Radiance .hdr
+read/write, PFM read/write, OpenEXR write, Wavefront OBJ import.  Image
+arrays are (h, w, 4) float32 RGBA with row 0 at the top; RGB input gets
+alpha 1."""
 from __future__ import annotations
 
 import ctypes as C
@@ -59,3 +60,26 @@ def save_pfm(path, img) -> None:
 
 def save_exr(path, img) -> None:
     _save(capi.lib().clhip_image_save_exr, path, img)
+
+
+def load_obj(path):
+    """A TriangleMesh from a Wavefront OBJ file (triangulated, smooth
+    normals when absent, flipped texture v, tangents when textured)."""
+    from .integrator import TriangleMesh
+
+    lib = capi.lib()
+    nv, nt, huv = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    p = str(path).encode()
+    if lib.clhip_mesh_load_obj(p, None, None, None, None, None, C.byref(nv), C.byref(nt), C.byref(huv)) != 0:
+        raise OSError(lib.clhip_mesh_last_error().decode())
+    v = np.empty((nv.value, 3), np.float32)
+    n = np.empty((nv.value, 3), np.float32)
+    t = np.empty((nv.value, 3), np.float32)
+    uv = np.empty((nv.value, 2), np.float32)
+    idx = np.empty((nt.value, 3), np.uint32)
+    if lib.clhip_mesh_load_obj(p, capi.fptr(v), capi.fptr(n), capi.fptr(t), capi.fptr(uv), capi.uptr(idx),
+                               C.byref(nv), C.byref(nt), C.byref(huv)) != 0:
+        raise OSError(lib.clhip_mesh_last_error().decode())
+    if huv.value:
+        return TriangleMesh(v, idx, normals=n, tangents=t, uvs=uv)
+    return TriangleMesh(v, idx, normals=n)

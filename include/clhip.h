@@ -346,6 +346,18 @@ int clhip_image_save_pfm(const char* path, const float* rgba, uint32_t w, uint32
  * FreeImage, imageutil.cpp:31-79). */
 int clhip_image_save_exr(const char* path, const float* rgba, uint32_t w, uint32_t h);
 
+/* ---- mesh import (MeshImporter, src/delegate/meshimporter.cpp:41-121,
+ *      without assimp): Wavefront OBJ, polygons fanned into triangles, one
+ *      vertex per distinct (v, vt, vn) corner, area-weighted smooth normals
+ *      when the file has none, flipped texture v, tangents from the UV
+ *      parametrisation (Gram-Schmidt against the normal).  Call with
+ *      vertices = NULL to get the counts, then with arrays of 3 * nv floats
+ *      (vertices, normals, tangents), 2 * nv (uvs) and 3 * nt indices;
+ *      tangents and uvs are written only when *has_uvs. */
+const char* clhip_mesh_last_error(void);
+int clhip_mesh_load_obj(const char* path, float* vertices, float* normals, float* tangents, float* uvs,
+                        uint32_t* indices, uint32_t* num_vertices, uint32_t* num_triangles, uint32_t* has_uvs);
+
 /* Measurement helper with no reference counterpart: `iters` launches of a
  * plain 16-byte streaming copy of `bytes` bytes; returns ms per copy.  Used
  * as the known-traffic kernel for HBM byte counters and the copy ceiling. */
