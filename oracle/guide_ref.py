@@ -164,12 +164,15 @@ def mix_sample(out, usel, u1, u2):
 
 
 def loss_and_grad(params, q, wi, weight, bf=False):
-    """L = -sum_n w_n / W log p(wi_n) and dL/dparams (flat), float64 (bf:
-    the same with the GPU's bfloat16 rounding points mirrored)."""
+    """L = -sum_n w_n / W log p(wi_n) and dL/dparams (flat), float64.
+    bf=True mirrors every bfloat16 rounding point of the GPU (weights,
+    features, activations, backward gradients); bf="forward" is the exact
+    float64 gradient of the bfloat16 network (rounded forward, unrounded
+    backward)."""
     Ws = split_params(params)
-    r = bf16 if bf else (lambda a: a)
+    r = bf16 if bf is True else (lambda a: a)
     X = encode(q)
-    out, A = forward(Ws, X, bf)
+    out, A = forward(Ws, X, bool(bf))
     wn = np.asarray(weight, np.float64) / np.sum(weight)
     wi = np.asarray(wi, np.float64)
     N = out.shape[0]
@@ -199,7 +202,8 @@ def loss_and_grad(params, q, wi, weight, bf=False):
     for l in range(4, -1, -1):
         grads[l] = A[l].T @ g
         if l > 0:
-            g = r((g @ r(Ws[l]).T) * (A[l] > 0))
+            Wl = bf16(Ws[l]) if bf else Ws[l]
+            g = r((g @ Wl.T) * (A[l] > 0))
     return loss, join_params(grads)
 
 
