@@ -110,28 +110,46 @@ def batched_views(integ, args) -> dict:
             "autotune": None if tune is None else {"trials": len(tune["trials"]), "best_sample_ms": tune["ms"]}}
 
 
+GUIDE_TUNED = {"bsdf_fraction": 0.5, "train": 1, "record_paths": 1 << 20, "batch_size": 65536,
+               "learning_rate": 3e-3, "steps_per_batch": 4, "max_steps": 256, "warm_spp": 8}
+GUIDE_SPEC_LR = dict(GUIDE_TUNED, learning_rate=1e-3, max_steps=512)
+
+
 def guided_room(args) -> dict:
     """Time to equal error, guided (online training included) vs plain, on
-    the closed room lit through a window (1920x1080, depth 10)."""
-    from colvillea_amd import guided_eval, scenes
+    the closed room lit through a window (1920x1080, depth 10).  The guided
+    schedule: train on a discarded 8-spp warm-up and the first batches (4
+    Adam steps of 2^16 samples per batch, 2^20 recorded paths per batch),
+    freeze after 256 steps.  Also the same schedule at the configuration's
+    nominal learning rate 1e-3 (512 steps)."""
+    from colvillea_amd import Integrator, IntegratorType, guided_eval, scenes, upload_scene
 
     t = time.perf_counter()
     desc = scenes.make_c3(spp=args.guided_spp)
     log(f"room scene: {desc.num_triangles()} triangles in {time.perf_counter() - t:.1f} s")
     pts = [p for p in (8, 16, 32, 64, 128, 256, 512, 1024) if p < args.guided_spp] + [args.guided_spp]
-    guide = {"bsdf_fraction": 0.5, "train": 1, "record_paths": 32768, "batch_size": 65536, "learning_rate": 1e-3}
-    r = guided_eval.time_to_equal_error(desc, args.guided_ref_spp, pts, guide, log=log)
+    it = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
+    upload_scene(it, desc)
+    r = guided_eval.time_to_equal_error(desc, args.guided_ref_spp, pts, GUIDE_TUNED, it=it, log=log)
+    r2 = guided_eval.time_to_equal_error(desc, args.guided_ref_spp, pts, GUIDE_SPEC_LR, it=it, log=log)
+    it.close()
     nsamp = desc.width * desc.height * args.guided_spp
+
+    def rnd(x, k=3):
+        return None if x is None else round(x, k)
+
     return {"workload": f"closed 10x10x4 room lit through a 1x1 window by the 1080p scene's HDRI, "
                         f"{desc.num_triangles()} triangles, {desc.width}x{desc.height}, {args.guided_spp} spp, "
                         f"depth {desc.max_depth}, seed 4",
-            "t_plain_s": round(r["t_plain"], 3), "t_guided_s": None if r["t_guided"] is None else round(r["t_guided"], 3),
-            "time_ratio": None if r["ratio"] is None else round(r["ratio"], 3), "relmse_plain_final": r["eps"],
+            "t_plain_s": rnd(r["t_plain"]), "t_guided_s": rnd(r["t_guided"]), "time_ratio": rnd(r["ratio"]),
+            "relmse_plain_final": r["eps"], "relmse_guided_final": r["guided"][-1]["relmse"],
             "plain_Msamples_per_s": round(nsamp / r["t_plain"] / 1e6, 2),
             "guided_Msamples_per_s": round(nsamp / r["guided"][-1]["seconds"] / 1e6, 2),
-            "relmse_guided_final": r["guided"][-1]["relmse"], "reference_spp": r["ref_spp"],
-            "train_steps": r["train_steps"], "samples_trained": r["samples_trained"], "guide": r["guide"],
-            "curves": {"plain": r["plain"], "guided": r["guided"]}}
+            "reference_spp": r["ref_spp"], "train_steps": r["train_steps"], "samples_trained": r["samples_trained"],
+            "guide": r["guide"],
+            "spec_lr": {"guide": r2["guide"], "t_guided_s": rnd(r2["t_guided"]), "time_ratio": rnd(r2["ratio"]),
+                        "relmse_guided_final": r2["guided"][-1]["relmse"]},
+            "curves": {"plain": r["plain"], "guided": r["guided"], "guided_spec_lr": r2["guided"]}}
 
 
 def main():

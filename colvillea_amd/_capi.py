@@ -132,6 +132,7 @@ class clhip_guide(C.Structure):
         ("learning_rate", C.c_float),
         ("max_steps", C.c_uint32),
         ("precision", C.c_uint32),
+        ("steps_per_batch", C.c_uint32),
     ]
 
 

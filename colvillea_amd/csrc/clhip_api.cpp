@@ -436,8 +436,13 @@ int guide_reset_weights(clhip_ctx* c, uint64_t seed)
     return CLHIP_OK;
 }
 
-int guide_train_buffers(clhip_ctx* c, uint32_t N)
+int guide_train_buffers(clhip_ctx* c, uint32_t N, uint32_t nsteps = 1)
 {
+    if ((size_t)N * nsteps > c->g_samples.bytes / sizeof(GuideSample))
+    {
+        CLHIP_CHECK(c, hipStreamSynchronize(c->stream));
+        CLHIP_CHECK(c, c->g_samples.alloc((size_t)N * nsteps * sizeof(GuideSample)));
+    }
     if (N <= c->g_N)
         return CLHIP_OK;
     CLHIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -449,7 +454,7 @@ int guide_train_buffers(clhip_ctx* c, uint32_t N)
     CLHIP_CHECK(c, c->g_out.alloc((size_t)N * CLPT_G_HID * sizeof(float)));
     CLHIP_CHECK(c, c->g_loss.alloc((size_t)(N / 256) * sizeof(float)));
     CLHIP_CHECK(c, c->g_partial.alloc((size_t)(N / 1024) * CLPT_G_PARAMS * sizeof(float)));
-    CLHIP_CHECK(c, c->g_samples.alloc((size_t)N * sizeof(GuideSample)));
+    CLHIP_CHECK(c, c->g_samples.alloc((size_t)N * std::max(1u, nsteps) * sizeof(GuideSample)));
     c->g_N = N;
     return CLHIP_OK;
 }
@@ -522,7 +527,7 @@ clpt::GuideTrainBufs guide_bufs(clhip_ctx* c)
     b.W         = c->g_W.as<__bf16>();
     b.WT        = c->g_WT.as<__bf16>();
     b.img       = c->g_img.as<__bf16>();
-    b.img8      = c->g_img8.as<uint8_t>();
+    b.img8      = c->guide.precision == 1u ? c->g_img8.as<uint8_t>() : nullptr; /* FP8 image only when used */
     b.scl       = c->g_scl.as<float>();
     return b;
 }
@@ -649,7 +654,7 @@ BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
     {
         a.gsamples = c->g_samples.as<GuideSample>();
         a.gc       = c->g_gc.as<GuideCounters>();
-        a.gcap     = c->guide.batch_size;
+        a.gcap     = c->guide.batch_size * c->guide.steps_per_batch;
     }
     a.max_depth     = max_depth;
     a.blocks_trace  = c->blocks_trace;
@@ -715,11 +720,13 @@ int issue_train(clhip_ctx* c)
     tl.a = pool_event(c);
     tl.b = pool_event(c);
     CLHIP_CHECK(c, hipEventRecord(tl.a, c->stream));
-    CLHIP_CHECK(c, clpt::launch_guide_train(c->stream, guide_bufs(c), c->guide.batch_size, c->guide.learning_rate,
-                                            nullptr));
+    const uint32_t ns = c->guide.steps_per_batch;
+    for (uint32_t k = 0; k < ns; ++k)
+        CLHIP_CHECK(c, clpt::launch_guide_train(c->stream, guide_bufs(c), c->guide.batch_size, c->guide.learning_rate,
+                                                nullptr, k, ns));
     CLHIP_CHECK(c, hipEventRecord(tl.b, c->stream));
     c->train_timed.push_back(tl);
-    ++c->g_issued;
+    c->g_issued += ns;
     return CLHIP_OK;
 }
 
@@ -810,7 +817,7 @@ int run_batches(clhip_ctx* c, uint32_t nbatches, uint32_t max_depth, double* acc
     {
         int rc = guide_record_buffers(c, c->guide.train ? c->guide.record_paths : 0u, c->qcap);
         if (!rc && c->guide.train)
-            rc = guide_train_buffers(c, c->guide.batch_size);
+            rc = guide_train_buffers(c, c->guide.batch_size, c->guide.steps_per_batch);
         if (rc)
             return rc;
     }
@@ -1996,13 +2003,38 @@ int clhip_set_guide(clhip_ctx* ctx, const clhip_guide* g)
         return fail(ctx, CLHIP_E_INVALID, "record_paths %u above 2^22", v.record_paths);
     if (v.learning_rate == 0.0f)
         v.learning_rate = 1e-3f;
+    if (v.steps_per_batch == 0)
+        v.steps_per_batch = 1;
+    if (v.steps_per_batch > 64)
+        return fail(ctx, CLHIP_E_INVALID, "steps_per_batch %u above 64", v.steps_per_batch);
     if (v.precision > 1u)
         return fail(ctx, CLHIP_E_INVALID, "precision %u: 0 (bfloat16) or 1 (FP8)", v.precision);
     if (!(v.learning_rate > 0.0f))
         return fail(ctx, CLHIP_E_INVALID, "learning_rate must be positive");
-    ctx->guide = v;
+    const bool fp8_on = v.precision == 1u && ctx->guide.precision != 1u;
+    ctx->guide        = v;
     if (v.enable && !ctx->guide_ready)
-        return guide_reset_weights(ctx, 1u);
+    {
+        int rc = guide_reset_weights(ctx, 1u);
+        if (rc)
+            return rc;
+    }
+    if (fp8_on && ctx->guide_ready) /* refresh the e4m3 image for the current weights */
+    {
+        CLHIP_CHECK(ctx, clpt::launch_guide_pack_fp8(ctx->stream, ctx->g_params.as<const float>(),
+                                                     ctx->g_gc.as<GuideCounters>(), ctx->g_img8.as<uint8_t>(),
+                                                     ctx->g_scl.as<float>()));
+        CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    /* training buffers now rather than inside the first render */
+    if (v.enable && v.train)
+    {
+        int rc = guide_train_buffers(ctx, v.batch_size, v.steps_per_batch);
+        if (!rc)
+            rc = guide_record_buffers(ctx, v.record_paths, ctx->qcap);
+        if (rc)
+            return rc;
+    }
     return CLHIP_OK;
 }
 
@@ -2128,10 +2160,9 @@ int clhip_guide_train(clhip_ctx* ctx, const float* s13, uint32_t n, float* grad,
     Tmp g;
     if (grad)
         CLHIP_CHECK(ctx, g.zeros(CLPT_G_PARAMS * sizeof(float)));
-    CLHIP_CHECK(ctx, clpt::launch_guide_train(ctx->stream, guide_bufs(ctx), N, ctx->guide.learning_rate > 0.0f
-                                                                                 ? ctx->guide.learning_rate
-                                                                                 : 1e-3f,
-                                              grad ? g.b.as<float>() : nullptr));
+    CLHIP_CHECK(ctx, clpt::launch_guide_train(ctx->stream, guide_bufs(ctx), N,
+                                              ctx->guide.learning_rate > 0.0f ? ctx->guide.learning_rate : 1e-3f,
+                                              grad ? g.b.as<float>() : nullptr, 0u, 1u));
     CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (grad)
         CLHIP_CHECK(ctx, g.to_host(grad, CLPT_G_PARAMS));

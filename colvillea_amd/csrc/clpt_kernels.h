@@ -68,9 +68,12 @@ hipError_t launch_guide_pack_fp8(hipStream_t s, const float* params, const Guide
 hipError_t launch_guide_pack(hipStream_t s, const float* params, __bf16* W, __bf16* WT, __bf16* img);
 hipError_t launch_guide_gather(hipStream_t s, const float4* V, const float4* E, uint32_t nslots, uint32_t maxd,
                                GuideSample* out, uint32_t cap, GuideCounters* gc);
-/* One Adam step on the samples in b.samples (count in b.gc, at most N, a
- * multiple of 1024).  grad_out (optional) receives the summed gradient. */
-hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N, float lr, float* grad_out);
+/* Adam step `step` of `nsteps` on the gathered samples: slice
+ * [step N, step N + N) of b.samples (count in b.gc; N a multiple of 1024).
+ * The last step empties the buffer.  grad_out (optional) receives the
+ * summed gradient. */
+hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N, float lr, float* grad_out,
+                              uint32_t step, uint32_t nsteps);
 } // namespace clpt
 
 #endif

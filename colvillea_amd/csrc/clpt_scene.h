@@ -199,11 +199,12 @@ struct GuideCounters
     uint32_t           amax[4]; /* max hidden activation of each layer in the last step (float bits) */
 };
 
-/* What the guided shading kernels need.  Training records: per recorded
- * path (slot) and vertex depth j, V holds 4 float4 (query q0..q8, the
- * sampled direction, p_mix and R = f cos / T_next per channel; p_mix = 0
- * marks no record) and E the contribution (rgb) that arrived at depth j
- * (escape of ray j or the shadow ray of vertex j). */
+/* What the guided shading kernels need.  Training records, per vertex depth
+ * j and recorded path (slot), at rec = j * nslots + slot: V is 4 planes of
+ * float4, plane k at k * (CLPT_MAXD + 1) * nslots (query q0..q8, the
+ * sampled direction, then p_mix and R = f cos / T_next per channel; p_mix =
+ * 0 marks no record), and E[rec] the contribution (rgb) that arrived at
+ * depth j (escape of ray j or the shadow ray of vertex j). */
 struct GuideView
 {
     const __bf16*  img;  /* packed bf16 weights */

@@ -22,6 +22,13 @@ namespace
 
 __device__ __forceinline__ float lum3(float r, float g, float b) { return 0.2126f * r + 0.7152f * g + 0.0722f * b; }
 
+/* Samples of training step slice [off, off + N) of the gathered buffer. */
+__device__ __forceinline__ uint32_t guide_count(const GuideCounters* gc, uint32_t off, uint32_t N)
+{
+    const uint32_t c = gc->count;
+    return c > off ? min(c - off, N) : 0u;
+}
+
 } // namespace
 
 /* ------------------------------------------------------------ inference */
@@ -61,13 +68,13 @@ __global__ __launch_bounds__(256) void k_guide_infer(const void* img, const floa
 
 /* ------------------------------------------------------------ training */
 
-__global__ __launch_bounds__(256) void k_guide_features(const GuideSample* s, const GuideCounters* gc, uint32_t N,
-                                                        __bf16* X)
+__global__ __launch_bounds__(256) void k_guide_features(const GuideSample* s, const GuideCounters* gc, uint32_t off,
+                                                        uint32_t N, __bf16* X)
 {
     const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N)
         return;
-    const uint32_t cnt = min(gc->count, N);
+    const uint32_t cnt = guide_count(gc, off, N);
     float          v[9];
     if (n < cnt)
     {
@@ -119,11 +126,20 @@ __global__ __launch_bounds__(256) void k_guide_gemm_nt(const __bf16* __restrict_
     f32x16         acc = guide_zero16();
     const __bf16*  pa  = A + (size_t)(n0 + c) * K + 8u * h;
     const __bf16*  pb  = Bt + (size_t)(j0 + c) * K + 8u * h;
-    for (uint32_t k0 = 0; k0 < K; k0 += 16u)
+    /* K is 64 or 256: fragments of 4 k-steps are loaded before their
+     * products, keeping 8 loads in flight per wave */
+    for (uint32_t k0 = 0; k0 < K; k0 += 64u)
     {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(pa + k0);
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(pb + k0);
-        acc            = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        bf16x8 a[4], b[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+        {
+            a[t] = *reinterpret_cast<const bf16x8*>(pa + k0 + 16u * t);
+            b[t] = *reinterpret_cast<const bf16x8*>(pb + k0 + 16u * t);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[t], b[t], acc, 0, 0, 0);
     }
     float mx = 0.0f;
 #pragma unroll
@@ -164,17 +180,33 @@ __global__ __launch_bounds__(256) void k_guide_gemm_tn(const __bf16* __restrict_
         return;
     const uint32_t lane = __lane_id(), c = lane & 31u, h = lane >> 5;
     const uint32_t k0 = 32u * tk, m0 = 32u * tm;
-    f32x16         acc = guide_zero16();
+    /* per wave, a 16 (n) x 32 tile of A and of G goes through LDS: each lane
+     * loads 8 consecutive elements of one row (16 bytes), then reads its
+     * fragment column-wise */
+    __shared__ __bf16 ta[4][16][32 + 8], tg[4][16][32 + 8];
+    const uint32_t    w  = (threadIdx.x >> 6) & 3u;
+    const uint32_t    lr = lane >> 2, lc = 8u * (lane & 3u);
+    f32x16            acc = guide_zero16();
     for (uint32_t n0 = ch * chunk; n0 < (ch + 1u) * chunk; n0 += 16u)
     {
+        const bf16x8 va = *reinterpret_cast<const bf16x8*>(A + (size_t)(n0 + lr) * K + k0 + lc);
+        const bf16x8 vg = *reinterpret_cast<const bf16x8*>(G + (size_t)(n0 + lr) * CLPT_G_HID + m0 + lc);
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+        {
+            ta[w][lr][lc + s] = va[s];
+            tg[w][lr][lc + s] = vg[s];
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
         bf16x8 a, b;
 #pragma unroll
         for (int s = 0; s < 8; ++s)
         {
-            const size_t n = (size_t)(n0 + 8u * h + s);
-            a[s]           = A[n * K + k0 + c];
-            b[s]           = G[n * CLPT_G_HID + m0 + c];
+            a[s] = ta[w][8u * h + s][c];
+            b[s] = tg[w][8u * h + s][c];
         }
+        __builtin_amdgcn_wave_barrier();
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
     float* P = partial + (size_t)ch * CLPT_G_PARAMS + param_off;
@@ -184,10 +216,10 @@ __global__ __launch_bounds__(256) void k_guide_gemm_tn(const __bf16* __restrict_
 }
 
 /* Sum of the sample weights (one block, fixed order). */
-__global__ __launch_bounds__(1024) void k_guide_wsum(const GuideSample* s, GuideCounters* gc, uint32_t N)
+__global__ __launch_bounds__(1024) void k_guide_wsum(const GuideSample* s, GuideCounters* gc, uint32_t off, uint32_t N)
 {
     __shared__ double part[1024];
-    const uint32_t    cnt = min(gc->count, N);
+    const uint32_t    cnt = guide_count(gc, off, N);
     double            acc = 0.0;
     for (uint32_t n = threadIdx.x; n < cnt; n += blockDim.x)
         acc += (double)s[n].d.x;
@@ -213,11 +245,12 @@ __global__ __launch_bounds__(1024) void k_guide_wsum(const GuideSample* s, Guide
  * and normalised by the batch weight sum W); rows beyond the sample count
  * and output columns 40..63 are zero.  Per-block loss sums to loss_part. */
 __global__ __launch_bounds__(256) void k_guide_loss(const float* __restrict__ out, const GuideSample* s,
-                                                    const GuideCounters* gc, uint32_t N, __bf16* G5, float* loss_part)
+                                                    const GuideCounters* gc, uint32_t off, uint32_t N, __bf16* G5,
+                                                    float* loss_part)
 {
     __shared__ float red[256];
     const uint32_t   n   = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t   cnt = min(gc->count, N);
+    const uint32_t   cnt = guide_count(gc, off, N);
     float            g[CLPT_G_HID];
 #pragma unroll
     for (int k = 0; k < CLPT_G_HID; ++k)
@@ -381,8 +414,10 @@ __global__ void k_guide_adam(float* params, float* m1, float* m2, const float* p
     guide_store_param(p, th, W, WT, img);
 }
 
-/* Close a step: loss of the batch, step counter, empty sample buffer. */
-__global__ __launch_bounds__(256) void k_guide_finish(GuideCounters* gc, const float* loss_part, uint32_t nparts)
+/* Close a step: loss of the batch, step counter; the last step of a batch
+ * empties the sample buffer. */
+__global__ __launch_bounds__(256) void k_guide_finish(GuideCounters* gc, const float* loss_part, uint32_t nparts,
+                                                      int last)
 {
     __shared__ float red[256];
     float            acc = 0.0f;
@@ -404,7 +439,8 @@ __global__ __launch_bounds__(256) void k_guide_finish(GuideCounters* gc, const f
             gc->steps += 1u;
             gc->samples_total += gc->trained;
         }
-        gc->count = 0u;
+        if (last)
+            gc->count = 0u;
     }
 }
 
@@ -421,11 +457,12 @@ __global__ __launch_bounds__(256) void k_guide_gather(const float4* V, const flo
 {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     const bool     live = slot < nslots;
+    const size_t   pl   = (size_t)(CLPT_MAXD + 1) * nslots;
     float          sr = 0.0f, sg = 0.0f, sb = 0.0f;
     for (uint32_t j = maxd; j >= 1u; --j)
     {
-        const size_t rec  = (size_t)(live ? slot : 0u) * (CLPT_MAXD + 1) + j;
-        float4       v3   = live ? V[4 * rec + 3] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const size_t rec  = (size_t)j * nslots + (live ? slot : 0u);
+        float4       v3   = live ? V[3 * pl + rec] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float        w    = 0.0f;
         if (v3.x > 0.0f)
             w = lum3(sr * v3.y, sg * v3.z, sb * v3.w) / v3.x;
@@ -442,9 +479,9 @@ __global__ __launch_bounds__(256) void k_guide_gather(const float4* V, const flo
             if (want && idx < cap)
             {
                 GuideSample g;
-                g.a      = V[4 * rec + 0];
-                g.b      = V[4 * rec + 1];
-                g.c      = V[4 * rec + 2];
+                g.a      = V[rec];
+                g.b      = V[pl + rec];
+                g.c      = V[2 * pl + rec];
                 g.d      = make_float4(w, 0.0f, 0.0f, 0.0f);
                 out[idx] = g;
             }
@@ -495,13 +532,16 @@ hipError_t launch_guide_gather(hipStream_t s, const float4* V, const float4* E, 
     return hipGetLastError();
 }
 
-hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N, float lr, float* grad_out)
+hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N, float lr, float* grad_out,
+                              uint32_t step, uint32_t nsteps)
 {
-    if (N % 1024u != 0u || N == 0u)
+    if (N % 1024u != 0u || N == 0u || step >= nsteps)
         return hipErrorInvalidValue;
-    const uint32_t nt = N / 32u;
-    hipLaunchKernelGGL(k_guide_wsum, dim3(1), dim3(1024), 0, s, b.samples, b.gc, N);
-    hipLaunchKernelGGL(k_guide_features, dim3(N / 256), dim3(256), 0, s, b.samples, b.gc, N, b.X);
+    const uint32_t     nt   = N / 32u;
+    const uint32_t     off  = step * N;
+    const GuideSample* smp  = b.samples + off;
+    hipLaunchKernelGGL(k_guide_wsum, dim3(1), dim3(1024), 0, s, smp, b.gc, off, N);
+    hipLaunchKernelGGL(k_guide_features, dim3(N / 256), dim3(256), 0, s, smp, b.gc, off, N, b.X);
     /* forward: waves = tiles (N/32) x (64/32) */
     const __bf16* in = b.X;
     for (int l = 0; l < 4; ++l)
@@ -514,7 +554,7 @@ hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N
     hipLaunchKernelGGL(k_guide_gemm_nt<GEMM_F32>, dim3(nt * 2 / 4), dim3(256), 0, s, in, b.WT + guide_layer_offset(4),
                        N, (uint32_t)CLPT_G_HID, (uint32_t)CLPT_G_HID, (void*)b.out, (const __bf16*)nullptr,
                        (uint32_t*)nullptr);
-    hipLaunchKernelGGL(k_guide_loss, dim3(N / 256), dim3(256), 0, s, b.out, b.samples, b.gc, N, b.grad[4],
+    hipLaunchKernelGGL(k_guide_loss, dim3(N / 256), dim3(256), 0, s, b.out, smp, b.gc, off, N, b.grad[4],
                        b.loss_part);
     /* backward: G[l] is the gradient w.r.t. the input of layer l+1 (N x 64) */
     const uint32_t chunk = 1024u, nch = N / chunk;
@@ -532,7 +572,8 @@ hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N
     }
     hipLaunchKernelGGL(k_guide_adam, dim3(CLPT_G_PARAMS / 256), dim3(256), 0, s, b.params, b.m1, b.m2, b.partial,
                        nch, b.gc, lr, 0.9f, 0.999f, 1e-8f, b.W, b.WT, b.img, grad_out);
-    hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(256), 0, s, b.gc, b.loss_part, N / 256);
+    hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(256), 0, s, b.gc, b.loss_part, N / 256,
+                       (int)(step + 1u == nsteps));
     if (b.img8)
         hipLaunchKernelGGL(k_guide_pack_fp8, dim3(CLPT_G_LAYERS), dim3(256), 0, s, b.params, b.gc, b.img8, b.scl);
     return hipGetLastError();

@@ -478,12 +478,14 @@ __global__ __launch_bounds__(256) void k_camera(BatchTable bt, GuideView gv, Ray
         q.T[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         if (guide_records(i, tstr, *bt.cursor))
         {
-            const size_t  rec = (size_t)(i / tstr) * (CLPT_MAXD + 1);
-            const float4 z   = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const uint32_t slot = i / tstr;
+            const size_t   pl   = (size_t)(CLPT_MAXD + 1) * gv.nslots;
+            const float4   z    = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             for (uint32_t j = 1; j <= v.max_depth; ++j)
             {
-                gv.V[4 * (rec + j) + 3] = z;
-                gv.E[rec + j]           = z;
+                const size_t rec = (size_t)j * gv.nslots + slot;
+                gv.V[3 * pl + rec] = z;
+                gv.E[rec]          = z;
             }
         }
     }
@@ -714,7 +716,9 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
         const bool     hit    = active && h4.x >= 0.0f;
         const bool     vertex = hit && depth < fr.max_depth;
         const bool     train  = active && guide_records(path, tstr, *bt.cursor);
-        const size_t   rec    = train ? (size_t)(path / tstr) * (CLPT_MAXD + 1) + depth : 0;
+        /* record (depth, slot) of a training path; planes of V are (CLPT_MAXD+1) * nslots apart */
+        const size_t   rec    = train ? (size_t)depth * gv.nslots + path / tstr : 0;
+        const size_t   pl     = (size_t)(CLPT_MAXD + 1) * gv.nslots;
         Surface        si{};
         MaterialRec    mt{};
         if (vertex)
@@ -845,11 +849,11 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
                 }
                 if (train && guided && want_next)
                 {
-                    gv.V[4 * rec + 0] = make_float4(q[0], q[1], q[2], q[3]);
-                    gv.V[4 * rec + 1] = make_float4(q[4], q[5], q[6], q[7]);
-                    gv.V[4 * rec + 2] = make_float4(q[8], ndir.x, ndir.y, ndir.z);
-                    gv.V[4 * rec + 3] = make_float4(pdf, T.x > 0.0f ? fc.x / T.x : 0.0f, T.y > 0.0f ? fc.y / T.y : 0.0f,
-                                                    T.z > 0.0f ? fc.z / T.z : 0.0f);
+                    gv.V[rec]          = make_float4(q[0], q[1], q[2], q[3]);
+                    gv.V[pl + rec]     = make_float4(q[4], q[5], q[6], q[7]);
+                    gv.V[2 * pl + rec] = make_float4(q[8], ndir.x, ndir.y, ndir.z);
+                    gv.V[3 * pl + rec] = make_float4(pdf, T.x > 0.0f ? fc.x / T.x : 0.0f,
+                                                     T.y > 0.0f ? fc.y / T.y : 0.0f, T.z > 0.0f ? fc.z / T.z : 0.0f);
                 }
             }
         }

@@ -289,8 +289,8 @@ int clhip_view_latency(clhip_ctx* ctx, double* ms, uint32_t num_views);
  *      bsdf_fraction * p_bsdf + (1 - bsdf_fraction) * p_mlp and weighted by
  *      that mixture density (one-sample MIS), which also enters the MIS
  *      weights against HDRI sampling.  Training is online: recorded paths
- *      give (query, direction, lum(L f cos) / p) samples, and one Adam step
- *      on the negative log-likelihood follows every batch. ---- */
+ *      give (query, direction, lum(L f cos) / p) samples, and Adam steps on
+ *      the negative log-likelihood follow every batch. ---- */
 typedef struct
 {
     uint32_t enable;        /* 0: plain tracer */
@@ -301,6 +301,7 @@ typedef struct
     float    learning_rate; /* Adam (0 = 1e-3) */
     uint32_t max_steps;     /* stop training after this many steps (0 = never) */
     uint32_t precision;     /* inference: 0 = bfloat16, 1 = FP8 e4m3 with per-layer scales (ablation) */
+    uint32_t steps_per_batch; /* Adam steps after each batch, each on its own batch_size samples (0 = 1) */
 } clhip_guide;
 
 typedef struct
