@@ -130,8 +130,9 @@ def guided_room(args) -> dict:
     pts = [p for p in (8, 16, 32, 64, 128, 256, 512, 1024) if p < args.guided_spp] + [args.guided_spp]
     it = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
     upload_scene(it, desc)
-    r = guided_eval.time_to_equal_error(desc, args.guided_ref_spp, pts, GUIDE_TUNED, it=it, log=log)
-    r2 = guided_eval.time_to_equal_error(desc, args.guided_ref_spp, pts, GUIDE_SPEC_LR, it=it, log=log)
+    ref = guided_eval.reference(it, desc, args.guided_ref_spp, log)
+    r = guided_eval.time_to_equal_error(desc, args.guided_ref_spp, pts, GUIDE_TUNED, it=it, log=log, ref=ref)
+    r2 = guided_eval.time_to_equal_error(desc, args.guided_ref_spp, pts, GUIDE_SPEC_LR, it=it, log=log, ref=ref)
     it.close()
     nsamp = desc.width * desc.height * args.guided_spp
 

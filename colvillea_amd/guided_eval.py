@@ -73,22 +73,28 @@ def _crossing(curve: list, eps: float) -> Optional[float]:
     return None
 
 
+def reference(it: Integrator, desc: SceneDesc, ref_spp: int, log=print):
+    """Plain-tracer reference: the mean of two halves of ref_spp (RNG lanes
+    2 and 3) and the per-pixel variance of that mean."""
+    t0 = time.perf_counter()
+    halves = []
+    for lane in (2, 3):
+        it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=lane)
+        it.render(ref_spp // 2)
+        halves.append(it.readbackSums() / (ref_spp // 2))
+    log(f"reference: {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+    return 0.5 * (halves[0] + halves[1]), ((halves[0] - halves[1]) * 0.5) ** 2
+
+
 def time_to_equal_error(desc: SceneDesc, ref_spp: int, spp_points: List[int], guide: dict,
-                        it: Optional[Integrator] = None, log=print) -> dict:
+                        it: Optional[Integrator] = None, log=print, ref=None) -> dict:
+    """ref: a (reference, variance) pair from reference() to reuse."""
     own = it is None
     if own:
         it = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
         upload_scene(it, desc)
     try:
-        t0 = time.perf_counter()
-        halves = []
-        for lane in (2, 3):
-            it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=lane)
-            it.render(ref_spp // 2)
-            halves.append(it.readbackSums() / (ref_spp // 2))
-        ref = 0.5 * (halves[0] + halves[1])
-        ref_var = ((halves[0] - halves[1]) * 0.5) ** 2
-        log(f"reference: {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+        ref, ref_var = ref if ref is not None else reference(it, desc, ref_spp, log)
         it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0, samples_per_batch=1)
         it.setGuide(False)
         plain = _render_curve(it, spp_points, ref, ref_var)
