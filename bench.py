@@ -228,8 +228,16 @@ def main():
     algo_bytes = rays_traced * QUEUE_BYTES_PER_RAY + nodes * NODE_BYTES + tris * TRI_BYTES
     achieved = algo_bytes / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
     per_launch = algo_bytes / max(1, launches)
+    # HBM bytes per k_trace launch from the FETCH_SIZE / WRITE_SIZE counter
+    # runs (scaled by the copy kernel), committed with their summary
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "k_trace_c1_traffic.json")
+    if os.path.exists(tpath) and desc.width == 1920 and desc.height == 1080:
+        with open(tpath) as fh:
+            traffic = json.load(fh)["hbm_bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/k_trace_c1_1080p.md" if traffic else None,
                 "kernel": "k_trace", "avg_launch_ms": round(trace_ms / max(1, launches), 4),
                 "algorithmic_bytes_per_launch": int(per_launch),
                 "nodes_per_ray": round(nodes / max(1, rays_traced), 2),
