@@ -19,7 +19,7 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 QUEUE_BYTES_PER_RAY = 32 + 16 + 4  # ray o/d float4 x2 read, hit float4 + prim write
-NODE_BYTES = 128
+NODE_BYTES = {4: 128, 8: 96}  # Node4: float child boxes; Node8: byte-quantised child boxes
 TRI_BYTES = 48
 
 
@@ -166,6 +166,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the streaming-copy probe")
     ap.add_argument("--copy-iters", type=int, default=20)
+    ap.add_argument("--bvh-width", type=int, default=4, choices=(4, 8), help="tree width of the C1 scene's BVH")
     ap.add_argument("--no-c2", action="store_true", help="skip the batched 256-view render")
     ap.add_argument("--c2-steps", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true")
@@ -185,10 +186,11 @@ def main():
     desc = scenes.make_c1(width=args.width, height=args.height, spp=args.spp)
     log(f"scene: {desc.num_triangles()} triangles, generated in {time.perf_counter() - t0:.1f} s")
     integ = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
+    integ.setBVHWidth(args.bvh_width)
     t0 = time.perf_counter()
     upload_scene(integ, desc)
     integ.sync()
-    log(f"upload + BVH + HDRI tables: {time.perf_counter() - t0:.1f} s")
+    log(f"BVH{args.bvh_width} upload + BVH + HDRI tables: {time.perf_counter() - t0:.1f} s")
 
     for i in range(args.warmup):
         integ.resetIterationIndex()
@@ -225,20 +227,20 @@ def main():
     samples = desc.width * desc.height * desc.spp * args.steps
     value = samples / t_total / 1e6
     batched = None if args.no_c2 else batched_views(integ, args)
-    algo_bytes = rays_traced * QUEUE_BYTES_PER_RAY + nodes * NODE_BYTES + tris * TRI_BYTES
+    algo_bytes = rays_traced * QUEUE_BYTES_PER_RAY + nodes * NODE_BYTES[args.bvh_width] + tris * TRI_BYTES
     achieved = algo_bytes / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
     per_launch = algo_bytes / max(1, launches)
     # HBM bytes per k_trace launch from the FETCH_SIZE / WRITE_SIZE counter
     # runs (scaled by the copy kernel), committed with their summary
     traffic = None
     tpath = os.path.join(REPO, "profiles", "k_trace_c1_traffic.json")
-    if os.path.exists(tpath) and desc.width == 1920 and desc.height == 1080:
+    if os.path.exists(tpath) and desc.width == 1920 and desc.height == 1080 and args.bvh_width == 4:
         with open(tpath) as fh:
             traffic = json.load(fh)["hbm_bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": "profiles/k_trace_c1_1080p.md" if traffic else None,
-                "kernel": "k_trace", "avg_launch_ms": round(trace_ms / max(1, launches), 4),
+                "kernel": "k_trace", "bvh_width": args.bvh_width, "avg_launch_ms": round(trace_ms / max(1, launches), 4),
                 "algorithmic_bytes_per_launch": int(per_launch),
                 "nodes_per_ray": round(nodes / max(1, rays_traced), 2),
                 "tris_per_ray": round(tris / max(1, rays_traced), 2)}

@@ -159,6 +159,7 @@ SIGNATURES = {
     "clhip_destroy": (C.c_int, [P]),
     "clhip_last_error": (C.c_char_p, [P]),
     "clhip_create_texture": (C.c_int, [P, f32p, u32, u32, u32, C.POINTER(i32)]),
+    "clhip_set_bvh_width": (C.c_int, [P, u32]),
     "clhip_build_blas": (C.c_int, [P, C.POINTER(clhip_mesh), u32]),
     "clhip_build_tlas": (C.c_int, [P, u32p, u32p, f32p, u32]),
     "clhip_build_materials": (C.c_int, [P, C.POINTER(clhip_material), u32]),

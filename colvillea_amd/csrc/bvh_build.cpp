@@ -3,9 +3,12 @@
  * (OptiXDevice::buildOptiXAccelBLASes/TLAS, optixdevice.cpp:136-233).
  *
  * Binned SAH (16 bins per axis) produces a binary tree, which is collapsed
- * to a 4-wide tree by repeatedly opening the child with the largest surface
- * area.  Nodes are the 128-byte Node4 records the gfx950 traversal fetches
- * with eight 16-byte loads.
+ * to a 4- or 8-wide tree by repeatedly opening the child with the largest
+ * surface area.  BVH4 nodes are 128-byte Node4 records with float boxes
+ * (eight 16-byte loads); BVH8 nodes are 96-byte Node8 records with child
+ * boxes quantised to bytes against the node's box, in the spirit of Ylitie,
+ * Karras and Laine, "Efficient Incoherent Ray Traversal on GPUs Through
+ * Compressed Wide BVHs", HPG 2017 (six 16-byte loads).
  */
 #include "bvh_build.h"
 
@@ -205,19 +208,89 @@ double node_area(const Node2& n)
     return b.area();
 }
 
+float pow2_scale(uint8_t e)
+{
+    uint32_t u = (uint32_t)e << 23;
+    float    f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+} // namespace
+
+float node8_lo(const Node8& nd, int a, int c)
+{
+    const volatile float t = (float)nd.qlo[a][c] * pow2_scale(nd.e[a]); /* exact */
+    return nd.p[a] + t;
+}
+float node8_hi(const Node8& nd, int a, int c)
+{
+    const volatile float t = (float)nd.qhi[a][c] * pow2_scale(nd.e[a]);
+    return nd.p[a] + t;
+}
+
+namespace
+{
+/* Quantise the child boxes of a node against its own box [nlo, nhi]: per
+ * axis the smallest power-of-two scale with 255 steps covering the extent,
+ * then per child the largest lower byte and smallest upper byte whose
+ * decoded bounds (float32, as the device computes them) still contain the
+ * child; the scale grows if a child cannot be covered. */
+void quantise(Node8& nd, const float nlo[3], const float nhi[3], const float (*clo)[3], const float (*chi)[3], int nk)
+{
+    for (int a = 0; a < 3; ++a)
+    {
+        nd.p[a]          = nlo[a];
+        const double ext = (double)nhi[a] - (double)nlo[a];
+        int          e   = 127;
+        if (ext > 0.0)
+            e = 127 + (int)std::ceil(std::log2(ext / 255.0));
+        e = std::min(std::max(e, 1), 254);
+        for (;; ++e)
+        {
+            nd.e[a]   = (uint8_t)e;
+            const double sc = (double)pow2_scale((uint8_t)e);
+            bool         ok = true;
+            for (int c = 0; c < nk && ok; ++c)
+            {
+                double ql = std::floor(((double)clo[c][a] - nlo[a]) / sc);
+                double qh = std::ceil(((double)chi[c][a] - nlo[a]) / sc);
+                ql        = std::min(std::max(ql, 0.0), 255.0);
+                qh        = std::min(std::max(qh, 0.0), 255.0);
+                nd.qlo[a][c] = (uint8_t)ql;
+                nd.qhi[a][c] = (uint8_t)qh;
+                while (node8_lo(nd, a, c) > clo[c][a] && nd.qlo[a][c] > 0)
+                    --nd.qlo[a][c];
+                while (node8_hi(nd, a, c) < chi[c][a] && nd.qhi[a][c] < 255)
+                    ++nd.qhi[a][c];
+                ok = node8_lo(nd, a, c) <= clo[c][a] && node8_hi(nd, a, c) >= chi[c][a];
+            }
+            if (ok || e >= 254)
+                break;
+        }
+        for (int c = nk; c < 8; ++c)
+        {
+            nd.qlo[a][c] = 255;
+            nd.qhi[a][c] = 0;
+        }
+    }
+}
+
+template <typename NodeT, int W>
 struct Collapser
 {
     const std::vector<Node2>&                               n2;
-    std::vector<Node4>&                                     nodes;
+    std::vector<NodeT>&                                     nodes;
     const std::function<uint32_t(uint32_t, uint32_t)>&     leaf_ref;
     uint32_t                                                max_depth = 0;
     double                                                  sah       = 0.0;
     double                                                  root_area = 1.0;
 
+    /* The node for BVH2 node idx: its children are found by repeatedly
+     * opening the internal child of largest surface area until W remain. */
     uint32_t make(uint32_t idx, uint32_t depth)
     {
         max_depth = std::max(max_depth, depth);
-        uint32_t kids[4];
+        uint32_t kids[W];
         int      nk = 0;
         if (n2[idx].count > 0)
             kids[nk++] = idx;
@@ -226,7 +299,7 @@ struct Collapser
             kids[nk++] = n2[idx].left;
             kids[nk++] = n2[idx].right;
         }
-        while (nk < 4)
+        while (nk < W)
         {
             int    pick = -1;
             double pa   = -1.0;
@@ -249,62 +322,96 @@ struct Collapser
         }
         const uint32_t me = (uint32_t)nodes.size();
         nodes.emplace_back();
-        Node4 nd;
+        NodeT nd;
         std::memset(&nd, 0, sizeof(nd));
-        for (int c = 0; c < 4; ++c)
-        {
-            nd.child[c] = CLPT_CHILD_EMPTY;
-            nd.lox[c] = nd.loy[c] = nd.loz[c] = 0.0f;
-            nd.hix[c] = nd.hiy[c] = nd.hiz[c] = 0.0f;
-        }
+        float    clo[W][3], chi[W][3];
+        uint32_t ref[W];
+        for (int c = 0; c < W; ++c)
+            ref[c] = CLPT_CHILD_EMPTY;
         sah += kTraverseCost * node_area(n2[idx]) / root_area;
         for (int c = 0; c < nk; ++c)
         {
             const Node2& k = n2[kids[c]];
-            nd.lox[c]      = k.lo[0];
-            nd.loy[c]      = k.lo[1];
-            nd.loz[c]      = k.lo[2];
-            nd.hix[c]      = k.hi[0];
-            nd.hiy[c]      = k.hi[1];
-            nd.hiz[c]      = k.hi[2];
+            for (int a = 0; a < 3; ++a)
+            {
+                clo[c][a] = k.lo[a];
+                chi[c][a] = k.hi[a];
+            }
             if (k.count > 0)
             {
-                nd.child[c] = leaf_ref(k.first, k.count);
+                ref[c] = leaf_ref(k.first, k.count);
                 sah += (double)k.count * node_area(k) / root_area;
             }
             else
-                nd.child[c] = make(kids[c], depth + 1);
+                ref[c] = make(kids[c], depth + 1);
         }
+        fill(nd, n2[idx], clo, chi, ref, nk);
         nodes[me] = nd;
         return me;
     }
-};
-} // namespace
 
-Bvh4Result build_bvh4(const float* lo, const float* hi, uint32_t n, uint32_t leaf_max, std::vector<Node4>& nodes,
-                      std::vector<uint32_t>& order, const std::function<uint32_t(uint32_t, uint32_t)>& leaf_ref)
+    static void fill(Node4& nd, const Node2&, const float (*clo)[3], const float (*chi)[3], const uint32_t* ref, int nk)
+    {
+        for (int c = 0; c < 4; ++c)
+        {
+            nd.child[c] = ref[c];
+            if (c >= nk)
+                continue;
+            nd.lox[c] = clo[c][0];
+            nd.loy[c] = clo[c][1];
+            nd.loz[c] = clo[c][2];
+            nd.hix[c] = chi[c][0];
+            nd.hiy[c] = chi[c][1];
+            nd.hiz[c] = chi[c][2];
+        }
+    }
+    static void fill(Node8& nd, const Node2& self, const float (*clo)[3], const float (*chi)[3], const uint32_t* ref,
+                     int nk)
+    {
+        for (int c = 0; c < 8; ++c)
+            nd.child[c] = ref[c];
+        quantise(nd, self.lo, self.hi, clo, chi, nk);
+    }
+};
+
+template <typename NodeT, int W>
+BvhResult build_wide(const float* lo, const float* hi, uint32_t n, uint32_t leaf_max, std::vector<NodeT>& nodes,
+                     std::vector<uint32_t>& order, const std::function<uint32_t(uint32_t, uint32_t)>& leaf_ref)
 {
-    Bvh4Result res;
-    Builder    b(lo, hi, n, leaf_max, order);
+    BvhResult res;
+    Builder   b(lo, hi, n, leaf_max, order);
     if (n == 0)
     {
         /* an empty tree: one node with no children */
-        Node4 nd;
+        NodeT nd;
         std::memset(&nd, 0, sizeof(nd));
-        for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < W; ++c)
             nd.child[c] = CLPT_CHILD_EMPTY;
         res.root = (uint32_t)nodes.size();
         nodes.push_back(nd);
         return res;
     }
     b.build(n);
-    Collapser c{b.n2, nodes, leaf_ref};
-    c.root_area = std::max(node_area(b.n2[0]), 1e-30);
-    res.root    = c.make(0, 1);
+    Collapser<NodeT, W> c{b.n2, nodes, leaf_ref};
+    c.root_area   = std::max(node_area(b.n2[0]), 1e-30);
+    res.root      = c.make(0, 1);
     res.max_depth = c.max_depth;
     res.sah_cost  = c.sah;
     std::memcpy(res.lo, b.n2[0].lo, sizeof(res.lo));
     std::memcpy(res.hi, b.n2[0].hi, sizeof(res.hi));
     return res;
+}
+} // namespace
+
+BvhResult build_bvh4(const float* lo, const float* hi, uint32_t n, uint32_t leaf_max, std::vector<Node4>& nodes,
+                     std::vector<uint32_t>& order, const std::function<uint32_t(uint32_t, uint32_t)>& leaf_ref)
+{
+    return build_wide<Node4, 4>(lo, hi, n, leaf_max, nodes, order, leaf_ref);
+}
+
+BvhResult build_bvh8(const float* lo, const float* hi, uint32_t n, uint32_t leaf_max, std::vector<Node8>& nodes,
+                     std::vector<uint32_t>& order, const std::function<uint32_t(uint32_t, uint32_t)>& leaf_ref)
+{
+    return build_wide<Node8, 8>(lo, hi, n, leaf_max, nodes, order, leaf_ref);
 }
 } // namespace clpt

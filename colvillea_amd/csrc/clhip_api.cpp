@@ -110,7 +110,9 @@ struct clhip_ctx
     std::vector<TexHost> textures;
 
     /* geometry */
-    std::vector<Node4>       blas_nodes;
+    uint32_t                 bvh_width = 4;
+    std::vector<Node4>       blas_nodes4;
+    std::vector<Node8>       blas_nodes8;
     std::vector<MeshInfo>    meshes;
     std::vector<MeshRec>     mesh_recs;
     bool                     have_blas = false, have_tlas = false;
@@ -250,7 +252,8 @@ void inverse3x4(const float m[12], float out[12])
 SceneView scene_view(const clhip_ctx* c)
 {
     SceneView s{};
-    s.nodes      = c->d_nodes.as<const Node4>();
+    s.nodes      = c->d_nodes.p;
+    s.bvh_width  = c->bvh_width;
     s.tris       = c->d_tris.as<const TriRec>();
     s.inst       = c->d_inst.as<const InstanceRec>();
     s.meshes     = c->d_meshes.as<const MeshRec>();
@@ -1036,11 +1039,27 @@ int clhip_create_texture(clhip_ctx* ctx, const float* data, uint32_t width, uint
     return CLHIP_OK;
 }
 
+int clhip_set_bvh_width(clhip_ctx* ctx, uint32_t width)
+{
+    if (!ctx || (width != 4 && width != 8))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_set_bvh_width: width must be 4 or 8");
+    if (width != ctx->bvh_width)
+    {
+        CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->bvh_width = width;
+        ctx->have_blas = ctx->have_tlas = false; /* the trees must be built again */
+        ctx->blas_nodes4.clear();
+        ctx->blas_nodes8.clear();
+    }
+    return CLHIP_OK;
+}
+
 int clhip_build_blas(clhip_ctx* ctx, const clhip_mesh* meshes, uint32_t num_meshes)
 {
     if (!ctx || (!meshes && num_meshes))
         return fail(ctx, CLHIP_E_INVALID, "clhip_build_blas: bad arguments");
-    std::vector<Node4>    nodes;
+    std::vector<Node4>    nodes4;
+    std::vector<Node8>    nodes8;
     std::vector<TriRec>   tris;
     std::vector<MeshInfo> infos(num_meshes);
     std::vector<MeshRec>  recs(num_meshes);
@@ -1079,10 +1098,12 @@ int clhip_build_blas(clhip_ctx* ctx, const clhip_mesh* meshes, uint32_t num_mesh
             }
         const uint32_t        tri_base = (uint32_t)tris.size();
         std::vector<uint32_t> order;
-        clpt::Bvh4Result      r = clpt::build_bvh4(lo.data(), hi.data(), nt, CLPT_MAX_LEAF_TRIS, nodes, order,
-                                                   [tri_base](uint32_t first, uint32_t count) {
-                                                       return CLPT_TRI_LEAF(tri_base + first, count);
-                                                   });
+        const auto           leaf = [tri_base](uint32_t first, uint32_t count) {
+            return CLPT_TRI_LEAF(tri_base + first, count);
+        };
+        const clpt::BvhResult r = ctx->bvh_width == 8
+                                      ? clpt::build_bvh8(lo.data(), hi.data(), nt, CLPT_MAX_LEAF_TRIS, nodes8, order, leaf)
+                                      : clpt::build_bvh4(lo.data(), hi.data(), nt, CLPT_MAX_LEAF_TRIS, nodes4, order, leaf);
         for (uint32_t i = 0; i < nt; ++i)
         {
             const uint32_t p  = order[i];
@@ -1143,7 +1164,8 @@ int clhip_build_blas(clhip_ctx* ctx, const clhip_mesh* meshes, uint32_t num_mesh
     CLHIP_CHECK(ctx, upload(ctx->d_tan, tan));
     CLHIP_CHECK(ctx, upload(ctx->d_uv, uv));
     CLHIP_CHECK(ctx, upload(ctx->d_idx, idx));
-    ctx->blas_nodes = std::move(nodes);
+    ctx->blas_nodes4 = std::move(nodes4);
+    ctx->blas_nodes8 = std::move(nodes8);
     ctx->meshes     = std::move(infos);
     ctx->mesh_recs  = std::move(recs);
     ctx->have_blas  = true;
@@ -1208,17 +1230,31 @@ int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uin
             }
         }
     }
-    std::vector<Node4>    nodes = ctx->blas_nodes;
     std::vector<uint32_t> order;
-    clpt::Bvh4Result      r = clpt::build_bvh4(lo.data(), hi.data(), num_instances, 1u, nodes, order,
-                                               [&order](uint32_t first, uint32_t) { return CLPT_INST_LEAF(order[first]); });
-    /* stack bound: each 4-wide level leaves at most 3 entries behind, plus
-     * the instance marker and the popped entry */
-    const uint32_t need = 3u * (r.max_depth + max_blas_depth) + 4u;
+    const auto            leaf = [&order](uint32_t first, uint32_t) { return CLPT_INST_LEAF(order[first]); };
+    std::vector<Node4>    nodes4;
+    std::vector<Node8>    nodes8;
+    clpt::BvhResult       r;
+    if (ctx->bvh_width == 8)
+    {
+        nodes8 = ctx->blas_nodes8;
+        r      = clpt::build_bvh8(lo.data(), hi.data(), num_instances, 1u, nodes8, order, leaf);
+    }
+    else
+    {
+        nodes4 = ctx->blas_nodes4;
+        r      = clpt::build_bvh4(lo.data(), hi.data(), num_instances, 1u, nodes4, order, leaf);
+    }
+    /* stack bound: each W-wide level leaves at most W - 1 entries behind,
+     * plus the instance marker and the popped entry */
+    const uint32_t need = (ctx->bvh_width - 1u) * (r.max_depth + max_blas_depth) + 4u;
     if (need > 128u)
         return fail(ctx, CLHIP_E_UNSUPPORTED, "tree too deep for the traversal stack (TLAS %u + BLAS %u levels)",
                     r.max_depth, max_blas_depth);
-    CLHIP_CHECK(ctx, upload(ctx->d_nodes, nodes));
+    if (ctx->bvh_width == 8)
+        CLHIP_CHECK(ctx, upload(ctx->d_nodes, nodes8));
+    else
+        CLHIP_CHECK(ctx, upload(ctx->d_nodes, nodes4));
     ctx->inst        = std::move(inst);
     ctx->inst_entity = std::move(ent);
     ctx->tlas_root   = r.root;

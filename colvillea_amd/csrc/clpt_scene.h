@@ -22,6 +22,25 @@ struct alignas(128) Node4
 };
 static_assert(sizeof(Node4) == 128, "Node4 is two 64-byte halves");
 
+/* ---- BVH8 node: 96 bytes, read as 6 dwordx4.  Child boxes are stored
+ * relative to the node's lower corner p, quantised to bytes per axis with a
+ * power-of-two scale 2^(e - 127): child c spans
+ *   [p + qlo[a][c] * 2^(e_a - 127), p + qhi[a][c] * 2^(e_a - 127)]
+ * on axis a, evaluated in float32 exactly as written (a product of a byte
+ * and a power of two is exact; the sum rounds once).  The builder picks the
+ * bytes so that the decoded box contains the child's true box.  Empty slots
+ * have child = CLPT_CHILD_EMPTY. ---- */
+struct alignas(16) Node8
+{
+    float    p[3];
+    uint8_t  e[3];
+    uint8_t  pad;
+    uint8_t  qlo[3][8];
+    uint8_t  qhi[3][8];
+    uint32_t child[8];
+};
+static_assert(sizeof(Node8) == 96, "Node8 is six 16-byte words");
+
 /* Child reference encoding. */
 #define CLPT_CHILD_EMPTY 0xffffffffu
 #define CLPT_STACK_MARKER 0xfffffffeu /* pops back to the world-space ray */
@@ -96,7 +115,7 @@ struct alignas(16) EmitterRec
 /* Everything a kernel needs to find the scene; passed by value. */
 struct SceneView
 {
-    const Node4*       nodes;
+    const void*        nodes; /* Node4 or Node8 array (the tree width is a kernel template argument) */
     const TriRec*      tris;
     const InstanceRec* inst;
     const MeshRec*     meshes;
@@ -117,6 +136,7 @@ struct SceneView
     const float*       cdfv;
     uint32_t           tlas_root;
     uint32_t           ninst;
+    uint32_t           bvh_width; /* 4 (Node4) or 8 (Node8) */
 };
 
 /* One camera view: film size, path depth, RNG view lane, the view's offset

@@ -139,7 +139,126 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32
     rb = r1;
 }
 
-/* Two-level BVH4 traversal as a per-lane state machine: step() does one
+/* One inner node: test the children's boxes against the ray (slab test,
+ * padded by 1e-5 relative so that rounding never drops a box), push the
+ * hits farther than the nearest (nearest last) with their entry distance,
+ * and continue at the nearest.  Returns false when no child was hit. */
+template <int W>
+__device__ __forceinline__ bool visit_node(const void* nodes, const RayW& r, float tmax, TraverseStack& st,
+                                           uint32_t& ref);
+
+template <>
+__device__ __forceinline__ bool visit_node<4>(const void* nodes, const RayW& r, float tmax, TraverseStack& st,
+                                              uint32_t& ref)
+{
+    const float4* np  = reinterpret_cast<const float4*>(static_cast<const Node4*>(nodes) + ref);
+    const float4  lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+    const uint4   ch  = *reinterpret_cast<const uint4*>(np + 6);
+    float         tn[4];
+    uint32_t      cr[4]   = {ch.x, ch.y, ch.z, ch.w};
+    const float   lo_x[4] = {lox.x, lox.y, lox.z, lox.w}, hi_x[4] = {hix.x, hix.y, hix.z, hix.w};
+    const float   lo_y[4] = {loy.x, loy.y, loy.z, loy.w}, hi_y[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+    const float   lo_z[4] = {loz.x, loz.y, loz.z, loz.w}, hi_z[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+    {
+        const float t0x = (lo_x[c] - r.o.x) * r.inv.x, t1x = (hi_x[c] - r.o.x) * r.inv.x;
+        const float t0y = (lo_y[c] - r.o.y) * r.inv.y, t1y = (hi_y[c] - r.o.y) * r.inv.y;
+        const float t0z = (lo_z[c] - r.o.z) * r.inv.z, t1z = (hi_z[c] - r.o.z) * r.inv.z;
+        float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+        float tfar  = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+        tnear       = tnear * 0.99999f;
+        tfar        = tfar * 1.00001f;
+        tn[c]       = (cr[c] != CLPT_CHILD_EMPTY && tnear <= tfar) ? tnear : CLPT_FLT_MAX;
+    }
+    /* nearest first: a 5-comparator sorting network, fixed indices */
+    cswap(tn[0], cr[0], tn[1], cr[1]);
+    cswap(tn[2], cr[2], tn[3], cr[3]);
+    cswap(tn[0], cr[0], tn[2], cr[2]);
+    cswap(tn[1], cr[1], tn[3], cr[3]);
+    cswap(tn[1], cr[1], tn[2], cr[2]);
+    if (tn[3] < CLPT_FLT_MAX)
+        st.push(cr[3], tn[3]);
+    if (tn[2] < CLPT_FLT_MAX)
+        st.push(cr[2], tn[2]);
+    if (tn[1] < CLPT_FLT_MAX)
+        st.push(cr[1], tn[1]);
+    if (tn[0] < CLPT_FLT_MAX)
+    {
+        ref = cr[0];
+        return true;
+    }
+    return false;
+}
+
+template <>
+__device__ __forceinline__ bool visit_node<8>(const void* nodes, const RayW& r, float tmax, TraverseStack& st,
+                                              uint32_t& ref)
+{
+    const uint4* np = reinterpret_cast<const uint4*>(static_cast<const Node8*>(nodes) + ref);
+    const uint4  w0 = np[0], q0 = np[1], q1 = np[2], q2 = np[3], c0 = np[4], c1 = np[5];
+    /* node origin and per-axis power-of-two scales */
+    const float px = __uint_as_float(w0.x), py = __uint_as_float(w0.y), pz = __uint_as_float(w0.z);
+    const float sx = __uint_as_float((w0.w & 0xffu) << 23);
+    const float sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23);
+    const float sz = __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
+    /* byte planes: lo x (q0.xy), lo y (q0.zw), lo z (q1.xy), hi x (q1.zw), hi y (q2.xy), hi z (q2.zw) */
+    const uint32_t lxw[2] = {q0.x, q0.y}, lyw[2] = {q0.z, q0.w}, lzw[2] = {q1.x, q1.y};
+    const uint32_t hxw[2] = {q1.z, q1.w}, hyw[2] = {q2.x, q2.y}, hzw[2] = {q2.z, q2.w};
+    uint32_t       cr[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    float          tn[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        const int   wi = c >> 2, sh = 8 * (c & 3);
+        const float lx = px + (float)((lxw[wi] >> sh) & 0xffu) * sx;
+        const float ly = py + (float)((lyw[wi] >> sh) & 0xffu) * sy;
+        const float lz = pz + (float)((lzw[wi] >> sh) & 0xffu) * sz;
+        const float hx = px + (float)((hxw[wi] >> sh) & 0xffu) * sx;
+        const float hy = py + (float)((hyw[wi] >> sh) & 0xffu) * sy;
+        const float hz = pz + (float)((hzw[wi] >> sh) & 0xffu) * sz;
+        const float t0x = (lx - r.o.x) * r.inv.x, t1x = (hx - r.o.x) * r.inv.x;
+        const float t0y = (ly - r.o.y) * r.inv.y, t1y = (hy - r.o.y) * r.inv.y;
+        const float t0z = (lz - r.o.z) * r.inv.z, t1z = (hz - r.o.z) * r.inv.z;
+        float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+        float tfar  = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+        tnear       = tnear * 0.99999f;
+        tfar        = tfar * 1.00001f;
+        tn[c]       = (cr[c] != CLPT_CHILD_EMPTY && tnear <= tfar) ? tnear : CLPT_FLT_MAX;
+    }
+    /* nearest first: a 19-comparator sorting network for 8 (fixed indices) */
+    cswap(tn[0], cr[0], tn[2], cr[2]);
+    cswap(tn[1], cr[1], tn[3], cr[3]);
+    cswap(tn[4], cr[4], tn[6], cr[6]);
+    cswap(tn[5], cr[5], tn[7], cr[7]);
+    cswap(tn[0], cr[0], tn[4], cr[4]);
+    cswap(tn[1], cr[1], tn[5], cr[5]);
+    cswap(tn[2], cr[2], tn[6], cr[6]);
+    cswap(tn[3], cr[3], tn[7], cr[7]);
+    cswap(tn[0], cr[0], tn[1], cr[1]);
+    cswap(tn[2], cr[2], tn[3], cr[3]);
+    cswap(tn[4], cr[4], tn[5], cr[5]);
+    cswap(tn[6], cr[6], tn[7], cr[7]);
+    cswap(tn[2], cr[2], tn[4], cr[4]);
+    cswap(tn[3], cr[3], tn[5], cr[5]);
+    cswap(tn[1], cr[1], tn[4], cr[4]);
+    cswap(tn[3], cr[3], tn[6], cr[6]);
+    cswap(tn[1], cr[1], tn[2], cr[2]);
+    cswap(tn[3], cr[3], tn[4], cr[4]);
+    cswap(tn[5], cr[5], tn[6], cr[6]);
+#pragma unroll
+    for (int c = 7; c >= 1; --c)
+        if (tn[c] < CLPT_FLT_MAX)
+            st.push(cr[c], tn[c]);
+    if (tn[0] < CLPT_FLT_MAX)
+    {
+        ref = cr[0];
+        return true;
+    }
+    return false;
+}
+
+/* Two-level BVH traversal (W = 4 or 8 wide) as a per-lane state machine: step() does one
  * unit of work (one node, or one triangle of a leaf) and reports when the
  * ray is finished, so a lane whose ray ends can start its next ray at once
  * instead of idling until the slowest lane of its wave is done.
@@ -147,7 +266,7 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32
  * rays with closest-hit flags).  The world ray is (wo, wd); inside an
  * instance the object-space ray replaces it until the marker pushed on entry
  * is popped. */
-template <bool ANY>
+template <bool ANY, int W>
 struct Tracer
 {
     f3       wo, wd;
@@ -220,43 +339,8 @@ struct Tracer
                     st.push(CLPT_STACK_MARKER, -CLPT_FLT_MAX);
                     ref = ip->blas_root;
                 }
-                const float4* np  = reinterpret_cast<const float4*>(sc.nodes + ref);
-                const float4  lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-                const uint4   ch  = *reinterpret_cast<const uint4*>(np + 6);
                 ++nnodes;
-                float       tn[4];
-                uint32_t    cr[4]   = {ch.x, ch.y, ch.z, ch.w};
-                const float lo_x[4] = {lox.x, lox.y, lox.z, lox.w}, hi_x[4] = {hix.x, hix.y, hix.z, hix.w};
-                const float lo_y[4] = {loy.x, loy.y, loy.z, loy.w}, hi_y[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-                const float lo_z[4] = {loz.x, loz.y, loz.z, loz.w}, hi_z[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                {
-                    const float t0x = (lo_x[c] - r.o.x) * r.inv.x, t1x = (hi_x[c] - r.o.x) * r.inv.x;
-                    const float t0y = (lo_y[c] - r.o.y) * r.inv.y, t1y = (hi_y[c] - r.o.y) * r.inv.y;
-                    const float t0z = (lo_z[c] - r.o.z) * r.inv.z, t1z = (hi_z[c] - r.o.z) * r.inv.z;
-                    float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
-                    float tfar  = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best.t));
-                    tnear       = tnear * 0.99999f;
-                    tfar        = tfar * 1.00001f;
-                    tn[c]       = (cr[c] != CLPT_CHILD_EMPTY && tnear <= tfar) ? tnear : CLPT_FLT_MAX;
-                }
-                /* nearest first: a 5-comparator sorting network, fixed indices */
-                cswap(tn[0], cr[0], tn[1], cr[1]);
-                cswap(tn[2], cr[2], tn[3], cr[3]);
-                cswap(tn[0], cr[0], tn[2], cr[2]);
-                cswap(tn[1], cr[1], tn[3], cr[3]);
-                cswap(tn[1], cr[1], tn[2], cr[2]);
-                if (tn[3] < CLPT_FLT_MAX)
-                    st.push(cr[3], tn[3]);
-                if (tn[2] < CLPT_FLT_MAX)
-                    st.push(cr[2], tn[2]);
-                if (tn[1] < CLPT_FLT_MAX)
-                    st.push(cr[1], tn[1]);
-                if (tn[0] < CLPT_FLT_MAX)
-                    ref = cr[0];
-                else
-                    need_pop = true;
+                need_pop = !visit_node<W>(sc.nodes, r, best.t, st, ref);
             }
         }
         if (leaf_left > 0)
@@ -491,6 +575,7 @@ __global__ __launch_bounds__(256) void k_camera(BatchTable bt, GuideView gv, Ray
     }
 }
 
+template <int W>
 __global__ __launch_bounds__(256) void k_trace(SceneView sc, RayQueue q, const uint32_t* count, uint32_t depth,
                                                HitBuf hb, uint2* gstack, unsigned long long* ctr)
 {
@@ -500,7 +585,7 @@ __global__ __launch_bounds__(256) void k_trace(SceneView sc, RayQueue q, const u
     const float    tmin = depth == 1 ? 0.0f : CLPT_RAY_EPS;
     uint32_t       i    = st.gid;
     bool           active = i < n;
-    Tracer<false>  tr;
+    Tracer<false, W> tr;
     if (active)
     {
         const float4 o = q.o[i], d = q.d[i];
@@ -876,6 +961,7 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
     }
 }
 
+template <int W>
 __global__ __launch_bounds__(256) void k_shadow(SceneView sc, ShadowQueue sq, uint32_t depth, double* accum,
                                                 BatchCounters* bc, uint2* gstack, unsigned long long* ctr,
                                                 float4* guide_E)
@@ -885,7 +971,7 @@ __global__ __launch_bounds__(256) void k_shadow(SceneView sc, ShadowQueue sq, ui
     const uint32_t n      = bc->shadow[depth];
     uint32_t       i      = st.gid;
     bool           active = i < n;
-    Tracer<true>   tr;
+    Tracer<true, W> tr;
     if (active)
     {
         const float4 o = sq.o[i], d = sq.d[i];
@@ -1175,6 +1261,7 @@ __global__ void k_dome_eval_batch(SceneView sc, const float* dir3, uint32_t n, f
     }
 }
 
+template <int W>
 __global__ __launch_bounds__(256) void k_trace_batch(SceneView sc, const float4* rays, uint32_t n, int any,
                                                      float4* out, uint32_t* prim, uint2* gstack)
 {
@@ -1186,7 +1273,7 @@ __global__ __launch_bounds__(256) void k_trace_batch(SceneView sc, const float4*
         const f3     ro{a.x, a.y, a.z}, rd{b.x, b.y, b.z};
         if (any)
         {
-            Tracer<true> tr;
+            Tracer<true, W> tr;
             tr.start(sc, st, ro, rd, a.w, b.w);
             while (!tr.step(sc, st, nn, nt))
             {
@@ -1196,7 +1283,7 @@ __global__ __launch_bounds__(256) void k_trace_batch(SceneView sc, const float4*
         }
         else
         {
-            Tracer<false> tr;
+            Tracer<false, W> tr;
             tr.start(sc, st, ro, rd, a.w, b.w);
             while (!tr.step(sc, st, nn, nt))
             {
@@ -1240,7 +1327,10 @@ hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const GuideView& g
 hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, const uint32_t* count, uint32_t depth,
                         const HitBuf& hb, uint2* gstack, unsigned long long* ctr, uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(256), 0, s, sc, q, count, depth, hb, gstack, ctr);
+    if (sc.bvh_width == 8)
+        hipLaunchKernelGGL(k_trace<8>, dim3(blocks), dim3(256), 0, s, sc, q, count, depth, hb, gstack, ctr);
+    else
+        hipLaunchKernelGGL(k_trace<4>, dim3(blocks), dim3(256), 0, s, sc, q, count, depth, hb, gstack, ctr);
     return hipGetLastError();
 }
 
@@ -1268,7 +1358,10 @@ hipError_t launch_shade_guided(hipStream_t s, const SceneView& sc, const BatchTa
 hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& sq, uint32_t depth, double* accum,
                          BatchCounters* bc, uint2* gstack, unsigned long long* ctr, float4* guide_E, uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_shadow, dim3(blocks), dim3(256), 0, s, sc, sq, depth, accum, bc, gstack, ctr, guide_E);
+    if (sc.bvh_width == 8)
+        hipLaunchKernelGGL(k_shadow<8>, dim3(blocks), dim3(256), 0, s, sc, sq, depth, accum, bc, gstack, ctr, guide_E);
+    else
+        hipLaunchKernelGGL(k_shadow<4>, dim3(blocks), dim3(256), 0, s, sc, sq, depth, accum, bc, gstack, ctr, guide_E);
     return hipGetLastError();
 }
 
@@ -1339,9 +1432,9 @@ uint32_t resident_blocks_per_cu(int which)
     int n = 0;
     hipError_t e;
     if (which == 0)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace, 256, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<4>, 256, 0);
     else if (which == 1)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shadow, 256, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shadow<4>, 256, 0);
     else if (which == 2)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_shade, 256, 0);
     else
@@ -1358,7 +1451,10 @@ hipError_t launch_copy_probe(hipStream_t s, const float4* src, float4* dst, size
 hipError_t launch_trace_batch(hipStream_t s, const SceneView& sc, const float4* rays, uint32_t n, int any, float4* out,
                               uint32_t* prim, uint2* gstack, uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_trace_batch, dim3(blocks), dim3(256), 0, s, sc, rays, n, any, out, prim, gstack);
+    if (sc.bvh_width == 8)
+        hipLaunchKernelGGL(k_trace_batch<8>, dim3(blocks), dim3(256), 0, s, sc, rays, n, any, out, prim, gstack);
+    else
+        hipLaunchKernelGGL(k_trace_batch<4>, dim3(blocks), dim3(256), 0, s, sc, rays, n, any, out, prim, gstack);
     return hipGetLastError();
 }
 } // namespace clpt

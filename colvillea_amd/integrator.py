@@ -163,6 +163,10 @@ class Integrator:
         check(rc, self._ctx)
 
     # -- build* (integrator.h:47-58)
+    def setBVHWidth(self, width: int):
+        """4 or 8; call before buildBLAS (no reference counterpart)."""
+        self._check(self._lib.clhip_set_bvh_width(self._ctx, int(width)))
+
     def buildBLAS(self, trimeshes: Sequence[TriangleMesh]):
         arr = (capi.clhip_mesh * max(1, len(trimeshes)))()
         keep = []

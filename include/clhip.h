@@ -145,6 +145,13 @@ const char* clhip_last_error(const clhip_ctx* ctx); /* ctx may be NULL: last glo
 int clhip_create_texture(clhip_ctx* ctx, const float* data, uint32_t width, uint32_t height, uint32_t channels,
                          int32_t* texture_id);
 
+/* Tree width of the acceleration structures built from now on: 4 (default;
+ * 128-byte nodes with float boxes) or 8 (96-byte nodes with child boxes
+ * quantised to bytes).  No reference counterpart (OptiX builds its own).
+ * Results do not depend on it; changing it requires building the BLAS and
+ * TLAS again. */
+int clhip_set_bvh_width(clhip_ctx* ctx, uint32_t width);
+
 /* Integrator::buildBLAS (integrator.h:47; OptiXDevice::buildOptiXAccelBLASes,
  * optixdevice.cpp:136-192): one BVH4 per mesh. */
 int clhip_build_blas(clhip_ctx* ctx, const clhip_mesh* meshes, uint32_t num_meshes);

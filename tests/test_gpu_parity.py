@@ -35,10 +35,13 @@ def bits_equal(a, b):
     return np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-@pytest.fixture(scope="module")
-def c0_small():
+@pytest.fixture(scope="module", params=[4, 8], ids=["bvh4", "bvh8"])
+def c0_small(request):
+    """The small C0 scene, built once per BVH width (4: float boxes, 8:
+    byte-quantised boxes); every test on it must pass for both trees."""
     desc = scenes.make_c0(width=64, height=48, spp=4, hdri_w=128, hdri_h=64)
     integ = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
+    integ.setBVHWidth(request.param)
     upload_scene(integ, desc)
     yield desc, integ
     integ.close()
