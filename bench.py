@@ -153,6 +153,51 @@ def guided_room(args) -> dict:
             "curves": {"plain": r["plain"], "guided": r["guided"], "guided_spec_lr": r2["guided"]}}
 
 
+def stress_c4(args) -> dict:
+    """The stress configuration (3840x2160, 128 spp, depth 12, 16.8 M-triangle
+    fBm grid + 2000 instanced icospheres, half dielectric, 8192x4096 HDRI):
+    plain throughput over the full frame, then the FP8 ablation of the
+    learned proposal -- the same guided schedule with the BF16 and the FP8
+    network, each against plain at equal error (reference: plain at
+    --c4-ref-spp)."""
+    from colvillea_amd import Integrator, IntegratorType, guided_eval, scenes, upload_scene
+
+    t = time.perf_counter()
+    desc = scenes.make_c4()
+    log(f"C4: {desc.num_triangles()} triangles generated in {time.perf_counter() - t:.1f} s")
+    it = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
+    t = time.perf_counter()
+    upload_scene(it, desc)
+    it.sync()
+    log(f"C4 upload + BVH + HDRI tables: {time.perf_counter() - t:.1f} s")
+    it.render(1)  # warm-up
+    it.resetIterationIndex()
+    it.sync()
+    t = time.perf_counter()
+    it.render(desc.spp)
+    it.sync()
+    dt = time.perf_counter() - t
+    plain = desc.width * desc.height * desc.spp / dt / 1e6
+    log(f"C4 plain: {desc.spp} spp in {dt:.2f} s -> {plain:.1f} Msamples/s")
+    pts = [p for p in (4, 8, 16, 32, 64) if p < args.c4_guided_spp] + [args.c4_guided_spp]
+    ref = guided_eval.reference(it, desc, args.c4_ref_spp, log)
+    ablation = {}
+    for name, prec in (("bf16", 0), ("fp8", 1)):
+        r = guided_eval.time_to_equal_error(desc, args.c4_ref_spp, pts, dict(GUIDE_TUNED, precision=prec), it=it,
+                                            log=log, ref=ref)
+        ablation[name] = {"time_ratio": None if r["ratio"] is None else round(r["ratio"], 3),
+                          "t_guided_s": None if r["t_guided"] is None else round(r["t_guided"], 3),
+                          "t_plain_s": round(r["t_plain"], 3), "relmse_plain_final": r["eps"],
+                          "relmse_guided_final": r["guided"][-1]["relmse"],
+                          "guided_s_final": round(r["guided"][-1]["seconds"], 3),
+                          "curve": r["guided"]}
+    it.close()
+    return {"workload": f"C4: {desc.width}x{desc.height}, {desc.spp} spp, depth {desc.max_depth}, "
+                        f"{desc.num_triangles()} triangles, 8192x4096 HDRI, seed {desc.info['seed']}",
+            "plain_Msamples_per_s": round(plain, 2), "ablation_spp": args.c4_guided_spp,
+            "reference_spp": args.c4_ref_spp, "guide": dict(GUIDE_TUNED), "ablation": ablation}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +218,9 @@ def main():
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain room evaluation")
     ap.add_argument("--guided-ref-spp", type=int, default=1024)
     ap.add_argument("--guided-spp", type=int, default=256)
+    ap.add_argument("--c4", action="store_true", help="also run the 4K stress scene and the FP8 ablation")
+    ap.add_argument("--c4-ref-spp", type=int, default=256)
+    ap.add_argument("--c4-guided-spp", type=int, default=32)
     ap.add_argument("--autotune-log", default=None, help="write the autotuner's trials here (JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -258,6 +306,7 @@ def main():
     integ.close()
     integ = None
     guided = None if args.no_guided else guided_room(args)
+    stress = stress_c4(args) if args.c4 else None
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(desc, args.cpu_seconds, args.cpu_threads)
@@ -270,6 +319,8 @@ def main():
                                   f"instanced icospheres), 2048x1024 log-normal HDRI, seed {desc.info['seed']}",
                       "width": desc.width, "height": desc.height, "spp": desc.spp, "max_depth": desc.max_depth},
            "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "guided": guided}
+    if stress is not None:
+        out["stress"] = stress
     if integ is not None:
         integ.close()
     print(json.dumps(out), flush=True)

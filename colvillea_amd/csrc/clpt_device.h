@@ -556,11 +556,16 @@ CLPT_D float pdf2d(const SceneView& sc, float u, float v)
     return sc.pcond[(size_t)iv * (sc.dome_w + 1) + iu] / sc.pv[sc.dome_h];
 }
 
+/* cos(theta) of a direction for acos: clamped to [-1, 1], since a unit
+ * vector rotated through a float32 frame can come out with |y| just above 1
+ * (the reference passes dir.y unclamped and gets NaN there). */
+CLPT_D float dome_cos_theta(f3 dir) { return fminf(1.0f, fmaxf(-1.0f, dir.y)); }
+
 /* HDRIDome::evalEnvironment + directionToUVCoords: hdridome.h:291-297, 472-476 */
 CLPT_D f3 dome_le(const SceneView& sc, f3 dir)
 {
     const float  u = clpt_atan2f(dir.x, dir.z) * -CLPT_INV_PI_F * 0.5f + 0.5f;
-    const float  v = clpt_acosf(dir.y) * CLPT_INV_PI_F;
+    const float  v = clpt_acosf(dome_cos_theta(dir)) * CLPT_INV_PI_F;
     const float4 c = tex_eval(sc.dome_tex, sc.dome_w, sc.dome_h, u, v);
     return f3{c.x, c.y, c.z};
 }
@@ -569,7 +574,7 @@ CLPT_D f3 dome_le(const SceneView& sc, f3 dir)
 CLPT_D float dome_pdf(const SceneView& sc, f3 dir)
 {
     const float phi   = clpt_atan2f(dir.x, dir.z);
-    const float theta = clpt_acosf(dir.y);
+    const float theta = clpt_acosf(dome_cos_theta(dir));
     float       st, ct;
     clpt_sincosf(theta, &st, &ct);
     if (st == 0.0f)

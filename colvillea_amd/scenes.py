@@ -168,11 +168,13 @@ CONDUCTOR_TABLE = [
 
 # ------------------------------------------------------------------ HDRIs
 
-def hdri_directions(w: int, h: int) -> np.ndarray:
+def hdri_directions(w: int, h: int, rows=None) -> np.ndarray:
     """Direction of every texel centre under the dome mapping
-    (HDRIDome::sampleDirect, hdridome.h:207-222)."""
+    (HDRIDome::sampleDirect, hdridome.h:207-222); `rows` = (j0, j1)
+    restricts it to texel rows [j0, j1)."""
+    j0, j1 = rows if rows is not None else (0, h)
     u = (np.arange(w) + 0.5) / w
-    v = (np.arange(h) + 0.5) / h
+    v = (np.arange(j0, j1) + 0.5) / h
     U, V = np.meshgrid(u, v)
     theta, phi = np.pi * V, -2.0 * np.pi * (U - 0.5)
     return np.stack([np.sin(theta) * np.sin(phi), np.cos(theta), np.sin(theta) * np.cos(phi)], -1)
@@ -208,22 +210,35 @@ def lognormal_hdri(w: int, h: int, sigma: float, rng: np.random.Generator, hot_f
     y0 = np.minimum(np.floor(ys).astype(int), ch - 1)
     x0 = np.minimum(np.floor(xs).astype(int), cw - 1)
     fy, fx = (ys - y0)[:, None], (xs - x0)[None, :]
-    f = (g[y0][:, x0] * (1 - fx) * (1 - fy) + g[y0][:, x0 + 1] * fx * (1 - fy) + g[y0 + 1][:, x0] * (1 - fx) * fy
-         + g[y0 + 1][:, x0 + 1] * fx * fy)
+    # evaluated in row blocks (the same per-texel operations) so that the
+    # temporaries stay cache-sized at 8k x 4k
+    B = 128
+    f = np.empty((h, w))
+    for r in range(0, h, B):
+        y, ry = y0[r:r + B], fy[r:r + B]
+        f[r:r + B] = (g[y][:, x0] * (1 - fx) * (1 - ry) + g[y][:, x0 + 1] * fx * (1 - ry)
+                      + g[y + 1][:, x0] * (1 - fx) * ry + g[y + 1][:, x0 + 1] * fx * ry)
     f = (f - f.mean()) / f.std()
     lum = np.exp(sigma * f)
     tint = 0.75 + 0.25 * rng.uniform(size=(ch + 1, cw + 1, 3))
-    tint = tint[y0][:, x0]
-    col = lum[..., None] * tint
+    col = np.empty((h, w, 3))
+    for r in range(0, h, B):
+        col[r:r + B] = lum[r:r + B, :, None] * tint[y0[r:r + B]][:, x0]
     mean = col.mean()
-    d = hdri_directions(w, h)
     th, ph = rng.uniform(0.25 * np.pi, 0.45 * np.pi), rng.uniform(-np.pi, np.pi)
     hot = np.array([np.sin(th) * np.sin(ph), np.cos(th), np.sin(th) * np.cos(ph)])
-    ang = np.arccos(np.clip(d @ hot, -1, 1))
     sig = 2.0 * np.pi / w * 2.0
-    col = col + hot_factor * mean * np.exp(-0.5 * (ang / sig) ** 2)[..., None]
+    # the lobe only in the texel rows within 12 sigma of the hot spot's
+    # polar angle: beyond them it is below 1e4 * exp(-72) of the mean, which
+    # leaves every float64 sum unchanged
+    j0 = max(0, int(math.floor((th - 12 * sig) / np.pi * h)) - 1)
+    j1 = min(h, int(math.ceil((th + 12 * sig) / np.pi * h)) + 1)
+    d = hdri_directions(w, h, (j0, j1))
+    ang = np.arccos(np.clip(d @ hot, -1, 1))
+    col[j0:j1] = col[j0:j1] + hot_factor * mean * np.exp(-0.5 * (ang / sig) ** 2)[..., None]
     out = np.ones((h, w, 4), np.float32)
-    out[..., :3] = col
+    for r in range(0, h, B):
+        out[r:r + B, :, :3] = col[r:r + B]
     return out
 
 
@@ -284,6 +299,57 @@ def make_c1(seed: int = 2, width: int = 1920, height: int = 1080, spp: int = 64,
     return SceneDesc("C1", [grid_mesh, sphere], [0] + [1] * n_spheres, list(range(n_spheres + 1)),
                      xf.astype(np.float32), materials, [dome], cam, width, height, spp, 8, 0,
                      info={"seed": seed, "triangles": int(ntri)})
+
+
+def make_c4(seed: int = 5, width: int = 3840, height: int = 2160, spp: int = 128, grid=(4096, 2048),
+            n_spheres: int = 2000, sphere_subdiv: int = 4, hdri_w: int = 8192, hdri_h: int = 4096,
+            max_depth: int = 12) -> SceneDesc:
+    """The stress configuration: a 4096 x 2048 fBm grid (4 octaves) over
+    64 x 32 units and 2000 instanced icospheres (subdivision 4, radius
+    U[0.15, 0.5]), half of them GGX dielectric (IOR 1.5, alpha U[0.01, 0.3]), a quarter diffuse and a
+    quarter GGX conductor; an 8192 x 4096 log-normal HDRI (sigma 1.5, hot
+    spot at 1e4 x the mean).  3840x2160, 128 spp, depth 12."""
+    rng = np.random.default_rng(seed)
+    hx, hz = 32.0, 16.0
+    grid_mesh, H = fbm_grid(grid[0], grid[1], hx, hz, 2.0, 6.0, rng)
+    sphere = icosphere(sphere_subdiv)
+    cx = rng.uniform(-0.9 * hx, 0.9 * hx, n_spheres)
+    cz = rng.uniform(-0.9 * hz, 0.9 * hz, n_spheres)
+    r = rng.uniform(0.15, 0.5, n_spheres)
+    gi = np.clip(((cx + hx) / (2 * hx) * (grid[0] - 1)).round().astype(int), 0, grid[0] - 1)
+    gj = np.clip(((cz + hz) / (2 * hz) * (grid[1] - 1)).round().astype(int), 0, grid[1] - 1)
+    cy = H[gj, gi] + r * rng.uniform(0.3, 1.2, n_spheres)
+    materials = [DiffuseMtl((0.45, 0.42, 0.38))]
+    for i in range(n_spheres):
+        kind = i % 4
+        if kind == 1:
+            materials.append(DiffuseMtl(tuple(rng.uniform(0.2, 0.8, 3))))
+        elif kind == 3:
+            eta, k = CONDUCTOR_TABLE[int(rng.integers(len(CONDUCTOR_TABLE)))]
+            materials.append(MetalMtl((1.0, 1.0, 1.0), float(np.sqrt(rng.uniform(0.05, 0.5))), eta, k))
+        else:
+            materials.append(GlassMtl(float(np.sqrt(rng.uniform(0.01, 0.3))), 1.5))
+    xf = np.stack([IDENTITY] + [translate_scale((x, y, z), rr) for x, y, z, rr in zip(cx, cy, cz, r)])
+    dome = HDRIDome(ImageTexture2D(lognormal_hdri(hdri_w, hdri_h, 1.5, rng)))
+    cam = look_at((0.0, 18.0, 20.0), (0.0, 0.0, -2.0), (0, 1, 0), 55.0, width, height)
+    ntri = grid_mesh.triangles.shape[0] + n_spheres * sphere.triangles.shape[0]
+    return SceneDesc("C4", [grid_mesh, sphere], [0] + [1] * n_spheres, list(range(n_spheres + 1)),
+                     xf.astype(np.float32), materials, [dome], cam, width, height, spp, max_depth, 0,
+                     info={"seed": seed, "triangles": int(ntri)})
+
+
+def crop(desc: SceneDesc, x0: int, y0: int, w: int, h: int, spp: Optional[int] = None) -> SceneDesc:
+    """The same scene seen through the pixel rectangle [x0, x0 + w) x
+    [y0, y0 + h) of desc's film, as a w x h film of its own (the camera's
+    pixel plane restricted to the rectangle)."""
+    from dataclasses import replace
+    c = desc.camera
+    d00, ddu, ddv = (np.asarray(v, np.float64) for v in (c.m_camera_d00, c.m_camera_ddu, c.m_camera_ddv))
+    f = lambda v: np.asarray(v, np.float32)
+    cam = Camera(f(c.m_camera_pos), f(d00 + ddu * (x0 / desc.width) + ddv * (y0 / desc.height)),
+                 f(ddu * (w / desc.width)), f(ddv * (h / desc.height)))
+    return replace(desc, camera=cam, width=w, height=h, spp=desc.spp if spp is None else spp,
+                   info=dict(desc.info, crop=(x0, y0, w, h)))
 
 
 def make_c2_views(seed: int = 3, n_views: int = 256, side=(64, 512), spp=(4, 32), depth=(3, 10),

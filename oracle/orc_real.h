@@ -565,11 +565,17 @@ static R SFX(pdf2d)(const orc_dome* d, R u, R v)
     return (R)d->pcond[(size_t)iv * (d->w + 1) + iu] / (R)d->pv[d->h];
 }
 
+/* cos(theta) for acos, clamped to [-1, 1]: a unit vector rotated through a
+ * float32 frame can have |y| just above 1, where the reference's unclamped
+ * acos(dir.y) (hdridome.h:452, 475) returns NaN.  Pinned by the
+ * pole-direction test of the dome lookups. */
+static inline R SFX(cos_theta_clamped)(V3 dir) { return RFMIN(RC(1.0), RFMAX(RC(-1.0), dir.y)); }
+
 /* HDRIDome::directionToUVCoords: hdridome.h:472-476 */
 static void SFX(dir_to_uv)(V3 dir, R* u, R* v)
 {
     *u = RATAN2(dir.x, dir.z) * -ORC_INV_PI * RC(0.5) + RC(0.5);
-    *v = RACOS(dir.y) * ORC_INV_PI;
+    *v = RACOS(SFX(cos_theta_clamped)(dir)) * ORC_INV_PI;
 }
 
 /* HDRIDome::evalEnvironment: hdridome.h:291-297 */
@@ -585,7 +591,7 @@ static V3 SFX(dome_le)(const orc_dome* d, V3 dir)
 static R SFX(dome_pdf)(const orc_dome* d, V3 dir)
 {
     const R phi   = RATAN2(dir.x, dir.z);
-    const R theta = RACOS(dir.y);
+    const R theta = RACOS(SFX(cos_theta_clamped)(dir));
     R       st, ct;
     RSINCOS(theta, &st, &ct);
     if (st == RC(0.0))

@@ -179,9 +179,12 @@ def test_dome_eval_bit_exact(c0_small):
     n = 50000
     d = random_dirs(rng, n)
     d[:4] = [[0, 1, 0], [0, -1, 0], [1, 0, 0], [0, 0, -1]]
+    # |y| just above 1, as a float32 frame rotation can produce: clamped
+    d[4:6] = [[4.27260995e-4, 1.00000012, 2.73570418e-4], [1e-4, -1.0000001, -3e-4]]
     le, p = np.zeros((n, 3), f32), np.zeros(n, f32)
     capi.check(capi.lib().clhip_dome_eval_batch(integ.ctx, capi.fptr(d), n, capi.fptr(le), capi.fptr(p)), integ.ctx)
     ole, op = ref.dome_eval(d)
+    assert np.isfinite(le).all() and np.isfinite(p).all()
     assert bits_equal(le, ole)
     assert bits_equal(p, op)
 

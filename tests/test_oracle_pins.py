@@ -392,6 +392,25 @@ def test_dome_pdf_brute_force_and_sampling_chi_square():
     assert stats.chisquare(obs, exp).pvalue > 1e-4
 
 
+def test_dome_lookup_at_pole_overshoot_is_clamped():
+    """A direction rotated through a float32 frame can have |y| just above 1
+    (seen at depth 11 of a 4K render: y = 1.00000012).  acos of that is NaN;
+    the lookups clamp cos(theta) to [-1, 1], so they equal the lookups of
+    the same direction with y = +-1 exactly, in both oracle variants."""
+    rng = np.random.default_rng(13)
+    rgba = rng.lognormal(0, 1.0, (8, 16, 4)).astype(f32)
+    sc = orc.Scene(_dome_scene(rgba))
+    over = np.array([[4.27260995e-4, 1.00000012, 2.73570418e-4], [1e-4, -1.0000001, -3e-4],
+                     [0.0, np.nextafter(f32(1), f32(2)), 0.0]], f32)
+    exact = over.copy()
+    exact[:, 1] = np.sign(over[:, 1])
+    for variant in ("f32", "f64"):
+        le, p = sc.dome_eval(over, variant)
+        le1, p1 = sc.dome_eval(exact, variant)
+        assert np.isfinite(le).all() and np.isfinite(p).all()
+        assert np.array_equal(le, le1) and np.array_equal(p, p1)
+
+
 def test_bilinear_texel_centres_and_ramp():
     w, h = 8, 4
     rng = np.random.default_rng(13)
