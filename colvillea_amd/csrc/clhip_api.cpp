@@ -167,7 +167,7 @@ struct clhip_ctx
     clhip_guide guide{};
     bool        guide_ready = false;
     DevBuf      g_params, g_m1, g_m2, g_W, g_WT, g_img, g_gc, g_img8, g_scl;
-    DevBuf      g_X, g_act[4], g_out, g_grad[5], g_loss, g_partial, g_samples;
+    DevBuf      g_X, g_act[4], g_out, g_grad[5], g_loss, g_wpart, g_amaxp, g_partial, g_samples;
     uint32_t    g_N = 0;
     DevBuf      g_V, g_E, g_tag;
     uint32_t    g_slots = 0;
@@ -456,6 +456,8 @@ int guide_train_buffers(clhip_ctx* c, uint32_t N, uint32_t nsteps = 1)
         CLHIP_CHECK(c, c->g_grad[l].alloc((size_t)N * CLPT_G_HID * sizeof(__bf16)));
     CLHIP_CHECK(c, c->g_out.alloc((size_t)N * CLPT_G_HID * sizeof(float)));
     CLHIP_CHECK(c, c->g_loss.alloc((size_t)(N / 256) * sizeof(float)));
+    CLHIP_CHECK(c, c->g_wpart.alloc((size_t)(N / 256) * sizeof(double)));
+    CLHIP_CHECK(c, c->g_amaxp.alloc((size_t)4 * (N / 64) * sizeof(float)));
     CLHIP_CHECK(c, c->g_partial.alloc((size_t)(N / 1024) * CLPT_G_PARAMS * sizeof(float)));
     CLHIP_CHECK(c, c->g_samples.alloc((size_t)N * std::max(1u, nsteps) * sizeof(GuideSample)));
     c->g_N = N;
@@ -523,6 +525,8 @@ clpt::GuideTrainBufs guide_bufs(clhip_ctx* c)
     for (int l = 0; l < 5; ++l)
         b.grad[l] = c->g_grad[l].as<__bf16>();
     b.loss_part = c->g_loss.as<float>();
+    b.wpart     = c->g_wpart.as<double>();
+    b.amax_part = c->g_amaxp.as<float>();
     b.partial   = c->g_partial.as<float>();
     b.params    = c->g_params.as<float>();
     b.m1        = c->g_m1.as<float>();

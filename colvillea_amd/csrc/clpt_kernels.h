@@ -50,6 +50,8 @@ struct GuideTrainBufs
     float*         out;      /* N x 64 outputs */
     __bf16*        grad[5];  /* N x 64 gradients w.r.t. each layer's output */
     float*         loss_part;
+    double*        wpart;    /* (N / 256) per-block sample-weight sums */
+    float*         amax_part; /* 4 x (N / 64) per-block hidden-activation maxima */
     float*         partial;  /* (N / 1024) x 32768 split weight gradients */
     float*         params;
     float*         m1;

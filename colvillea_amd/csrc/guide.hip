@@ -68,20 +68,38 @@ __global__ __launch_bounds__(256) void k_guide_infer(const void* img, const floa
 
 /* ------------------------------------------------------------ training */
 
-__global__ __launch_bounds__(256) void k_guide_features(const GuideSample* s, const GuideCounters* gc, uint32_t off,
-                                                        uint32_t N, __bf16* X)
+/* Also the step's bookkeeping: per-block sums of the sample weights (fixed
+ * tree order) into wpart, the sample count, and the cleared activation
+ * maxima.  N is a multiple of 256 (one thread per sample row). */
+__global__ __launch_bounds__(256) void k_guide_features(const GuideSample* s, GuideCounters* gc, uint32_t off,
+                                                        uint32_t N, __bf16* X, double* wpart)
 {
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N)
-        return;
+    __shared__ double wred[256];
+    const uint32_t n   = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t cnt = guide_count(gc, off, N);
     float          v[9];
+    double         wt = 0.0;
     if (n < cnt)
     {
         const GuideSample g = s[n];
         v[0] = g.a.x; v[1] = g.a.y; v[2] = g.a.z; v[3] = g.a.w;
         v[4] = g.b.x; v[5] = g.b.y; v[6] = g.b.z; v[7] = g.b.w;
         v[8] = g.c.x;
+        wt   = (double)g.d.x;
+    }
+    wred[threadIdx.x] = wt;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1)
+    {
+        if (threadIdx.x < st)
+            wred[threadIdx.x] += wred[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        wpart[blockIdx.x] = wred[0];
+        if (blockIdx.x == 0)
+            gc->trained = cnt;
     }
     bf16x8* row = reinterpret_cast<bf16x8*>(X + (size_t)n * CLPT_G_IN);
     float   o[8];
@@ -114,13 +132,13 @@ enum
 template <int EPI>
 __global__ __launch_bounds__(256) void k_guide_gemm_nt(const __bf16* __restrict__ A, const __bf16* __restrict__ Bt,
                                                        uint32_t N, uint32_t K, uint32_t J, void* C,
-                                                       const __bf16* __restrict__ mask, uint32_t* amax)
+                                                       const __bf16* __restrict__ mask, float* amax_part)
 {
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t jt   = J / 32u;
     const uint32_t tn = wave / jt, tj = wave % jt;
-    if (tn >= N / 32u)
-        return;
+    /* the launch covers exactly N / 32 x J / 32 tiles (N a multiple of 1024,
+     * J = 64), so every wave of a block reaches the block-wide max below */
     const uint32_t lane = __lane_id(), c = lane & 31u, h = lane >> 5;
     const uint32_t n0 = 32u * tn, j0 = 32u * tj;
     f32x16         acc = guide_zero16();
@@ -156,87 +174,100 @@ __global__ __launch_bounds__(256) void k_guide_gemm_nt(const __bf16* __restrict_
             static_cast<__bf16*>(C)[idx] = (float)mask[idx] > 0.0f ? (__bf16)y : (__bf16)0.0f;
     }
     if constexpr (EPI == GEMM_RELU_BF16)
-        if (amax)
-        {
-            for (int off = 32; off > 0; off >>= 1)
-                mx = fmaxf(mx, __shfl_xor(mx, off));
-            if (__lane_id() == 0)
-                atomicMax(amax, __float_as_uint(mx)); /* non-negative floats order as their bits */
-        }
+    {
+        /* the block's activation maximum (for the FP8 scales), stored per
+         * block and reduced once per step by k_guide_loss */
+        __shared__ float bmx[4];
+        for (int off = 32; off > 0; off >>= 1)
+            mx = fmaxf(mx, __shfl_xor(mx, off));
+        if (lane == 0)
+            bmx[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            amax_part[blockIdx.x] = fmaxf(fmaxf(bmx[0], bmx[1]), fmaxf(bmx[2], bmx[3]));
+    }
 }
 
-/* Partial weight gradient over one chunk of samples:
+/* Partial weight gradient over one chunk of 1024 samples:
  * P[chunk][k][m] = sum_{n in chunk} A[n][k] G[n][m]; A is N x K, G is N x 64.
- * Written in the parameter layout (row k, column m) at param_off. */
+ * One block per (32-row k tile, 32-column m tile, chunk); each of its four
+ * waves takes 256 of the chunk's samples, 64 per iteration, with the next
+ * iteration's loads in flight during the current products.  The 64 x 32
+ * tiles of A and G go through LDS (16-byte row writes, column reads) to
+ * reach the MFMA operand layout, where a lane holds 8 consecutive samples.
+ * The four waves' sums are added in a fixed order, so the result does not
+ * depend on timing.  Written in the parameter layout (row k, column m) at
+ * param_off. */
 __global__ __launch_bounds__(256) void k_guide_gemm_tn(const __bf16* __restrict__ A, const __bf16* __restrict__ G,
-                                                       uint32_t K, uint32_t chunk, uint32_t nchunks, float* partial,
-                                                       uint32_t param_off)
+                                                       uint32_t K, float* partial, uint32_t param_off)
 {
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t kt   = K / 32u;
-    const uint32_t tk = wave % kt, rem = wave / kt;
+    const uint32_t kt = K / 32u;
+    const uint32_t tk = blockIdx.x % kt, rem = blockIdx.x / kt;
     const uint32_t tm = rem % 2u, ch = rem / 2u;
-    if (ch >= nchunks)
-        return;
-    const uint32_t lane = __lane_id(), c = lane & 31u, h = lane >> 5;
+    const uint32_t w = threadIdx.x >> 6, lane = __lane_id(), c = lane & 31u, h = lane >> 5;
     const uint32_t k0 = 32u * tk, m0 = 32u * tm;
-    /* per wave, a 16 (n) x 32 tile of A and of G goes through LDS: each lane
-     * loads 8 consecutive elements of one row (16 bytes), then reads its
-     * fragment column-wise */
-    __shared__ __bf16 ta[4][16][32 + 8], tg[4][16][32 + 8];
-    const uint32_t    w  = (threadIdx.x >> 6) & 3u;
-    const uint32_t    lr = lane >> 2, lc = 8u * (lane & 3u);
-    f32x16            acc = guide_zero16();
-    for (uint32_t n0 = ch * chunk; n0 < (ch + 1u) * chunk; n0 += 16u)
+    const uint32_t lr = lane >> 2, lc = 8u * (lane & 3u);
+    __shared__ __align__(16) __bf16 ta[4][64][40], tg[4][64][40];
+    const __bf16* pa = A + (size_t)(1024u * ch + 256u * w + lr) * K + k0 + lc;
+    const __bf16* pg = G + (size_t)(1024u * ch + 256u * w + lr) * CLPT_G_HID + m0 + lc;
+    bf16x8        va[4], vg[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
     {
-        const bf16x8 va = *reinterpret_cast<const bf16x8*>(A + (size_t)(n0 + lr) * K + k0 + lc);
-        const bf16x8 vg = *reinterpret_cast<const bf16x8*>(G + (size_t)(n0 + lr) * CLPT_G_HID + m0 + lc);
-#pragma unroll
-        for (int s = 0; s < 8; ++s)
-        {
-            ta[w][lr][lc + s] = va[s];
-            tg[w][lr][lc + s] = vg[s];
-        }
-        __builtin_amdgcn_s_waitcnt(0);
-        __builtin_amdgcn_wave_barrier();
-        bf16x8 a, b;
-#pragma unroll
-        for (int s = 0; s < 8; ++s)
-        {
-            a[s] = ta[w][8u * h + s][c];
-            b[s] = tg[w][8u * h + s][c];
-        }
-        __builtin_amdgcn_wave_barrier();
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        va[t] = *reinterpret_cast<const bf16x8*>(pa + (size_t)(16 * t) * K);
+        vg[t] = *reinterpret_cast<const bf16x8*>(pg + (size_t)(16 * t) * CLPT_G_HID);
     }
-    float* P = partial + (size_t)ch * CLPT_G_PARAMS + param_off;
+    f32x16 acc = guide_zero16();
+    for (uint32_t it = 0; it < 4u; ++it)
+    {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+        {
+            *reinterpret_cast<bf16x8*>(&ta[w][16 * t + lr][lc]) = va[t];
+            *reinterpret_cast<bf16x8*>(&tg[w][16 * t + lr][lc]) = vg[t];
+        }
+        if (it + 1u < 4u)
+        {
+            const size_t n1 = (size_t)64u * (it + 1u);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+            {
+                va[t] = *reinterpret_cast<const bf16x8*>(pa + (n1 + 16u * t) * K);
+                vg[t] = *reinterpret_cast<const bf16x8*>(pg + (n1 + 16u * t) * CLPT_G_HID);
+            }
+        }
+        /* the LDS writes have landed (lgkmcnt 0; vector loads stay in flight) */
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+        {
+            bf16x8 a, b;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+            {
+                a[q] = ta[w][16 * t + 8 * h + q][c];
+                b[q] = tg[w][16 * t + 8 * h + q][c];
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    /* the four waves' tiles, added in wave order */
+    __syncthreads();
+    float* red = reinterpret_cast<float*>(&ta[0][0][0]); /* 4 x 16 x 64 floats = 16 KB of the 20 KB */
 #pragma unroll
     for (int r = 0; r < 16; ++r)
-        P[(size_t)(k0 + 8u * (r / 4) + 4u * h + (r % 4)) * CLPT_G_HID + m0 + c] = acc[r];
-}
-
-/* Sum of the sample weights (one block, fixed order). */
-__global__ __launch_bounds__(1024) void k_guide_wsum(const GuideSample* s, GuideCounters* gc, uint32_t off, uint32_t N)
-{
-    __shared__ double part[1024];
-    const uint32_t    cnt = guide_count(gc, off, N);
-    double            acc = 0.0;
-    for (uint32_t n = threadIdx.x; n < cnt; n += blockDim.x)
-        acc += (double)s[n].d.x;
-    part[threadIdx.x] = acc;
+        red[(w * 16u + r) * 64u + lane] = acc[r];
     __syncthreads();
-    for (uint32_t st = 512; st > 0; st >>= 1)
+    float* P = partial + (size_t)ch * CLPT_G_PARAMS + param_off;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
     {
-        if (threadIdx.x < st)
-            part[threadIdx.x] += part[threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-    {
-        gc->wsum    = (float)part[0];
-        gc->trained = cnt;
-        for (int l = 0; l < 4; ++l)
-            gc->amax[l] = 0u;
+        const uint32_t e = threadIdx.x + 256u * q, r = e / 64u, l = e % 64u;
+        const float    v = ((red[r * 64u + l] + red[(16u + r) * 64u + l]) + red[(32u + r) * 64u + l]) +
+                        red[(48u + r) * 64u + l];
+        P[(size_t)(k0 + 8u * (r / 4u) + 4u * (l / 32u) + (r % 4u)) * CLPT_G_HID + m0 + (l % 32u)] = v;
     }
 }
 
@@ -245,21 +276,57 @@ __global__ __launch_bounds__(1024) void k_guide_wsum(const GuideSample* s, Guide
  * and normalised by the batch weight sum W); rows beyond the sample count
  * and output columns 40..63 are zero.  Per-block loss sums to loss_part. */
 __global__ __launch_bounds__(256) void k_guide_loss(const float* __restrict__ out, const GuideSample* s,
-                                                    const GuideCounters* gc, uint32_t off, uint32_t N, __bf16* G5,
-                                                    float* loss_part)
+                                                    GuideCounters* gc, uint32_t off, uint32_t N, __bf16* G5,
+                                                    float* loss_part, const double* wpart, const float* amax_part)
 {
-    __shared__ float red[256];
-    const uint32_t   n   = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t   cnt = guide_count(gc, off, N);
+    __shared__ float  red[256];
+    __shared__ double wred[256];
+    const uint32_t    n   = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t    cnt = guide_count(gc, off, N);
+    /* the hidden activations' maxima of this step; a step without samples
+     * keeps the previous ones (its activations are those of zero inputs) */
+    if (blockIdx.x == 0 && cnt > 0u)
+        for (int l = 0; l < 4; ++l)
+        {
+            float m = 0.0f;
+            for (uint32_t k = threadIdx.x; k < N / 64u; k += blockDim.x)
+                m = fmaxf(m, amax_part[(size_t)l * (N / 64u) + k]);
+            red[threadIdx.x] = m;
+            __syncthreads();
+            for (uint32_t st = 128; st > 0; st >>= 1)
+            {
+                if (threadIdx.x < st)
+                    red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
+                __syncthreads();
+            }
+            if (threadIdx.x == 0)
+                gc->amax[l] = __float_as_uint(red[0]);
+            __syncthreads();
+        }
+    /* W: the per-block weight sums in a fixed order (the same in every block) */
+    double wacc = 0.0;
+    for (uint32_t k = threadIdx.x; k < N / 256u; k += blockDim.x)
+        wacc += wpart[k];
+    wred[threadIdx.x] = wacc;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1)
+    {
+        if (threadIdx.x < st)
+            wred[threadIdx.x] += wred[threadIdx.x + st];
+        __syncthreads();
+    }
+    const float wsum = (float)wred[0];
+    if (n == 0)
+        gc->wsum = wsum;
     float            g[CLPT_G_HID];
 #pragma unroll
     for (int k = 0; k < CLPT_G_HID; ++k)
         g[k] = 0.0f;
     float loss = 0.0f;
-    if (n < cnt && gc->wsum > 0.0f)
+    if (n < cnt && wsum > 0.0f)
     {
         const GuideSample smp = s[n];
-        const float       wn  = smp.d.x / gc->wsum;
+        const float       wn  = smp.d.x / wsum;
         const f3          wi{smp.c.y, smp.c.z, smp.c.w};
         float             o[CLPT_G_NOUT];
 #pragma unroll
@@ -540,35 +607,33 @@ hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N
     const uint32_t     nt   = N / 32u;
     const uint32_t     off  = step * N;
     const GuideSample* smp  = b.samples + off;
-    hipLaunchKernelGGL(k_guide_wsum, dim3(1), dim3(1024), 0, s, smp, b.gc, off, N);
-    hipLaunchKernelGGL(k_guide_features, dim3(N / 256), dim3(256), 0, s, smp, b.gc, off, N, b.X);
+    hipLaunchKernelGGL(k_guide_features, dim3(N / 256), dim3(256), 0, s, smp, b.gc, off, N, b.X, b.wpart);
     /* forward: waves = tiles (N/32) x (64/32) */
     const __bf16* in = b.X;
     for (int l = 0; l < 4; ++l)
     {
         hipLaunchKernelGGL(k_guide_gemm_nt<GEMM_RELU_BF16>, dim3(nt * 2 / 4), dim3(256), 0, s, in,
                            b.WT + guide_layer_offset(l), N, guide_layer_in(l), (uint32_t)CLPT_G_HID,
-                           (void*)b.act[l], (const __bf16*)nullptr, &b.gc->amax[l]);
+                           (void*)b.act[l], (const __bf16*)nullptr, b.amax_part + (size_t)l * (N / 64u));
         in = b.act[l];
     }
     hipLaunchKernelGGL(k_guide_gemm_nt<GEMM_F32>, dim3(nt * 2 / 4), dim3(256), 0, s, in, b.WT + guide_layer_offset(4),
                        N, (uint32_t)CLPT_G_HID, (uint32_t)CLPT_G_HID, (void*)b.out, (const __bf16*)nullptr,
-                       (uint32_t*)nullptr);
+                       (float*)nullptr);
     hipLaunchKernelGGL(k_guide_loss, dim3(N / 256), dim3(256), 0, s, b.out, smp, b.gc, off, N, b.grad[4],
-                       b.loss_part);
+                       b.loss_part, b.wpart, b.amax_part);
     /* backward: G[l] is the gradient w.r.t. the input of layer l+1 (N x 64) */
     const uint32_t chunk = 1024u, nch = N / chunk;
     for (int l = 4; l >= 0; --l)
     {
         const __bf16*  A  = l == 0 ? b.X : b.act[l - 1];
         const uint32_t K  = guide_layer_in(l);
-        const uint32_t wv = (K / 32u) * 2u * nch;
-        hipLaunchKernelGGL(k_guide_gemm_tn, dim3((wv + 3) / 4), dim3(256), 0, s, A, b.grad[l], K, chunk, nch,
+        hipLaunchKernelGGL(k_guide_gemm_tn, dim3((K / 32u) * 2u * nch), dim3(256), 0, s, A, b.grad[l], K,
                            b.partial, guide_layer_offset(l));
         if (l > 0)
             hipLaunchKernelGGL(k_guide_gemm_nt<GEMM_MASK_BF16>, dim3(nt * 2 / 4), dim3(256), 0, s, b.grad[l],
                                b.W + guide_layer_offset(l), N, (uint32_t)CLPT_G_HID, (uint32_t)CLPT_G_HID,
-                               (void*)b.grad[l - 1], b.act[l - 1], (uint32_t*)nullptr);
+                               (void*)b.grad[l - 1], b.act[l - 1], (float*)nullptr);
     }
     hipLaunchKernelGGL(k_guide_adam, dim3(CLPT_G_PARAMS / 256), dim3(256), 0, s, b.params, b.m1, b.m2, b.partial,
                        nch, b.gc, lr, 0.9f, 0.999f, 1e-8f, b.W, b.WT, b.img, grad_out);

@@ -13,7 +13,8 @@ float64 oracle is tested separately at small sizes).
 Time is wall-clock around render + synchronise of each chunk (readbacks
 excluded), so the guided tracer's time includes its network inference and
 every training step.  With `warm_spp`, the guided tracer first renders and
-trains on that many spp and discards their film (their time is counted).  T_plain(eps) is the plain tracer's time to its final
+trains on that many spp (with `warm_steps_per_batch` Adam steps per batch,
+if given) and discards their film (their time is counted).  T_plain(eps) is the plain tracer's time to its final
 error eps; T_guided(eps) is where the guided tracer's error curve crosses
 eps (log-log interpolation between chunks).
 """
@@ -32,16 +33,21 @@ from .integrator import upload_scene
 EPS_DEN = 1e-2
 
 
-def _render_curve(it: Integrator, spp_points: List[int], ref, ref_var, warm_spp: int = 0) -> list:
+def _render_curve(it: Integrator, spp_points: List[int], ref, ref_var, warm_spp: int = 0,
+                  warm_guide: Optional[dict] = None, guide: Optional[dict] = None) -> list:
     """Error after each chunk.  warm_spp > 0: first render that many spp
-    (the guided tracer trains on them), then discard that film; the time is
-    counted."""
+    (the guided tracer trains on them, with the settings `warm_guide` if
+    given, then `guide`), then discard that film; the time is counted."""
     it.resetIterationIndex()
     it.sync()
     done, t_acc, curve = 0, 0.0, []
     if warm_spp:
         t = time.perf_counter()
+        if warm_guide:
+            it.setGuide(True, **warm_guide)
         it.render(warm_spp)
+        if warm_guide:
+            it.setGuide(True, **guide)
         it.resetIterationIndex()
         it.sync()
         t_acc += time.perf_counter() - t
@@ -100,8 +106,10 @@ def time_to_equal_error(desc: SceneDesc, ref_spp: int, spp_points: List[int], gu
         plain = _render_curve(it, spp_points, ref, ref_var)
         log(f"plain: {plain[-1]}")
         it.guideReset(guide.get("seed", 1))
-        it.setGuide(True, **{k: v for k, v in guide.items() if k not in ("seed", "warm_spp")})
-        guided = _render_curve(it, spp_points, ref, ref_var, guide.get("warm_spp", 0))
+        g = {k: v for k, v in guide.items() if k not in ("seed", "warm_spp", "warm_steps_per_batch")}
+        wg = dict(g, steps_per_batch=guide["warm_steps_per_batch"]) if guide.get("warm_steps_per_batch") else None
+        it.setGuide(True, **g)
+        guided = _render_curve(it, spp_points, ref, ref_var, guide.get("warm_spp", 0), wg, g)
         gst = it.guideStats()
         log(f"guided: {guided[-1]}, {gst['steps']} training steps, {gst['train_ms']:.1f} ms training in the last call")
         it.setGuide(False)

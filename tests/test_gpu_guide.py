@@ -107,6 +107,39 @@ def test_fp8_inference_matches_oracle(integ):
     assert rms8 < 0.2
 
 
+def test_fp8_scales_follow_training_activations(integ):
+    """The FP8 image is repacked after every step: activation scales
+    as_l = 448 / (1.25 max relu(z_l)) from that step's forward pass (the
+    bfloat16 network at the pre-step weights), weight scales from the
+    post-step weights.  A step without samples keeps the previous scales
+    (an all-zero forward pass measures nothing)."""
+    rng = np.random.default_rng(21)
+    integ.guideReset(9)
+    integ.setGuide(False, precision=1, learning_rate=1e-3)
+    try:
+        p0 = integ.guideWeights().astype(np.float64)
+        n = 4096
+        q = rand_queries(rng, n)
+        wi = rand_dirs(rng, n)
+        w = rng.uniform(0.05, 3.0, n).astype(np.float32)
+        integ.guideTrain(np.concatenate([q, wi, w[:, None]], 1))
+        p1 = integ.guideWeights().astype(np.float64)
+        scl = integ.guideFp8Scales().astype(np.float64)
+        _, A = G.forward(G.split_params(p0), G.encode(q), bf=True)
+        W1 = G.split_params(p1)
+        for l in range(1, 5):
+            amax = A[l].max()
+            assert abs(scl[2 * l] - 448.0 / (1.25 * amax)) <= 1e-2 * scl[2 * l], (l, scl[2 * l], amax)
+        for l in range(5):
+            ws = 448.0 / np.abs(np.asarray(W1[l], np.float32)).max()
+            assert abs(scl[2 * l + 1] * scl[2 * l] * ws - 1.0) < 1e-5, l
+        integ.guideTrain(np.zeros((0, 13), np.float32))  # no samples: no update, same scales
+        assert np.array_equal(integ.guideWeights().astype(np.float64), p1)
+        assert np.array_equal(integ.guideFp8Scales().astype(np.float64), scl)
+    finally:
+        integ.setGuide(False, precision=0)
+
+
 def test_training_gradient_and_adam_match_oracle(integ):
     rng = np.random.default_rng(2)
     integ.guideReset(5)
