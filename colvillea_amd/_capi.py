@@ -208,6 +208,8 @@ SIGNATURES = {
     "clhip_guide_infer": (C.c_int, [P, f32p, u32, f32p]),
     "clhip_guide_fp8_scales": (C.c_int, [P, f32p]),
     "clhip_guide_train": (C.c_int, [P, f32p, u32, f32p, f32p]),
+    "clhip_guide_records": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), u32,
+                                       C.POINTER(C.c_uint32)]),
 }
 
 

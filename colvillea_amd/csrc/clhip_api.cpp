@@ -169,7 +169,7 @@ struct clhip_ctx
     DevBuf      g_params, g_m1, g_m2, g_W, g_WT, g_img, g_gc, g_img8, g_scl;
     DevBuf      g_X, g_act[4], g_out, g_grad[5], g_loss, g_wpart, g_amaxp, g_partial, g_samples;
     uint32_t    g_N = 0;
-    DevBuf      g_V, g_E, g_tag;
+    DevBuf      g_V, g_E, g_tag, g_gcount, g_gblock;
     uint32_t    g_slots = 0;
     size_t      g_tagcap = 0;
     uint32_t    blocks_guided = 0;
@@ -473,6 +473,8 @@ int guide_record_buffers(clhip_ctx* c, uint32_t slots, size_t qcap)
         CLHIP_CHECK(c, hipStreamSynchronize(c->stream));
         CLHIP_CHECK(c, c->g_V.alloc((size_t)slots * (CLPT_MAXD + 1) * 4 * sizeof(float4)));
         CLHIP_CHECK(c, c->g_E.alloc((size_t)slots * (CLPT_MAXD + 1) * sizeof(float4)));
+        CLHIP_CHECK(c, c->g_gcount.alloc((size_t)slots * sizeof(uint32_t)));
+        CLHIP_CHECK(c, c->g_gblock.alloc((size_t)((slots + 255) / 256) * sizeof(uint32_t)));
         c->g_slots = slots;
     }
     if (qcap > c->g_tagcap)
@@ -509,6 +511,8 @@ GuideView guide_view(const clhip_ctx* c)
         g.nslots = c->guide.record_paths;
         g.V      = c->g_V.as<float4>();
         g.E      = c->g_E.as<float4>();
+        g.gcount = c->g_gcount.as<uint32_t>();
+        g.gblock = c->g_gblock.as<uint32_t>();
     }
     return g;
 }
@@ -713,8 +717,7 @@ int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
         }
     }
     if (a.gv.nslots)
-        CLHIP_CHECK(c, clpt::launch_guide_gather(c->stream, a.gv.V, a.gv.E, a.gv.nslots, a.max_depth, a.gsamples,
-                                                 a.gcap, a.gc));
+        CLHIP_CHECK(c, clpt::launch_guide_gather(c->stream, a.gv, a.max_depth, a.gsamples, a.gcap, a.gc));
     CLHIP_CHECK(c, clpt::launch_batch_end(c->stream, a.bc, a.tot, a.bt.cursor));
     return CLHIP_OK;
 }
@@ -2125,6 +2128,40 @@ int clhip_guide_fp8_scales(clhip_ctx* ctx, float* scl)
         return fail(ctx, CLHIP_E_STATE, "no weights: call clhip_guide_reset or clhip_set_guide first");
     CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     CLHIP_CHECK(ctx, hipMemcpy(scl, ctx->g_scl.p, 10 * sizeof(float), hipMemcpyDeviceToHost));
+    return CLHIP_OK;
+}
+
+int clhip_guide_records(clhip_ctx* ctx, float* V, float* E, float* samples13, uint32_t cap, uint32_t info[3])
+{
+    if (!ctx || !info)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_guide_records: bad arguments");
+    /* the records are laid out for the current record_paths (<= capacity) */
+    const uint32_t slots = ctx->g_slots ? std::min<uint32_t>(ctx->guide.record_paths, (uint32_t)ctx->g_slots) : 0u;
+    info[0]              = slots;
+    info[1]              = CLPT_MAXD + 1u;
+    info[2]              = (uint32_t)(ctx->g_samples.bytes / sizeof(GuideSample));
+    if (!V && !E && !samples13)
+        return CLHIP_OK;
+    if (slots == 0)
+        return fail(ctx, CLHIP_E_STATE, "clhip_guide_records: nothing recorded yet");
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t recs = (size_t)(CLPT_MAXD + 1) * slots;
+    if (V)
+        CLHIP_CHECK(ctx, hipMemcpy(V, ctx->g_V.p, 4 * recs * sizeof(float4), hipMemcpyDeviceToHost));
+    if (E)
+        CLHIP_CHECK(ctx, hipMemcpy(E, ctx->g_E.p, recs * sizeof(float4), hipMemcpyDeviceToHost));
+    if (samples13 && cap)
+    {
+        const uint32_t           n = std::min(cap, info[2]);
+        std::vector<GuideSample> hs(n);
+        CLHIP_CHECK(ctx, hipMemcpy(hs.data(), ctx->g_samples.p, n * sizeof(GuideSample), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i)
+        {
+            const float src[13] = {hs[i].a.x, hs[i].a.y, hs[i].a.z, hs[i].a.w, hs[i].b.x, hs[i].b.y, hs[i].b.z,
+                                   hs[i].b.w, hs[i].c.x, hs[i].c.y, hs[i].c.z, hs[i].c.w, hs[i].d.x};
+            std::memcpy(samples13 + 13 * (size_t)i, src, sizeof(src));
+        }
+    }
     return CLHIP_OK;
 }
 

@@ -68,8 +68,9 @@ hipError_t launch_guide_infer(hipStream_t s, const __bf16* img, const uint8_t* i
 hipError_t launch_guide_pack_fp8(hipStream_t s, const float* params, const GuideCounters* gc, uint8_t* img8,
                                  float* scl);
 hipError_t launch_guide_pack(hipStream_t s, const float* params, __bf16* W, __bf16* WT, __bf16* img);
-hipError_t launch_guide_gather(hipStream_t s, const float4* V, const float4* E, uint32_t nslots, uint32_t maxd,
-                               GuideSample* out, uint32_t cap, GuideCounters* gc);
+/* training samples of the recorded paths, appended in path-slot order */
+hipError_t launch_guide_gather(hipStream_t s, const GuideView& gv, uint32_t maxd, GuideSample* out, uint32_t cap,
+                               GuideCounters* gc);
 /* Adam step `step` of `nsteps` on the gathered samples: slice
  * [step N, step N + N) of b.samples (count in b.gc; N a multiple of 1024).
  * The last step empties the buffer.  grad_out (optional) receives the

@@ -235,6 +235,8 @@ struct GuideView
     uint32_t      nslots; /* recorded paths per batch (0 = no recording) */
     float4*       V;
     float4*       E;
+    uint32_t*     gcount; /* gather scratch: samples per recorded path */
+    uint32_t*     gblock; /* gather scratch: per-256-path block counts, then offsets */
 };
 
 /* Per-batch exact counters (device): queue counts per depth and traversal

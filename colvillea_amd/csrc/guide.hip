@@ -518,49 +518,115 @@ __global__ __launch_bounds__(256) void k_guide_finish(GuideCounters* gc, const f
  * Vertex k yields the sample (query, sampled direction, weight) with weight
  * lum(S * R) / p, R = f cos / T_next per channel and p the density the
  * direction was drawn with: the single-sample estimate of
- * lum(L_i f cos) / p of the radiance that came back along it. */
-__global__ __launch_bounds__(256) void k_guide_gather(const float4* V, const float4* E, uint32_t nslots,
-                                                      uint32_t maxd, GuideSample* out, uint32_t cap, GuideCounters* gc)
+ * lum(L_i f cos) / p of the radiance that came back along it.
+ *
+ * The samples are appended in a fixed order -- by path slot, then from the
+ * deepest vertex up -- so that training sees the same data in the same
+ * order on every run: a counting pass, a scan of the per-block counts and a
+ * writing pass that walks the paths again. */
+template <class F>
+__device__ __forceinline__ void guide_walk(const float4* V, const float4* E, uint32_t nslots, uint32_t slot,
+                                           uint32_t maxd, F&& emit)
 {
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool     live = slot < nslots;
-    const size_t   pl   = (size_t)(CLPT_MAXD + 1) * nslots;
-    float          sr = 0.0f, sg = 0.0f, sb = 0.0f;
+    const size_t pl = (size_t)(CLPT_MAXD + 1) * nslots;
+    float        sr = 0.0f, sg = 0.0f, sb = 0.0f;
     for (uint32_t j = maxd; j >= 1u; --j)
     {
-        const size_t rec  = (size_t)j * nslots + (live ? slot : 0u);
-        float4       v3   = live ? V[3 * pl + rec] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float        w    = 0.0f;
+        const size_t rec = (size_t)j * nslots + slot;
+        const float4 v3  = V[3 * pl + rec];
+        float        w   = 0.0f;
         if (v3.x > 0.0f)
             w = lum3(sr * v3.y, sg * v3.z, sb * v3.w) / v3.x;
-        const bool   want = w > 0.0f && w < 3.0e38f;
-        const unsigned long long mask = __ballot(want);
-        if (mask)
-        {
-            const uint32_t leader = (uint32_t)(__ffsll((long long)mask) - 1);
-            uint32_t       base   = 0;
-            if (__lane_id() == leader)
-                base = atomicAdd(&gc->count, (uint32_t)__popcll(mask));
-            base = __shfl(base, (int)leader);
-            const uint32_t idx = base + (uint32_t)__popcll(mask & ((1ull << __lane_id()) - 1ull));
-            if (want && idx < cap)
-            {
-                GuideSample g;
-                g.a      = V[rec];
-                g.b      = V[pl + rec];
-                g.c      = V[2 * pl + rec];
-                g.d      = make_float4(w, 0.0f, 0.0f, 0.0f);
-                out[idx] = g;
-            }
-        }
-        if (live)
-        {
-            const float4 e = E[rec];
-            sr += e.x;
-            sg += e.y;
-            sb += e.z;
-        }
+        if (w > 0.0f && w < 3.0e38f)
+            emit(rec, pl, w);
+        const float4 e = E[rec];
+        sr += e.x;
+        sg += e.y;
+        sb += e.z;
     }
+}
+
+/* Exclusive prefix sum over the 256 threads of a block (fixed order). */
+__device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t* total)
+{
+    __shared__ uint32_t wsum[4];
+    const uint32_t      lane = __lane_id(), w = threadIdx.x >> 6;
+    uint32_t            x    = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        const uint32_t y = __shfl_up(x, (unsigned)d);
+        if (lane >= (uint32_t)d)
+            x += y;
+    }
+    if (lane == 63u)
+        wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t k = 0; k < w; ++k)
+        before += wsum[k];
+    *total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_guide_gather_count(const float4* V, const float4* E, uint32_t nslots,
+                                                            uint32_t maxd, uint32_t* gcount, uint32_t* gblock)
+{
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t       n    = 0;
+    if (slot < nslots)
+    {
+        guide_walk(V, E, nslots, slot, maxd, [&](size_t, size_t, float) { ++n; });
+        gcount[slot] = n;
+    }
+    uint32_t total;
+    (void)block_scan_256(n, &total);
+    if (threadIdx.x == 0)
+        gblock[blockIdx.x] = total;
+}
+
+/* gblock[b] <- count before block b (starting at the samples already held),
+ * in block order; the new total goes to gc->count.  One block of 256. */
+__global__ __launch_bounds__(256) void k_guide_gather_scan(uint32_t* gblock, uint32_t nblocks, GuideCounters* gc)
+{
+    uint32_t base = gc->count;
+    for (uint32_t b0 = 0; b0 < nblocks; b0 += 256u)
+    {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint32_t v = b < nblocks ? gblock[b] : 0u;
+        uint32_t       total;
+        const uint32_t ex = block_scan_256(v, &total);
+        if (b < nblocks)
+            gblock[b] = base + ex;
+        base += total;
+    }
+    if (threadIdx.x == 0)
+        gc->count = base;
+}
+
+__global__ __launch_bounds__(256) void k_guide_gather_write(const float4* V, const float4* E, uint32_t nslots,
+                                                            uint32_t maxd, const uint32_t* gcount,
+                                                            const uint32_t* gblock, GuideSample* out, uint32_t cap)
+{
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n    = slot < nslots ? gcount[slot] : 0u;
+    uint32_t       total;
+    uint32_t       idx = gblock[blockIdx.x] + block_scan_256(n, &total);
+    if (slot >= nslots || n == 0u)
+        return;
+    guide_walk(V, E, nslots, slot, maxd, [&](size_t rec, size_t pl, float w) {
+        if (idx < cap)
+        {
+            GuideSample g;
+            g.a      = V[rec];
+            g.b      = V[pl + rec];
+            g.c      = V[2 * pl + rec];
+            g.d      = make_float4(w, 0.0f, 0.0f, 0.0f);
+            out[idx] = g;
+        }
+        ++idx;
+    });
 }
 
 /* ------------------------------------------------------------ launchers */
@@ -590,12 +656,17 @@ hipError_t launch_guide_pack(hipStream_t s, const float* params, __bf16* W, __bf
     return hipGetLastError();
 }
 
-hipError_t launch_guide_gather(hipStream_t s, const float4* V, const float4* E, uint32_t nslots, uint32_t maxd,
-                               GuideSample* out, uint32_t cap, GuideCounters* gc)
+hipError_t launch_guide_gather(hipStream_t s, const GuideView& gv, uint32_t maxd, GuideSample* out, uint32_t cap,
+                               GuideCounters* gc)
 {
-    if (nslots == 0 || maxd == 0)
+    if (gv.nslots == 0 || maxd == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_guide_gather, dim3((nslots + 255) / 256), dim3(256), 0, s, V, E, nslots, maxd, out, cap, gc);
+    const uint32_t nb = (gv.nslots + 255) / 256;
+    hipLaunchKernelGGL(k_guide_gather_count, dim3(nb), dim3(256), 0, s, gv.V, gv.E, gv.nslots, maxd, gv.gcount,
+                       gv.gblock);
+    hipLaunchKernelGGL(k_guide_gather_scan, dim3(1), dim3(256), 0, s, gv.gblock, nb, gc);
+    hipLaunchKernelGGL(k_guide_gather_write, dim3(nb), dim3(256), 0, s, gv.V, gv.E, gv.nslots, maxd, gv.gcount,
+                       gv.gblock, out, cap);
     return hipGetLastError();
 }
 

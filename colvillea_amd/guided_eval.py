@@ -93,8 +93,12 @@ def reference(it: Integrator, desc: SceneDesc, ref_spp: int, log=print):
 
 
 def time_to_equal_error(desc: SceneDesc, ref_spp: int, spp_points: List[int], guide: dict,
-                        it: Optional[Integrator] = None, log=print, ref=None) -> dict:
-    """ref: a (reference, variance) pair from reference() to reuse."""
+                        it: Optional[Integrator] = None, log=print, ref=None, seeds=None, plain=None) -> dict:
+    """ref: a (reference, variance) pair from reference() to reuse.
+    seeds: network initialisations to run the guided tracer from (default
+    guide["seed"] or 1); the ratio is reported per seed and as their mean.
+    plain: a plain curve from an earlier call on the same scene and points
+    (the plain tracer is deterministic), else it is rendered here."""
     own = it is None
     if own:
         it = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
@@ -102,23 +106,32 @@ def time_to_equal_error(desc: SceneDesc, ref_spp: int, spp_points: List[int], gu
     try:
         ref, ref_var = ref if ref is not None else reference(it, desc, ref_spp, log)
         it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0, samples_per_batch=1)
-        it.setGuide(False)
-        plain = _render_curve(it, spp_points, ref, ref_var)
-        log(f"plain: {plain[-1]}")
-        it.guideReset(guide.get("seed", 1))
+        if plain is None:
+            it.setGuide(False)
+            plain = _render_curve(it, spp_points, ref, ref_var)
+            log(f"plain: {plain[-1]}")
+        eps = plain[-1]["relmse"]
+        tp = plain[-1]["seconds"]
         g = {k: v for k, v in guide.items() if k not in ("seed", "warm_spp", "warm_steps_per_batch")}
         wg = dict(g, steps_per_batch=guide["warm_steps_per_batch"]) if guide.get("warm_steps_per_batch") else None
-        it.setGuide(True, **g)
-        guided = _render_curve(it, spp_points, ref, ref_var, guide.get("warm_spp", 0), wg, g)
-        gst = it.guideStats()
-        log(f"guided: {guided[-1]}, {gst['steps']} training steps, {gst['train_ms']:.1f} ms training in the last call")
-        it.setGuide(False)
-        eps = plain[-1]["relmse"]
-        tg = _crossing(guided, eps)
-        tp = plain[-1]["seconds"]
-        return {"plain": plain, "guided": guided, "eps": eps, "t_plain": tp, "t_guided": tg,
-                "ratio": (tg / tp) if tg is not None else None, "ref_spp": ref_spp,
-                "guide": dict(guide), "train_steps": int(gst["steps"]), "samples_trained": int(gst["samples"])}
+        runs = []
+        for seed in (seeds or [guide.get("seed", 1)]):
+            it.guideReset(seed)
+            it.setGuide(True, **g)
+            guided = _render_curve(it, spp_points, ref, ref_var, guide.get("warm_spp", 0), wg, g)
+            gst = it.guideStats()
+            log(f"guided (seed {seed}): {guided[-1]}, {gst['steps']} training steps")
+            it.setGuide(False)
+            tg = _crossing(guided, eps)
+            runs.append({"seed": seed, "guided": guided, "t_guided": tg, "ratio": (tg / tp) if tg is not None else None,
+                         "train_steps": int(gst["steps"]), "samples_trained": int(gst["samples"])})
+        ratios = [r["ratio"] for r in runs]
+        ok = all(r is not None for r in ratios)
+        first = runs[0]
+        return {"plain": plain, "guided": first["guided"], "eps": eps, "t_plain": tp, "t_guided": first["t_guided"],
+                "ratio": sum(ratios) / len(ratios) if ok else None, "ratios": ratios, "runs": runs,
+                "ref_spp": ref_spp, "guide": dict(guide), "train_steps": first["train_steps"],
+                "samples_trained": first["samples_trained"]}
     finally:
         it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=desc.view)
         if own:

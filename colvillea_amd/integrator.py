@@ -409,6 +409,20 @@ class Integrator:
         self._check(self._lib.clhip_guide_train(self._ctx, capi.fptr(s), s.shape[0], capi.fptr(g), C.byref(loss)))
         return loss.value, g
 
+    def guideRecords(self, max_samples: int = 1 << 20):
+        """Training records of the last recorded batch and the samples
+        gathered from them: (V [4, D, S, 4], E [D, S, 4], samples [n, 13])
+        with n = min(max_samples, capacity)."""
+        info = (C.c_uint32 * 3)()
+        self._check(self._lib.clhip_guide_records(self._ctx, None, None, None, 0, info))
+        S, D, capn = info[0], info[1], info[2]
+        V = np.empty((4, D, S, 4), np.float32)
+        E = np.empty((D, S, 4), np.float32)
+        n = min(int(max_samples), int(capn))
+        smp = np.empty((n, 13), np.float32)
+        self._check(self._lib.clhip_guide_records(self._ctx, capi.fptr(V), capi.fptr(E), capi.fptr(smp), n, info))
+        return V, E, smp
+
     # -- GL interop (integrator.h:66-71) has no counterpart without a display
     def registerFramebuffer(self, glTexture):
         raise NotImplementedError("GL interop is not available on MI355X; use readbackFramebuffer()")

@@ -337,6 +337,13 @@ int clhip_guide_infer(clhip_ctx* ctx, const float* q9, uint32_t n, float* out40)
 /* One Adam step on n samples of 13 floats (query 9, direction 3, weight);
  * grad (32768 floats, optional) receives the summed gradient. */
 int clhip_guide_train(clhip_ctx* ctx, const float* samples13, uint32_t n, float* grad, float* loss);
+/* Test surface (no reference counterpart): the training records of the
+ * last recorded batch and the samples gathered from them, in gather order.
+ * info = {recorded paths S, depth planes D, sample capacity}.  V holds
+ * 4 x D x S float4 (plane p, depth j, path s at ((p D + j) S + s)), E holds
+ * D x S float4 (path contributions per depth), samples13 the first `cap`
+ * gathered samples (query 9, direction 3, weight).  NULL buffers: info only. */
+int clhip_guide_records(clhip_ctx* ctx, float* V, float* E, float* samples13, uint32_t cap, uint32_t info[3]);
 
 /* ---- image files (ImageUtils, src/delegate/imageutil.cpp:13-139, without
  *      stb or FreeImage).  Images are RGBA float32, row 0 at the top.

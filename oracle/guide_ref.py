@@ -244,3 +244,42 @@ def init_params(seed=1):
         off += fan * HID
     del st
     return out.astype(np.float32).astype(np.float64)
+
+
+def gather(V, E, maxd):
+    """Training samples of recorded paths (the renderer's k_guide_gather_*,
+    restated in float32 with its operation order): for each path slot in
+    order, vertices from depth maxd up to 1, with S the float32 running sum
+    of the contributions E at the deeper vertices, weight
+    w = (0.2126 S.r R.r + 0.7152 S.g R.g) + 0.0722 S.b R.b, divided by p,
+    where plane 3 of V holds (p, R.r, R.g, R.b); a sample is kept when
+    0 < w < 3e38.  V: [4, D, S, 4], E: [D, S, 4].  Returns [n, 13]:
+    planes 0-2 of the vertex (query 9, direction 3) and w."""
+    f = np.float32
+    V = np.asarray(V, f)
+    E = np.asarray(E, f)
+    S = V.shape[2]
+    sr = np.zeros(S, f)
+    sg = np.zeros(S, f)
+    sb = np.zeros(S, f)
+    keys, rows = [], []
+    with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+        for j in range(maxd, 0, -1):
+            v3 = V[3, j]
+            lum = (f(0.2126) * (sr * v3[:, 1]) + f(0.7152) * (sg * v3[:, 2])) + f(0.0722) * (sb * v3[:, 3])
+            w = np.where(v3[:, 0] > 0, lum / np.where(v3[:, 0] > 0, v3[:, 0], f(1)), f(0)).astype(f)
+            keep = np.nonzero((w > 0) & (w < f(3.0e38)))[0]
+            if len(keep):
+                q = np.concatenate([V[0, j, keep], V[1, j, keep], V[2, j, keep], w[keep, None]], 1)
+                rows.append(q)
+                keys.append(np.stack([keep, np.full(len(keep), maxd - j)], 1))
+            sr = (sr + E[j, :, 0]).astype(f)
+            sg = (sg + E[j, :, 1]).astype(f)
+            sb = (sb + E[j, :, 2]).astype(f)
+    if not rows:
+        return np.zeros((0, 13), f)
+    rows = np.concatenate(rows)
+    keys = np.concatenate(keys)
+    order = np.lexsort((keys[:, 1], keys[:, 0]))
+    return rows[order]
+

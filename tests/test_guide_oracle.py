@@ -169,3 +169,49 @@ def test_init_params_matches_he_uniform_bounds():
     assert np.all(Ws[4][:, 40:] == 0)
     # a uniform draw fills its range
     assert np.abs(Ws[0]).max() > 0.95 * np.sqrt(6 / 256)
+
+
+def test_gather_known_paths_and_loop_form():
+    """The gather restatement: hand-made paths with known weights, and
+    random records against a plain per-path loop in float64."""
+    D, S = 5, 3
+    V = np.zeros((4, D, S, 4), np.float32)
+    E = np.zeros((D, S, 4), np.float32)
+    # path 0: light arrives at depth 3; vertex 2 sampled with p = 2, R = 1
+    E[3, 0, :3] = 1.0
+    V[3, 2, 0] = [2.0, 1.0, 1.0, 1.0]
+    V[0, 2, 0] = [0.1, 0.2, 0.3, 0.4]
+    # vertex 1 of path 0 sees the same light through R = (2, 0, 0), p = 4
+    V[3, 1, 0] = [4.0, 2.0, 0.0, 0.0]
+    # path 1: a vertex below its only contribution (no light after it)
+    E[1, 1, :3] = 5.0
+    V[3, 2, 1] = [1.0, 1.0, 1.0, 1.0]
+    # path 2: p = 0 (never sampled) is skipped
+    E[4, 2, :3] = 1.0
+    V[3, 3, 2] = [0.0, 1.0, 1.0, 1.0]
+    out = G.gather(V, E, D - 1)
+    assert out.shape == (2, 13)
+    assert np.isclose(out[0, 12], (0.2126 + 0.7152 + 0.0722) / 2.0, rtol=1e-6)  # deepest first
+    assert np.allclose(out[0, :4], [0.1, 0.2, 0.3, 0.4])
+    assert np.isclose(out[1, 12], 0.2126 * 2.0 / 4.0, rtol=1e-6)
+
+    rng = np.random.default_rng(4)
+    D, S = 7, 200
+    V = rng.uniform(0, 1, (4, D, S, 4)).astype(np.float32)
+    V[3, :, :, 0] *= rng.uniform(size=(D, S)) > 0.3  # some vertices unsampled
+    E = (rng.uniform(0, 1, (D, S, 4)) * (rng.uniform(size=(D, S, 1)) > 0.6)).astype(np.float32)
+    out = G.gather(V, E, D - 1)
+    ref = []
+    for s in range(S):
+        acc = np.zeros(3)
+        for j in range(D - 1, 0, -1):
+            p, r = float(V[3, j, s, 0]), V[3, j, s, 1:].astype(np.float64)
+            if p > 0:
+                w = float(np.dot([0.2126, 0.7152, 0.0722], acc * r)) / p
+                if w > 0:
+                    ref.append(np.concatenate([V[0, j, s], V[1, j, s], V[2, j, s], [w]]))
+            acc += E[j, s, :3]
+    ref = np.array(ref)
+    assert out.shape == ref.shape
+    assert np.allclose(out[:, :12], ref[:, :12])
+    assert np.allclose(out[:, 12], ref[:, 12], rtol=1e-5)
