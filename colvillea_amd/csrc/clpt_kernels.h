@@ -71,8 +71,9 @@ hipError_t launch_guide_pack(hipStream_t s, const float* params, __bf16* W, __bf
 /* training samples of the recorded paths, appended in path-slot order */
 hipError_t launch_guide_gather(hipStream_t s, const GuideView& gv, uint32_t maxd, GuideSample* out, uint32_t cap,
                                GuideCounters* gc);
-/* Adam step `step` of `nsteps` on the gathered samples: slice
- * [step N, step N + N) of b.samples (count in b.gc; N a multiple of 1024).
+/* Adam step `step` of `nsteps` on the gathered samples: rows step,
+ * step + nsteps, ... of b.samples, up to N of them (count in b.gc; N a
+ * multiple of 1024), so that each step draws from the whole batch.
  * The last step empties the buffer.  grad_out (optional) receives the
  * summed gradient. */
 hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N, float lr, float* grad_out,

@@ -23,10 +23,13 @@ namespace
 __device__ __forceinline__ float lum3(float r, float g, float b) { return 0.2126f * r + 0.7152f * g + 0.0722f * b; }
 
 /* Samples of training step slice [off, off + N) of the gathered buffer. */
-__device__ __forceinline__ uint32_t guide_count(const GuideCounters* gc, uint32_t off, uint32_t N)
+/* Samples of Adam step `step` of `nsteps` on the gathered samples: rows
+ * step, step + nsteps, step + 2 nsteps, ... of the first min(count,
+ * N nsteps), so every step draws from all of the batch's paths. */
+__device__ __forceinline__ uint32_t guide_count(const GuideCounters* gc, uint32_t step, uint32_t nsteps, uint32_t N)
 {
-    const uint32_t c = gc->count;
-    return c > off ? min(c - off, N) : 0u;
+    const uint32_t c = min(gc->count, N * nsteps);
+    return c > step ? (c - step + nsteps - 1u) / nsteps : 0u;
 }
 
 } // namespace
@@ -71,17 +74,17 @@ __global__ __launch_bounds__(256) void k_guide_infer(const void* img, const floa
 /* Also the step's bookkeeping: per-block sums of the sample weights (fixed
  * tree order) into wpart, the sample count, and the cleared activation
  * maxima.  N is a multiple of 256 (one thread per sample row). */
-__global__ __launch_bounds__(256) void k_guide_features(const GuideSample* s, GuideCounters* gc, uint32_t off,
-                                                        uint32_t N, __bf16* X, double* wpart)
+__global__ __launch_bounds__(256) void k_guide_features(const GuideSample* s, GuideCounters* gc, uint32_t step,
+                                                        uint32_t nsteps, uint32_t N, __bf16* X, double* wpart)
 {
     __shared__ double wred[256];
     const uint32_t n   = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t cnt = guide_count(gc, off, N);
+    const uint32_t cnt = guide_count(gc, step, nsteps, N);
     float          v[9];
     double         wt = 0.0;
     if (n < cnt)
     {
-        const GuideSample g = s[n];
+        const GuideSample g = s[(size_t)n * nsteps];
         v[0] = g.a.x; v[1] = g.a.y; v[2] = g.a.z; v[3] = g.a.w;
         v[4] = g.b.x; v[5] = g.b.y; v[6] = g.b.z; v[7] = g.b.w;
         v[8] = g.c.x;
@@ -276,13 +279,14 @@ __global__ __launch_bounds__(256) void k_guide_gemm_tn(const __bf16* __restrict_
  * and normalised by the batch weight sum W); rows beyond the sample count
  * and output columns 40..63 are zero.  Per-block loss sums to loss_part. */
 __global__ __launch_bounds__(256) void k_guide_loss(const float* __restrict__ out, const GuideSample* s,
-                                                    GuideCounters* gc, uint32_t off, uint32_t N, __bf16* G5,
-                                                    float* loss_part, const double* wpart, const float* amax_part)
+                                                    GuideCounters* gc, uint32_t step, uint32_t nsteps, uint32_t N,
+                                                    __bf16* G5, float* loss_part, const double* wpart,
+                                                    const float* amax_part)
 {
     __shared__ float  red[256];
     __shared__ double wred[256];
     const uint32_t    n   = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t    cnt = guide_count(gc, off, N);
+    const uint32_t    cnt = guide_count(gc, step, nsteps, N);
     /* the hidden activations' maxima of this step; a step without samples
      * keeps the previous ones (its activations are those of zero inputs) */
     if (blockIdx.x == 0 && cnt > 0u)
@@ -325,7 +329,7 @@ __global__ __launch_bounds__(256) void k_guide_loss(const float* __restrict__ ou
     float loss = 0.0f;
     if (n < cnt && wsum > 0.0f)
     {
-        const GuideSample smp = s[n];
+        const GuideSample smp = s[(size_t)n * nsteps];
         const float       wn  = smp.d.x / wsum;
         const f3          wi{smp.c.y, smp.c.z, smp.c.w};
         float             o[CLPT_G_NOUT];
@@ -676,9 +680,8 @@ hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N
     if (N % 1024u != 0u || N == 0u || step >= nsteps)
         return hipErrorInvalidValue;
     const uint32_t     nt   = N / 32u;
-    const uint32_t     off  = step * N;
-    const GuideSample* smp  = b.samples + off;
-    hipLaunchKernelGGL(k_guide_features, dim3(N / 256), dim3(256), 0, s, smp, b.gc, off, N, b.X, b.wpart);
+    const GuideSample* smp  = b.samples + step; /* rows step + k nsteps */
+    hipLaunchKernelGGL(k_guide_features, dim3(N / 256), dim3(256), 0, s, smp, b.gc, step, nsteps, N, b.X, b.wpart);
     /* forward: waves = tiles (N/32) x (64/32) */
     const __bf16* in = b.X;
     for (int l = 0; l < 4; ++l)
@@ -691,7 +694,7 @@ hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N
     hipLaunchKernelGGL(k_guide_gemm_nt<GEMM_F32>, dim3(nt * 2 / 4), dim3(256), 0, s, in, b.WT + guide_layer_offset(4),
                        N, (uint32_t)CLPT_G_HID, (uint32_t)CLPT_G_HID, (void*)b.out, (const __bf16*)nullptr,
                        (float*)nullptr);
-    hipLaunchKernelGGL(k_guide_loss, dim3(N / 256), dim3(256), 0, s, b.out, smp, b.gc, off, N, b.grad[4],
+    hipLaunchKernelGGL(k_guide_loss, dim3(N / 256), dim3(256), 0, s, b.out, smp, b.gc, step, nsteps, N, b.grad[4],
                        b.loss_part, b.wpart, b.amax_part);
     /* backward: G[l] is the gradient w.r.t. the input of layer l+1 (N x 64) */
     const uint32_t chunk = 1024u, nch = N / chunk;
