@@ -49,6 +49,9 @@ class clhip_material(C.Structure):
         ("eta", C.c_float * 3),
         ("k", C.c_float * 3),
         ("interior_ior", C.c_float),
+        ("reflectance_tex", C.c_uint32),
+        ("roughness_tex", C.c_uint32),
+        ("normal_tex", C.c_uint32),
     ]
 
 

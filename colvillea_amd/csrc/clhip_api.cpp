@@ -126,6 +126,7 @@ struct clhip_ctx
     std::vector<MaterialRec> mtl;
     std::vector<uint32_t>    entity_material;
     DevBuf                   d_mtl;
+    DevBuf                   d_texrec; /* TexRec per texture, for material lookups */
 
     /* emitters */
     std::vector<EmitterRec> emitters;
@@ -263,6 +264,8 @@ SceneView scene_view(const clhip_ctx* c)
     s.uv         = c->d_uv.as<const float>();
     s.idx        = c->d_idx.as<const uint32_t>();
     s.mtl        = c->d_mtl.as<const MaterialRec>();
+    s.tex        = c->d_texrec.as<const TexRec>();
+    s.ntex       = (uint32_t)c->textures.size();
     s.emitters   = c->d_emitters.as<const EmitterRec>();
     s.nemitters  = (uint32_t)c->emitters.size();
     s.dome_index = c->dome_index;
@@ -1043,6 +1046,16 @@ int clhip_create_texture(clhip_ctx* ctx, const float* data, uint32_t width, uint
     CLHIP_CHECK(ctx, upload(*t.buf, rgba));
     ctx->textures.push_back(std::move(t));
     *texture_id = (int32_t)ctx->textures.size() - 1;
+    /* the device table of all textures, for material lookups */
+    std::vector<TexRec> recs(ctx->textures.size());
+    for (size_t i = 0; i < recs.size(); ++i)
+    {
+        recs[i].data = ctx->textures[i].buf->as<const float4>();
+        recs[i].w    = ctx->textures[i].w;
+        recs[i].h    = ctx->textures[i].h;
+    }
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream)); /* the old table may be in use */
+    CLHIP_CHECK(ctx, upload(ctx->d_texrec, recs));
     return CLHIP_OK;
 }
 
@@ -1300,6 +1313,16 @@ int clhip_build_materials(clhip_ctx* ctx, const clhip_material* materials, uint3
         const float rough = m.roughness;
         r.alpha           = std::fmax(rough * rough, 1e-4f); /* metal.h:131, microfacet.h:27 */
         r.ior             = m.interior_ior;
+        const uint32_t nt = (uint32_t)ctx->textures.size();
+        if (m.reflectance_tex > nt || m.roughness_tex > nt || m.normal_tex > nt)
+            return fail(ctx, CLHIP_E_INVALID, "material %u: texture id beyond the %u created", i, nt);
+        if (m.reflectance_tex && m.type == CLHIP_MTL_GLASS)
+            return fail(ctx, CLHIP_E_INVALID, "material %u: glass has no reflectance texture (glass.h:61-67)", i);
+        if (m.roughness_tex && m.type != CLHIP_MTL_METAL)
+            return fail(ctx, CLHIP_E_INVALID, "material %u: only metal takes a roughness texture (metal.h:44-48)", i);
+        r.tex_r     = m.reflectance_tex;
+        r.tex_rough = m.roughness_tex;
+        r.tex_n     = m.normal_tex;
     }
     CLHIP_CHECK(ctx, upload(ctx->d_mtl, recs));
     ctx->mtl        = std::move(recs);

@@ -92,7 +92,9 @@ struct MeshRec
     uint32_t flags;
 };
 
-/* Material: type, reflectance, GGX alpha = max(roughness^2, 1e-4), eta, k, ior. */
+/* Material: type, reflectance, GGX alpha = max(roughness^2, 1e-4), eta, k,
+ * ior, and texture ids + 1 (0 = none) for the reflectance, the roughness
+ * and the normal map. */
 struct alignas(16) MaterialRec
 {
     uint32_t type;
@@ -101,7 +103,15 @@ struct alignas(16) MaterialRec
     float    eta[3];
     float    k[3];
     float    ior;
-    float    pad[2];
+    uint32_t tex_r, tex_rough, tex_n, pad;
+};
+static_assert(sizeof(MaterialRec) == 64, "MaterialRec is four 16-byte words");
+
+/* An RGBA32F texture: texels row-major, row 0 at v = 0. */
+struct alignas(16) TexRec
+{
+    const float4* data;
+    uint32_t      w, h;
 };
 
 struct alignas(16) EmitterRec
@@ -125,6 +135,8 @@ struct SceneView
     const float*       uv;
     const uint32_t*    idx;
     const MaterialRec* mtl;
+    const TexRec*      tex; /* material textures by id */
+    uint32_t           ntex;
     const EmitterRec*  emitters;
     uint32_t           nemitters;
     int32_t            dome_index;

@@ -476,6 +476,91 @@ __device__ __forceinline__ void surface_interaction(const SceneView& sc, const H
     si.material = ip->material;
 }
 
+/* The pull-up fix of a normal-mapped frame (Material::
+ * fixNormalPerturbingShadingFrame, material.h:236-285): on the side of wo,
+ * if wo mirrored about the shading normal would go below the geometric
+ * surface, tilt the normal to the half vector of wo and that mirror
+ * direction lifted by 1.1 times its depth below the surface. */
+__device__ __forceinline__ void pull_up(Frame& f, f3 ng, f3 wo)
+{
+    if (dot3(wo, ng) < 0.0f)
+    {
+        f.n = neg3(f.n);
+        ng  = neg3(ng);
+    }
+    if (!(dot3(wo, ng) > 0.0f))
+        return;
+    const f3 r = reflect3(wo, f.n);
+    if (dot3(r, ng) > 0.0f)
+        return;
+    const float a = fabsf(dot3(r, ng)), b = dot3(f.n, ng);
+    if (!(b > 0.0f))
+        return;
+    f.n = norm3(add3(norm3(add3(r, scl3(scl3(f.n, a / b), 1.1f))), wo));
+    f.t = norm3(sub3(f.t, scl3(f.n, dot3(f.t, f.n))));
+    f.s = norm3(cross3(f.n, f.t));
+    if (dot3(f.n, ng) < 0.0f)
+    {
+        f.n = neg3(f.n);
+        f.t = neg3(f.t);
+    }
+}
+
+/* Texture-driven inputs of a material at a surface point: the reflectance
+ * and the roughness lookups of Material::getBSDF (diffusemtl.h:22-31,
+ * metal.h:36-55; roughness = first channel, squared, alpha >= 1e-4), and
+ * the normal-mapped frame of Material::getShadingFrame (material.h:124-219):
+ * n = normalize(2 tex - 1) in the tangent frame (dpdu, dpdv, Ns) of the hit
+ * (Ns rather than Ng: D10), t = dpdu made orthogonal to n, s = n x t, turned
+ * to the side of Ng; then the pull-up fix on diffuse and metal.  A normal
+ * map texel of (0.5, 0.5, 0.5) -- no direction -- leaves the frame as is. */
+__device__ __forceinline__ void material_at(const SceneView& sc, MaterialRec& m, Surface& si, f3 wo)
+{
+    if (m.tex_r)
+    {
+        const TexRec t = sc.tex[m.tex_r - 1u];
+        const float4 c = tex_eval(t.data, t.w, t.h, si.u, si.v);
+        m.r[0]         = c.x;
+        m.r[1]         = c.y;
+        m.r[2]         = c.z;
+    }
+    if (m.tex_rough)
+    {
+        const TexRec t = sc.tex[m.tex_rough - 1u];
+        const float  x = tex_eval(t.data, t.w, t.h, si.u, si.v).x;
+        m.alpha        = fmaxf(x * x, 1e-4f);
+    }
+    if (m.tex_n)
+    {
+        const TexRec t  = sc.tex[m.tex_n - 1u];
+        const float4 c  = tex_eval(t.data, t.w, t.h, si.u, si.v);
+        const f3     nt = f3{c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f};
+        if (dot3(nt, nt) > 0.0f)
+        {
+            f3 n  = norm3(to_world(si.f, norm3(nt)));
+            f3 tm = sub3(si.f.t, scl3(n, dot3(si.f.t, n)));
+            f3 sm;
+            if (dot3(tm, tm) > 0.0f)
+            {
+                tm = norm3(tm);
+                sm = norm3(cross3(n, tm));
+            }
+            else
+                make_frame(n, &tm, &sm);
+            if (dot3(n, si.ng) < 0.0f)
+            {
+                n  = neg3(n);
+                tm = neg3(tm);
+            }
+            si.f.s = sm;
+            si.f.t = tm;
+            si.f.n = n;
+            if (m.type == 0u || m.type == 1u)
+                pull_up(si.f, si.ng, wo);
+        }
+    }
+}
+
 /* ------------------------------------------------------------ kernels */
 
 /* generateCameraRays (raygeneration.cu:83-94): seed (px, py, lane<<20 | s,
@@ -667,8 +752,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
             Surface si;
             surface_interaction(sc, hr, si);
             p                    = si.p;
-            const MaterialRec mt = sc.mtl[si.material];
-            const f3          wo = neg3(dir);
+            const f3    wo = neg3(dir);
+            MaterialRec mt = sc.mtl[si.material];
+            material_at(sc, mt, si, wo);
             const f3          woL = to_local(si.f, wo);
             const uint32_t    k0  = 2u + 4u * (depth - 1u);
             /* next-event estimation (raygeneration.cu:310-384; D8, D9) */
@@ -816,6 +902,7 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
             hr.prim = prim;
             surface_interaction(sc, hr, si);
             mt = sc.mtl[si.material];
+            material_at(sc, mt, si, neg3(dir));
         }
         const bool guided = vertex && (mt.type == 0u || mt.type == 1u);
         const f3   wo     = neg3(dir);

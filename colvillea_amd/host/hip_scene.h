@@ -115,19 +115,45 @@ private:
     int32_t            m_deviceId = -1; /* integrator texture id once uploaded */
 };
 
-/* core::Material (nodebase/material.h) compiled to clhip_material. */
+/* core::Material (nodebase/material.h) compiled to clhip_material: the
+ * constants, and the textures (reflectance, roughness, normal map) by the
+ * integrator ids they receive when the engine uploads them. */
 class Material : public Node
 {
 public:
-    Material(uint64_t id, const clhip_material& m, Scene* scene) : Node(id), m_mtl(m), m_scene(scene) {}
-    const clhip_material& compile() const noexcept { return m_mtl; }
+    Material(uint64_t id, const clhip_material& m, Scene* scene, std::shared_ptr<Texture> reflectance = nullptr,
+             std::shared_ptr<Texture> roughness = nullptr)
+        : Node(id), m_mtl(m), m_scene(scene), m_reflectanceTex(std::move(reflectance)),
+          m_roughnessTex(std::move(roughness))
+    {
+    }
+    clhip_material compile() const
+    {
+        clhip_material m  = m_mtl;
+        m.reflectance_tex = texId(m_reflectanceTex);
+        m.roughness_tex   = texId(m_roughnessTex);
+        m.normal_tex      = texId(m_normalmapTex);
+        return m;
+    }
     /* MaterialEdited */
     void setReflectance(const vec3f& r);
     void setRoughness(float r);
+    /* Material::setNormalmap / getNormalmap (nodebase/material.h:28-36) */
+    void                            setNormalmap(std::shared_ptr<Texture> normalmap);
+    const std::shared_ptr<Texture>& getNormalmap() const noexcept { return m_normalmapTex; }
 
 private:
-    clhip_material m_mtl;
-    Scene*         m_scene;
+    static uint32_t texId(const std::shared_ptr<Texture>& t)
+    {
+        if (!t)
+            return 0u;
+        if (t->deviceId() < 0)
+            throw Error(CLHIP_E_STATE, "material texture not uploaded: compile through the RenderEngine");
+        return (uint32_t)t->deviceId() + 1u;
+    }
+    clhip_material           m_mtl;
+    Scene*                   m_scene;
+    std::shared_ptr<Texture> m_reflectanceTex, m_roughnessTex, m_normalmapTex;
 };
 
 /* core::Emitter: directional sun or HDRI dome. */
@@ -282,6 +308,30 @@ public:
         }
         return add(m_materials, std::make_shared<Material>(nextId(), m, this), Action::MaterialAdded);
     }
+    /* createMaterial(Diffuse, reflectanceTex) (scene.h:163) */
+    std::shared_ptr<Material> createDiffuseMaterial(const std::shared_ptr<Texture>& reflectanceTex)
+    {
+        clhip_material m{};
+        m.type = CLHIP_MTL_DIFFUSE;
+        return add(m_materials, std::make_shared<Material>(nextId(), m, this, reflectanceTex),
+                   Action::MaterialAdded);
+    }
+    /* createMetalMaterial with a roughness texture (scene.h:166) */
+    std::shared_ptr<Material> createMetalMaterial(const vec3f& specularReflectance,
+                                                  const std::shared_ptr<Texture>& roughness, const vec3f& eta,
+                                                  const vec3f& k)
+    {
+        clhip_material m{};
+        m.type = CLHIP_MTL_METAL;
+        for (int c = 0; c < 3; ++c)
+        {
+            m.reflectance[c] = specularReflectance[c];
+            m.eta[c]         = eta[c];
+            m.k[c]           = k[c];
+        }
+        return add(m_materials, std::make_shared<Material>(nextId(), m, this, nullptr, roughness),
+                   Action::MaterialAdded);
+    }
     std::shared_ptr<Material> createGlassMaterial(float roughness, float interiorIOR)
     {
         clhip_material m{};
@@ -424,6 +474,11 @@ inline void Material::setReflectance(const vec3f& r)
 inline void Material::setRoughness(float r)
 {
     m_mtl.roughness = r;
+    m_scene->markEdited(Action::MaterialEdited);
+}
+inline void Material::setNormalmap(std::shared_ptr<Texture> normalmap)
+{
+    m_normalmapTex = std::move(normalmap);
     m_scene->markEdited(Action::MaterialEdited);
 }
 inline void TriangleMesh::setVertices(std::vector<float> verts)

@@ -51,23 +51,41 @@ class TriangleMesh:
             self.uvs = np.ascontiguousarray(self.uvs, np.float32).reshape(-1, 2)
 
 
-@dataclass
-class DiffuseMtl:
-    reflectance: Sequence[float]
+class _Normalmapped:
+    """Material::setNormalmap / getNormalmap (nodebase/material.h:28-36):
+    a tangent-space normal map, texel (r, g, b) -> normalize(2 rgb - 1)."""
+
+    def setNormalmap(self, normalmap: Optional["ImageTexture2D"]):
+        self.normalmap = normalmap
+
+    def getNormalmap(self) -> Optional["ImageTexture2D"]:
+        return self.normalmap
 
 
 @dataclass
-class MetalMtl:
-    specularReflectance: Sequence[float]
-    roughness: float
+class DiffuseMtl(_Normalmapped):
+    """reflectance: rgb, or an ImageTexture2D (Scene::createMaterial with a
+    reflectance texture, scene.h:161-163)."""
+    reflectance: object
+    normalmap: Optional["ImageTexture2D"] = None
+
+
+@dataclass
+class MetalMtl(_Normalmapped):
+    """specularReflectance: rgb or ImageTexture2D; roughness: float or an
+    ImageTexture2D whose first channel is the roughness (metal.h:36-55)."""
+    specularReflectance: object
+    roughness: object
     eta: Sequence[float]
     k: Sequence[float]
+    normalmap: Optional["ImageTexture2D"] = None
 
 
 @dataclass
-class GlassMtl:
+class GlassMtl(_Normalmapped):
     roughness: float
     interiorIOR: float
+    normalmap: Optional["ImageTexture2D"] = None
 
 
 @dataclass
@@ -197,15 +215,35 @@ class Integrator:
                                                capi.fptr(xf), len(mesh_of)))
 
     def buildMaterials(self, materials):
+        """Materials in order; their textures (reflectance, roughness,
+        normal map) are created on the device once per texture object."""
         arr = (capi.clhip_material * max(1, len(materials)))()
+        made = {}
+
+        def tex(t) -> int:  # texture id + 1 (0 = none)
+            if t is None:
+                return 0
+            if id(t) not in made:
+                made[id(t)] = (t, self._texture(t) + 1)
+            return made[id(t)][1]
+
         for i, m in enumerate(materials):
             if isinstance(m, DiffuseMtl):
                 arr[i].type = capi.MTL_DIFFUSE
-                arr[i].reflectance = _f3(m.reflectance)
+                if isinstance(m.reflectance, ImageTexture2D):
+                    arr[i].reflectance_tex = tex(m.reflectance)
+                else:
+                    arr[i].reflectance = _f3(m.reflectance)
             elif isinstance(m, MetalMtl):
                 arr[i].type = capi.MTL_METAL
-                arr[i].reflectance = _f3(m.specularReflectance)
-                arr[i].roughness = float(m.roughness)
+                if isinstance(m.specularReflectance, ImageTexture2D):
+                    arr[i].reflectance_tex = tex(m.specularReflectance)
+                else:
+                    arr[i].reflectance = _f3(m.specularReflectance)
+                if isinstance(m.roughness, ImageTexture2D):
+                    arr[i].roughness_tex = tex(m.roughness)
+                else:
+                    arr[i].roughness = float(m.roughness)
                 arr[i].eta = _f3(m.eta)
                 arr[i].k = _f3(m.k)
             elif isinstance(m, GlassMtl):
@@ -214,6 +252,7 @@ class Integrator:
                 arr[i].interior_ior = float(m.interiorIOR)
             else:
                 raise TypeError(f"unknown material {type(m).__name__}")
+            arr[i].normal_tex = tex(getattr(m, "normalmap", None))
         self._check(self._lib.clhip_build_materials(self._ctx, arr, len(materials)))
 
     def buildGeometryEntities(self, entities: Sequence[Entity]):
@@ -221,7 +260,12 @@ class Integrator:
         self._check(self._lib.clhip_build_entities(self._ctx, capi.uptr(idx), len(idx)))
 
     def _texture(self, tex: ImageTexture2D) -> int:
-        img = np.ascontiguousarray(tex.image, np.float32)
+        img = np.asarray(tex.image, np.float32)
+        if img.ndim == 2:  # one channel (e.g. a roughness map): replicated to rgb
+            img = img[..., None]
+        if img.shape[2] == 1:
+            img = np.repeat(img, 3, axis=2)
+        img = np.ascontiguousarray(img)
         h, w, ch = img.shape
         tid = C.c_int32()
         self._check(self._lib.clhip_create_texture(self._ctx, capi.fptr(img), w, h, ch, C.byref(tid)))

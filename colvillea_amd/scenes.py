@@ -338,6 +338,60 @@ def make_c4(seed: int = 5, width: int = 3840, height: int = 2160, spp: int = 128
                      info={"seed": seed, "triangles": int(ntri)})
 
 
+def checker_texture(n: int, a, b, cells: int = 8) -> ImageTexture2D:
+    i, j = np.meshgrid(np.arange(n), np.arange(n))
+    on = ((i * cells // n + j * cells // n) % 2 == 0)[..., None]
+    return ImageTexture2D(np.where(on, np.asarray(a, np.float32), np.asarray(b, np.float32)).astype(np.float32))
+
+
+def bump_normal_map(n: int, amp: float = 0.4, waves: int = 4) -> ImageTexture2D:
+    """A tangent-space normal map of sinusoidal bumps, encoded (n + 1) / 2."""
+    i, j = np.meshgrid(np.arange(n), np.arange(n))
+    x = amp * np.sin(2 * np.pi * waves * i / n)
+    y = amp * np.cos(2 * np.pi * waves * j / n)
+    v = np.stack([x, y, np.ones_like(x)], -1)
+    v /= np.linalg.norm(v, axis=-1, keepdims=True)
+    return ImageTexture2D(((v + 1.0) * 0.5).astype(np.float32))
+
+
+def make_textured(seed: int = 6, width: int = 320, height: int = 240, spp: int = 16, hdri_w: int = 256,
+                  hdri_h: int = 128) -> SceneDesc:
+    """Texture-driven materials: a uv-mapped, tangent-carrying ground with a
+    checker reflectance and a bump normal map; icospheres (no uvs: the
+    barycentric fallback; no tangents: the frame from the normal) with a
+    checker diffuse, a metal with reflectance and roughness textures and a
+    normal map, a normal-mapped diffuse and a normal-mapped rough glass;
+    sky HDRI and a sun.  Depth 6, Russian roulette from 3."""
+    rng = np.random.default_rng(seed)
+    half = 6.0
+    q = quad_y0(half)
+    uv = ((q.vertices[:, [0, 2]] + half) / (2 * half) * 3.0).astype(np.float32)  # wraps 3 times
+    ground = TriangleMesh(q.vertices, q.triangles, normals=q.normals,
+                          tangents=np.tile(np.array([[1, 0, 0]], np.float32), (4, 1)), uvs=uv)
+    sphere = icosphere(3)
+    rough = rng.uniform(0.05, 0.7, (16, 16)).astype(np.float32)
+    g = DiffuseMtl(checker_texture(64, (0.7, 0.7, 0.65), (0.15, 0.2, 0.3)))
+    g.setNormalmap(bump_normal_map(64))
+    eta, k = CONDUCTOR_TABLE[0]
+    metal = MetalMtl(checker_texture(32, (0.95, 0.8, 0.5), (0.6, 0.6, 0.6), cells=4),
+                     ImageTexture2D(np.repeat(rough[..., None], 3, 2)), eta, k)
+    metal.setNormalmap(bump_normal_map(32, 0.3, 2))
+    bumpy = DiffuseMtl((0.6, 0.3, 0.2))
+    bumpy.setNormalmap(bump_normal_map(32, 0.6, 3))
+    glass = GlassMtl(0.3, 1.5)
+    glass.setNormalmap(bump_normal_map(32, 0.2, 2))
+    materials = [g, DiffuseMtl(checker_texture(32, (0.8, 0.1, 0.1), (0.9, 0.9, 0.9))), metal, bumpy, glass]
+    centres = [(-2.2, 0.9, 0.0), (-0.7, 0.9, 0.6), (0.8, 0.9, -0.3), (2.3, 0.9, 0.4)]
+    xf = np.stack([IDENTITY] + [translate_scale(c, 0.85) for c in centres])
+    el, az = math.radians(40.0), math.radians(-35.0)
+    to_sun = np.array([math.cos(el) * math.sin(az), math.sin(el), math.cos(el) * math.cos(az)])
+    sun = DirectionalEmitter((4.0, 3.8, 3.5), tuple(-to_sun), math.radians(0.6))
+    dome = HDRIDome(ImageTexture2D(sky_hdri(hdri_w, hdri_h, to_sun, sun_peak=5.0)))
+    cam = look_at((0.0, 3.2, 7.5), (0.0, 0.6, 0.0), (0, 1, 0), 42.0, width, height)
+    return SceneDesc("textured", [ground, sphere], [0, 1, 1, 1, 1], [0, 1, 2, 3, 4], xf.astype(np.float32),
+                     materials, [sun, dome], cam, width, height, spp, 6, 3, info={"seed": seed})
+
+
 def crop(desc: SceneDesc, x0: int, y0: int, w: int, h: int, spp: Optional[int] = None) -> SceneDesc:
     """The same scene seen through the pixel rectangle [x0, x0 + w) x
     [y0, y0 + h) of desc's film, as a w x h film of its own (the camera's

@@ -72,7 +72,16 @@ enum
 
 /* kernel::Material payloads: DiffuseMtl (materials/diffusemtl.h:13-17),
  * MetalMtl (metal.h:93-113), GlassMtl (glass.h:53-56).  GGX alpha is
- * roughness^2 as in the reference (metal.h:131, glass.h:66). */
+ * roughness^2 as in the reference (metal.h:131, glass.h:66).
+ * Textures (clhip_create_texture ids, stored as id + 1 so that 0 = none)
+ * replace the constant where set, as Material::getBSDF does
+ * (diffusemtl.h:22-31, metal.h:36-55): reflectance_tex for the diffuse
+ * reflectance or the metal specular reflectance (rgb), roughness_tex for
+ * the metal roughness (first channel, squared after the lookup).
+ * normal_tex is the tangent-space normal map of Material::setNormalmap
+ * (material.h:81-84, getShadingFrameImpl 169-219, with the pull-up fix of
+ * 236-285 on diffuse and metal).  Lookups are bilinear with wrap at the
+ * surface uv (imagetex2d.h:34-37). */
 typedef struct
 {
     uint32_t type;
@@ -81,6 +90,9 @@ typedef struct
     float    eta[3];         /* metal */
     float    k[3];           /* metal */
     float    interior_ior;   /* glass (exterior IOR is 1) */
+    uint32_t reflectance_tex; /* texture id + 1, 0 = none (diffuse, metal) */
+    uint32_t roughness_tex;   /* texture id + 1, 0 = none (metal) */
+    uint32_t normal_tex;      /* texture id + 1, 0 = none (any type) */
 } clhip_material;
 
 /* EmitterType (libkernel/base/emitter.h:16-26) */

@@ -60,7 +60,15 @@ typedef struct
     float    roughness;      /* metal, glass: GGX alpha = roughness^2 */
     float    eta[3], k[3];   /* metal complex IOR */
     float    ior;            /* glass interior IOR (exterior = 1) */
+    uint32_t reflectance_tex, roughness_tex, normal_tex; /* texture index + 1, 0 = none */
 } orc_material;
+
+/* An RGBA float32 texture (row 0 at v = 0) for material lookups. */
+typedef struct
+{
+    const float* rgba;
+    uint32_t     w, h;
+} orc_texture;
 
 enum
 {
@@ -98,6 +106,8 @@ typedef struct
     uint32_t            nemitters;
     int32_t             dome_index; /* index into emitters, -1 = no dome */
     orc_dome            dome;
+    const orc_texture*  textures; /* material textures (kept alive by the caller) */
+    uint32_t            ntextures;
 } orc_scene_desc;
 
 typedef struct

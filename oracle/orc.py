@@ -29,7 +29,12 @@ class orc_instance(C.Structure):
 
 class orc_material(C.Structure):
     _fields_ = [("type", C.c_uint32), ("reflectance", C.c_float * 3), ("roughness", C.c_float),
-                ("eta", C.c_float * 3), ("k", C.c_float * 3), ("ior", C.c_float)]
+                ("eta", C.c_float * 3), ("k", C.c_float * 3), ("ior", C.c_float),
+                ("reflectance_tex", C.c_uint32), ("roughness_tex", C.c_uint32), ("normal_tex", C.c_uint32)]
+
+
+class orc_texture(C.Structure):
+    _fields_ = [("rgba", f32p), ("w", C.c_uint32), ("h", C.c_uint32)]
 
 
 class orc_emitter(C.Structure):
@@ -47,7 +52,8 @@ class orc_scene_desc(C.Structure):
                 ("instances", C.POINTER(orc_instance)), ("ninstances", C.c_uint32),
                 ("materials", C.POINTER(orc_material)), ("nmaterials", C.c_uint32),
                 ("emitters", C.POINTER(orc_emitter)), ("nemitters", C.c_uint32),
-                ("dome_index", C.c_int32), ("dome", orc_dome)]
+                ("dome_index", C.c_int32), ("dome", orc_dome),
+                ("textures", C.POINTER(orc_texture)), ("ntextures", C.c_uint32)]
 
 
 class orc_render_params(C.Structure):
@@ -121,17 +127,48 @@ def _up(a):
     return None if a is None else a.ctypes.data_as(u32p)
 
 
-def material_struct(m) -> orc_material:
-    """Convert an integrator material (DiffuseMtl / MetalMtl / GlassMtl)."""
-    from colvillea_amd.integrator import DiffuseMtl, GlassMtl, MetalMtl
+def texture_rgba(t) -> np.ndarray:
+    """An ImageTexture2D as contiguous RGBA float32 (1 or 3 channels are
+    widened as the renderer widens them: grey -> rgb, alpha 1)."""
+    img = np.asarray(t.image, np.float32)
+    if img.ndim == 2:
+        img = img[..., None]
+    if img.shape[2] == 1:
+        img = np.repeat(img, 3, axis=2)
+    if img.shape[2] == 3:
+        img = np.concatenate([img, np.ones(img.shape[:2] + (1,), np.float32)], 2)
+    return np.ascontiguousarray(img, np.float32)
+
+
+def material_struct(m, tex=None) -> orc_material:
+    """Convert an integrator material (DiffuseMtl / MetalMtl / GlassMtl).
+    tex(texture) -> index + 1 in the scene's texture table."""
+    from colvillea_amd.integrator import DiffuseMtl, GlassMtl, ImageTexture2D, MetalMtl
+
+    def tid(t):
+        if t is None:
+            return 0
+        if tex is None:
+            raise ValueError("a textured material needs a scene texture table")
+        return tex(t)
+
     s = orc_material()
     if isinstance(m, DiffuseMtl):
         s.type = 0
-        s.reflectance = (C.c_float * 3)(*m.reflectance)
+        if isinstance(m.reflectance, ImageTexture2D):
+            s.reflectance_tex = tid(m.reflectance)
+        else:
+            s.reflectance = (C.c_float * 3)(*m.reflectance)
     elif isinstance(m, MetalMtl):
         s.type = 1
-        s.reflectance = (C.c_float * 3)(*m.specularReflectance)
-        s.roughness = m.roughness
+        if isinstance(m.specularReflectance, ImageTexture2D):
+            s.reflectance_tex = tid(m.specularReflectance)
+        else:
+            s.reflectance = (C.c_float * 3)(*m.specularReflectance)
+        if isinstance(m.roughness, ImageTexture2D):
+            s.roughness_tex = tid(m.roughness)
+        else:
+            s.roughness = m.roughness
         s.eta = (C.c_float * 3)(*m.eta)
         s.k = (C.c_float * 3)(*m.k)
     elif isinstance(m, GlassMtl):
@@ -140,6 +177,7 @@ def material_struct(m) -> orc_material:
         s.ior = m.interiorIOR
     else:
         raise TypeError(type(m))
+    s.normal_tex = tid(getattr(m, "normalmap", None))
     return s
 
 
@@ -181,9 +219,20 @@ class Scene:
             inst[i].mesh = desc.instance_mesh[i]
             inst[i].material = desc.instance_material[i]
             inst[i].xform = (C.c_float * 12)(*np.asarray(desc.instance_xform[i], np.float32).reshape(-1))
+        texs, tex_index = [], {}
+
+        def tex(t):
+            if id(t) not in tex_index:
+                rgba = texture_rgba(t)
+                self._keep.append(rgba)
+                texs.append(orc_texture(_fp(rgba), rgba.shape[1], rgba.shape[0]))
+                tex_index[id(t)] = len(texs)
+            return tex_index[id(t)]
+
         mats = (orc_material * max(1, len(desc.materials)))()
         for i, m in enumerate(desc.materials):
-            mats[i] = material_struct(m)
+            mats[i] = material_struct(m, tex)
+        tarr = (orc_texture * max(1, len(texs)))(*texs)
         ems = (orc_emitter * max(1, len(desc.emitters)))()
         dome_index = -1
         dome = orc_dome()
@@ -205,9 +254,9 @@ class Scene:
                 self.tables = tb
                 dome = orc_dome(_fp(rgba), rgba.shape[1], rgba.shape[0], _fp(tb[0]), _fp(tb[1]), _fp(tb[2]),
                                 _fp(tb[3]))
-        self._keep += [meshes, inst, mats, ems]
+        self._keep += [meshes, inst, mats, ems, tarr]
         d = orc_scene_desc(meshes, len(desc.meshes), inst, n, mats, len(desc.materials), ems, len(desc.emitters),
-                           dome_index, dome)
+                           dome_index, dome, tarr, len(texs))
         self._desc_struct = d
         self.handle = L.orc_scene_create(C.byref(d))
 

@@ -1042,6 +1042,88 @@ static void SFX(surface_interaction)(const orc_scene* s, const SFX(hit) * h, SFX
     si->material = in->material;
 }
 
+/* Material::fixNormalPerturbingShadingFrame (material.h:236-285): the
+ * IRay pull-up of a normal-mapped frame on the side of wo. */
+static void SFX(pull_up)(SFX(frame) * f, V3 ng, V3 wo)
+{
+    if (SFX(dot)(wo, ng) < RC(0.0))
+    {
+        f->n = SFX(neg)(f->n);
+        ng   = SFX(neg)(ng);
+    }
+    if (!(SFX(dot)(wo, ng) > RC(0.0)))
+        return;
+    const V3 r = SFX(reflect)(wo, f->n);
+    if (SFX(dot)(r, ng) > RC(0.0))
+        return;
+    const R a = RFABS(SFX(dot)(r, ng)), b = SFX(dot)(f->n, ng);
+    if (!(b > RC(0.0)))
+        return;
+    f->n = SFX(normalize)(SFX(add)(SFX(normalize)(SFX(add)(r, SFX(scl)(SFX(scl)(f->n, a / b), (R)(float)1.1))), wo));
+    f->t = SFX(normalize)(SFX(sub)(f->t, SFX(scl)(f->n, SFX(dot)(f->t, f->n))));
+    f->s = SFX(normalize)(SFX(cross)(f->n, f->t));
+    if (SFX(dot)(f->n, ng) < RC(0.0))
+    {
+        f->n = SFX(neg)(f->n);
+        f->t = SFX(neg)(f->t);
+    }
+}
+
+/* Texture-driven material inputs at a surface point: Material::getBSDF's
+ * reflectance and roughness lookups (diffusemtl.h:22-31, metal.h:36-55;
+ * roughness = first channel, squared by mtl_alpha) and the normal-mapped
+ * frame of Material::getShadingFrame (material.h:124-219), taken in the
+ * hit's tangent frame (dpdu, dpdv, Ns) (D10), then the pull-up on diffuse
+ * and metal.  A normal-map texel with no direction (0.5, 0.5, 0.5) leaves
+ * the frame unchanged. */
+static void SFX(material_at)(const orc_scene* s, orc_material* m, SFX(surface) * si, V3 wo)
+{
+    R c[4];
+    if (m->reflectance_tex)
+    {
+        const orc_texture* t = &s->d.textures[m->reflectance_tex - 1u];
+        SFX(tex_eval)(t->rgba, t->w, t->h, si->u, si->v, c);
+        m->reflectance[0] = (float)c[0];
+        m->reflectance[1] = (float)c[1];
+        m->reflectance[2] = (float)c[2];
+    }
+    if (m->roughness_tex)
+    {
+        const orc_texture* t = &s->d.textures[m->roughness_tex - 1u];
+        SFX(tex_eval)(t->rgba, t->w, t->h, si->u, si->v, c);
+        m->roughness = (float)c[0];
+    }
+    if (m->normal_tex)
+    {
+        const orc_texture* t = &s->d.textures[m->normal_tex - 1u];
+        SFX(tex_eval)(t->rgba, t->w, t->h, si->u, si->v, c);
+        const V3 nt = SFX(mk)(c[0] * RC(2.0) - RC(1.0), c[1] * RC(2.0) - RC(1.0), c[2] * RC(2.0) - RC(1.0));
+        if (SFX(dot)(nt, nt) > RC(0.0))
+        {
+            V3 n  = SFX(normalize)(SFX(to_world)(&si->f, SFX(normalize)(nt)));
+            V3 tm = SFX(sub)(si->f.t, SFX(scl)(n, SFX(dot)(si->f.t, n)));
+            V3 sm;
+            if (SFX(dot)(tm, tm) > RC(0.0))
+            {
+                tm = SFX(normalize)(tm);
+                sm = SFX(normalize)(SFX(cross)(n, tm));
+            }
+            else
+                SFX(make_frame)(n, &tm, &sm);
+            if (SFX(dot)(n, si->ng) < RC(0.0))
+            {
+                n  = SFX(neg)(n);
+                tm = SFX(neg)(tm);
+            }
+            si->f.s = sm;
+            si->f.t = tm;
+            si->f.n = n;
+            if (m->type == ORC_MTL_DIFFUSE || m->type == ORC_MTL_METAL)
+                SFX(pull_up)(&si->f, si->ng, wo);
+        }
+    }
+}
+
 /* ------------------------------------------------------------ integrator */
 
 #define ORC_EXTRA_DRAW(depth, slot) (0x80000000u + 8u * (uint32_t)(depth) + (uint32_t)(slot))
@@ -1115,9 +1197,11 @@ static void SFX(path)(const orc_scene* s, const orc_render_params* P, uint32_t p
             break;
         SFX(surface) si;
         SFX(surface_interaction)(s, &h, &si);
-        const orc_material* mt = &s->materials[si.material];
-        const V3 wo  = SFX(neg)(ray.d);
-        const V3 woL = SFX(to_local)(&si.f, wo);
+        const V3     wo   = SFX(neg)(ray.d);
+        orc_material mloc = s->materials[si.material];
+        SFX(material_at)(s, &mloc, &si, wo);
+        const orc_material* mt  = &mloc;
+        const V3            woL = SFX(to_local)(&si.f, wo);
         const uint32_t k0 = 2u + 4u * (depth - 1u);
 
         /* next-event estimation (raygeneration.cu:310-384) */

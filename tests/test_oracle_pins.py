@@ -14,7 +14,7 @@ from scipy import stats
 
 import orc
 from colvillea_amd import DiffuseMtl, GlassMtl, MetalMtl, scenes
-from colvillea_amd.integrator import Camera, DirectionalEmitter, HDRIDome, ImageTexture2D
+from colvillea_amd.integrator import Camera, DirectionalEmitter, HDRIDome, ImageTexture2D, TriangleMesh
 from colvillea_amd.scenes import SceneDesc
 
 f32 = np.float32
@@ -575,6 +575,73 @@ def test_sun_lit_plane_closed_form():
     img /= 16
     expect = rho * L * math.sin(R) ** 2 * math.sin(el)
     assert np.allclose(img, expect, rtol=2e-3)
+
+
+def _mapped_quad(half):
+    """quad_y0 with uvs over [0, 1]^2 and tangent +x (so dpdu = +x,
+    dpdv = Ns x dpdu = -z at the hit)."""
+    q = scenes.quad_y0(half)
+    uv = (q.vertices[:, [0, 2]] + half) / (2 * half)
+    return TriangleMesh(q.vertices, q.triangles, normals=q.normals,
+                        tangents=np.tile(np.array([[1, 0, 0]], np.float32), (4, 1)), uvs=uv.astype(np.float32))
+
+
+def _const_tex(rgb, w=4, h=4):
+    return ImageTexture2D(np.tile(np.asarray(rgb, np.float32), (h, w, 1)))
+
+
+def test_normal_mapped_sun_lit_plane_closed_form():
+    """The sun-lit plane with a constant tangent-space normal map: radiance =
+    albedo * L_sun * sin^2(R) * (n' . l), n' = normalize(x dpdu + y dpdv +
+    z Ns) for the decoded texel (x, y, z) (Material::getShadingFrameImpl,
+    material.h:169-219); the sun has x and z components, so swapping dpdu
+    and dpdv or the sign of either changes the value."""
+    rho, L, R = 0.6, 5.0, math.radians(0.5)
+    el, az = math.radians(60.0), math.radians(30.0)
+    to_sun = np.array([math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az)])
+    nt = np.array([0.3, 0.2, 1.0])
+    mtl = DiffuseMtl((rho,) * 3)
+    mtl.setNormalmap(_const_tex((nt + 1.0) / 2.0))
+    cam = scenes.look_at((0, 2.0, 0.0), (0.0, 0.0, 0.5), (0, 1, 0), 30.0, 16, 16)
+    desc = SceneDesc("sun_nm", [_mapped_quad(50.0)], [0], [0], scenes.IDENTITY[None].copy(), [mtl],
+                     [DirectionalEmitter((L,) * 3, tuple(-to_sun), R)], cam, 16, 16, 1, 3, 0)
+    img, _ = orc.Scene(desc).render("f64", samples=(0, 16))
+    img /= 16
+    n = nt[0] * np.array([1, 0, 0]) + nt[1] * np.array([0, 0, -1]) + nt[2] * np.array([0, 1, 0])
+    n /= np.linalg.norm(n)
+    expect = rho * L * math.sin(R) ** 2 * float(n @ to_sun)
+    assert np.allclose(img, expect, rtol=2e-3), (img.mean(), expect)
+
+
+def test_constant_textures_equal_constants():
+    """A constant reflectance / roughness texture renders as the constant
+    (up to the bilinear weights' rounding); a flat normal map (0, 0, 1) on a
+    flat quad leaves the frame as it is."""
+    quad = _mapped_quad(4.0)
+    sph = scenes.icosphere(2)
+    el = math.radians(50.0)
+    sun = DirectionalEmitter((4.0,) * 3, (-math.cos(el), -math.sin(el), 0.0), math.radians(1.0))
+    dome = HDRIDome(ImageTexture2D(np.full((8, 16, 4), 0.3, np.float32)))
+    cam = scenes.look_at((0, 3.0, 5.0), (0.0, 0.3, 0.0), (0, 1, 0), 40.0, 24, 16)
+    xf = np.stack([scenes.IDENTITY, scenes.translate_scale((0.0, 0.8, 0.0), 0.7)])
+
+    def render(mats):
+        d = SceneDesc("tex", [quad, sph], [0, 1], [0, 1], xf.astype(np.float32), mats, [sun, dome], cam, 24, 16,
+                      1, 4, 0)
+        img, cnt = orc.Scene(d).render("f32", samples=(0, 4))
+        return img, cnt
+
+    eta, k = (0.2, 0.9, 1.1), (3.9, 2.4, 2.2)
+    base, cb = render([DiffuseMtl((0.5, 0.4, 0.3)), MetalMtl((0.9, 0.8, 0.7), 0.4, eta, k)])
+    flat = DiffuseMtl((0.5, 0.4, 0.3))
+    flat.setNormalmap(_const_tex((0.5, 0.5, 1.0)))
+    texd, ct = render([DiffuseMtl(_const_tex((0.5, 0.4, 0.3))),
+                       MetalMtl(_const_tex((0.9, 0.8, 0.7)), _const_tex((0.4, 0.0, 0.0)), eta, k)])
+    flatd, cf = render([flat, MetalMtl((0.9, 0.8, 0.7), 0.4, eta, k)])
+    assert base.mean() > 0
+    for img, cnt in ((texd, ct), (flatd, cf)):
+        assert np.array_equal(cnt["rays"], cb["rays"]) and np.array_equal(cnt["visible"], cb["visible"])
+        assert np.allclose(img, base, rtol=1e-4, atol=1e-6)
 
 
 def test_russian_roulette_unbiased():
