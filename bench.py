@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the streaming-copy probe")
     ap.add_argument("--copy-iters", type=int, default=20)
     ap.add_argument("--bvh-width", type=int, default=4, choices=(4, 8), help="tree width of the C1 scene's BVH")
+    ap.add_argument("--samples-per-batch", type=int, default=0,
+                    help="spp of every pixel per batch for the 1080p render (0 = library default)")
     ap.add_argument("--no-c2", action="store_true", help="skip the batched 256-view render")
     ap.add_argument("--c2-steps", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true")
@@ -242,6 +244,7 @@ def main():
     log(f"scene: {desc.num_triangles()} triangles, generated in {time.perf_counter() - t0:.1f} s")
     integ = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
     integ.setBVHWidth(args.bvh_width)
+    desc.samples_per_batch = args.samples_per_batch
     t0 = time.perf_counter()
     upload_scene(integ, desc)
     integ.sync()

@@ -146,6 +146,8 @@ class clhip_guide_stats(C.Structure):
         ("last_count", C.c_uint32),
         ("last_loss", C.c_float),
         ("train_ms", C.c_double),
+        ("issued", C.c_uint64),
+        ("training", C.c_uint32),
     ]
 
 

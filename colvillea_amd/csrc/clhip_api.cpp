@@ -312,6 +312,16 @@ int sync_instances(clhip_ctx* c)
     return CLHIP_OK;
 }
 
+/* Paths per batch: the launch setting, else 2^25.  A persistent traversal
+ * launch finishes with its slowest lane, so the tail costs about 0.2 ms per
+ * launch on MI355X whatever its size; 2^25 paths (16 spp of a 1080p film,
+ * 156 B of queues per path = 5.2 GB) make it small against the launch's
+ * work. */
+uint32_t default_paths_per_batch(const clhip_ctx* c)
+{
+    return c->launch.paths_per_batch ? c->launch.paths_per_batch : (32u << 20);
+}
+
 int ensure_render_buffers(clhip_ctx* c, size_t paths)
 {
     const size_t npix = (size_t)c->width * c->height;
@@ -1431,8 +1441,12 @@ int clhip_set_params(clhip_ctx* ctx, const clhip_params* p)
         return fail(ctx, CLHIP_E_INVALID, "max_depth %u outside [1, %d]", p->max_depth, CLHIP_MAX_DEPTH);
     if (p->view >= (1u << 12))
         return fail(ctx, CLHIP_E_INVALID, "view %u outside [0, 4096)", p->view);
+    /* the batch size does not change any sample, so only the estimator's
+     * parameters restart the accumulation */
+    const bool same = p->max_depth == ctx->params.max_depth && p->rr_depth == ctx->params.rr_depth &&
+                      p->view == ctx->params.view;
     ctx->params = *p;
-    return clhip_reset(ctx);
+    return same ? CLHIP_OK : clhip_reset(ctx);
 }
 
 int clhip_reset(clhip_ctx* ctx)
@@ -1497,8 +1511,8 @@ int clhip_render(clhip_ctx* ctx, uint32_t spp)
         return fail(ctx, CLHIP_E_INVALID, "%u + %u samples exceed the 2^20 sample indices of a view", ctx->spp, spp);
     const uint64_t npix = (uint64_t)ctx->width * ctx->height;
     uint32_t       spb  = ctx->params.samples_per_batch;
-    if (spb == 0)
-        spb = (uint32_t)std::max<uint64_t>(1, (2u << 20) / npix);
+    if (spb == 0) /* whole samples of the film up to the batch's path budget */
+        spb = (uint32_t)std::max<uint64_t>(1, default_paths_per_batch(ctx) / npix);
     spb = std::min(spb, spp);
     while ((uint64_t)spb * npix > 0xffffffffull / 2)
         spb = std::max(1u, spb / 2);
@@ -1559,7 +1573,7 @@ int clhip_get_launch(const clhip_ctx* ctx, clhip_launch* out)
     out->trace_blocks_per_cu  = ctx->blocks_trace / ctx->num_cus;
     out->shadow_blocks_per_cu = ctx->blocks_shadow / ctx->num_cus;
     out->shade_blocks_per_cu  = ctx->blocks_shade / ctx->num_cus;
-    out->paths_per_batch      = ctx->launch.paths_per_batch ? ctx->launch.paths_per_batch : (4u << 20);
+    out->paths_per_batch      = default_paths_per_batch(ctx);
     out->graph                = use_graph(ctx) ? 1u : 2u;
     out->order                = ctx->launch.order ? ctx->launch.order : 1u;
     return CLHIP_OK;
@@ -1630,7 +1644,7 @@ int clhip_render_views(clhip_ctx* ctx, const clhip_view* views, uint32_t nviews)
             const uint64_t wy = (uint64_t)ctx->vviews[y].npix * ctx->vspp[y] * ctx->vviews[y].max_depth;
             return wx < wy;
         });
-    const uint32_t cap = std::max(maxpix, ctx->launch.paths_per_batch ? ctx->launch.paths_per_batch : (4u << 20));
+    const uint32_t cap = std::max(maxpix, default_paths_per_batch(ctx));
     std::vector<BatchRec>   batches;
     std::vector<BatchEntry> entries;
     uint32_t                fill = cap;
@@ -2202,6 +2216,8 @@ int clhip_guide_stats_get(clhip_ctx* ctx, clhip_guide_stats* out)
     out->samples    = g.samples_total;
     out->last_count = g.trained;
     out->last_loss  = g.loss;
+    out->issued     = ctx->g_issued;
+    out->training   = guide_recording(ctx) ? 1u : 0u;
     for (auto& t : ctx->train_timed)
     {
         float ms = 0.0f;

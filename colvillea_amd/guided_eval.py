@@ -12,7 +12,9 @@ float64 oracle is tested separately at small sizes).
 
 Time is wall-clock around render + synchronise of each chunk (readbacks
 excluded), so the guided tracer's time includes its network inference and
-every training step.  With `warm_spp`, the guided tracer first renders and
+every training step.  The plain tracer runs at the library's default batch
+size (its fastest); the guided tracer at the schedule's `samples_per_batch`
+(it trains after every batch).  With `warm_spp`, the guided tracer first renders and
 trains on that many spp (with `warm_steps_per_batch` Adam steps per batch,
 if given) and discards their film (their time is counted).  T_plain(eps) is the plain tracer's time to its final
 error eps; T_guided(eps) is where the guided tracer's error curve crosses
@@ -34,10 +36,13 @@ EPS_DEN = 1e-2
 
 
 def _render_curve(it: Integrator, spp_points: List[int], ref, ref_var, warm_spp: int = 0,
-                  warm_guide: Optional[dict] = None, guide: Optional[dict] = None) -> list:
+                  warm_guide: Optional[dict] = None, guide: Optional[dict] = None, desc=None) -> list:
     """Error after each chunk.  warm_spp > 0: first render that many spp
     (the guided tracer trains on them, with the settings `warm_guide` if
-    given, then `guide`), then discard that film; the time is counted."""
+    given, then `guide`), then discard that film; the time is counted.
+    With `guide` and `desc`, once training has stopped (max_steps issued)
+    the remaining chunks use the library's default batch size, as the plain
+    tracer does."""
     it.resetIterationIndex()
     it.sync()
     done, t_acc, curve = 0, 0.0, []
@@ -51,7 +56,11 @@ def _render_curve(it: Integrator, spp_points: List[int], ref, ref_var, warm_spp:
         it.resetIterationIndex()
         it.sync()
         t_acc += time.perf_counter() - t
+    fast = guide is None
     for s in spp_points:
+        if not fast and desc is not None and not it.guideStats()["training"]:
+            it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0, samples_per_batch=0)
+            fast = True
         t = time.perf_counter()
         it.render(s - done)
         it.sync()
@@ -105,20 +114,27 @@ def time_to_equal_error(desc: SceneDesc, ref_spp: int, spp_points: List[int], gu
         upload_scene(it, desc)
     try:
         ref, ref_var = ref if ref is not None else reference(it, desc, ref_spp, log)
-        it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0, samples_per_batch=1)
         if plain is None:
+            # the plain tracer at its own fastest setting: the library's batch size
+            it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0, samples_per_batch=0)
             it.setGuide(False)
             plain = _render_curve(it, spp_points, ref, ref_var)
             log(f"plain: {plain[-1]}")
         eps = plain[-1]["relmse"]
         tp = plain[-1]["seconds"]
-        g = {k: v for k, v in guide.items() if k not in ("seed", "warm_spp", "warm_steps_per_batch")}
+        # the guided tracer trains once per batch: its batch size is part of the schedule
+        it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0,
+                     samples_per_batch=int(guide.get("samples_per_batch", 1)))
+        skip = ("seed", "warm_spp", "warm_steps_per_batch", "samples_per_batch")
+        g = {k: v for k, v in guide.items() if k not in skip}
         wg = dict(g, steps_per_batch=guide["warm_steps_per_batch"]) if guide.get("warm_steps_per_batch") else None
         runs = []
         for seed in (seeds or [guide.get("seed", 1)]):
             it.guideReset(seed)
             it.setGuide(True, **g)
-            guided = _render_curve(it, spp_points, ref, ref_var, guide.get("warm_spp", 0), wg, g)
+            it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0,
+                         samples_per_batch=int(guide.get("samples_per_batch", 1)))
+            guided = _render_curve(it, spp_points, ref, ref_var, guide.get("warm_spp", 0), wg, g, desc)
             gst = it.guideStats()
             log(f"guided (seed {seed}): {guided[-1]}, {gst['steps']} training steps")
             it.setGuide(False)

@@ -119,7 +119,8 @@ typedef struct
     uint32_t max_depth;        /* number of path segments; NEE at vertices 1..max_depth-1 */
     uint32_t rr_depth;         /* Russian roulette from this vertex depth (0 = off) */
     uint32_t view;             /* RNG view lane: z = (view << 20) | sample index */
-    uint32_t samples_per_batch;/* samples of every pixel in flight together (0 = auto) */
+    uint32_t samples_per_batch;/* samples of every pixel in flight together (0 = as many whole samples
+                                  as fit the launch's paths_per_batch, default 2^25 paths) */
 } clhip_params;
 
 /* Exact counters of the last clhip_render (sum over its samples).  Index b
@@ -281,7 +282,8 @@ typedef struct
     uint32_t trace_blocks_per_cu;  /* persistent closest-hit grid (256-thread blocks per CU) */
     uint32_t shadow_blocks_per_cu; /* persistent any-hit grid */
     uint32_t shade_blocks_per_cu;  /* camera and shading grids */
-    uint32_t paths_per_batch;      /* queue chunk of clhip_render_views (raised to the largest view) */
+    uint32_t paths_per_batch;      /* paths in flight per batch (0 = 2^25; clhip_render_views raises it to the
+                                      largest view; clhip_render uses whole samples of the film) */
     uint32_t graph;                /* 1: replay a captured hipGraph per batch; 2: plain launches */
     uint32_t order;                /* clhip_render_views: 1 = given order, 2 = least work first */
 } clhip_launch;
@@ -330,6 +332,8 @@ typedef struct
     uint32_t last_count;    /* samples in the last step */
     float    last_loss;     /* weighted negative log-likelihood of the last step (per unit weight) */
     double   train_ms;      /* summed device time of the training steps of the last render */
+    uint64_t issued;        /* Adam steps issued since the last reset (steps counts those with samples) */
+    uint32_t training;      /* 1 while the next batch records and trains (enabled, train, below max_steps) */
 } clhip_guide_stats;
 
 int clhip_set_guide(clhip_ctx* ctx, const clhip_guide* guide);

@@ -273,9 +273,11 @@ def test_trained_guide_beats_plain_at_equal_samples(room_ref):
         # then compare at equal samples with frozen weights (different seeds: view lane 1)
         it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=1, samples_per_batch=1)
         it.setGuide(True, bsdf_fraction=0.5, train=0)
+        it.resetIterationIndex()
         it.render(64)
         guided = it.readbackSums() / 64
-        it.setGuide(False)
+        it.setGuide(False)  # switching the proposal keeps the film: start the plain one afresh
+        it.resetIterationIndex()
         it.render(64)
         plain = it.readbackSums() / 64
     finally:
@@ -337,3 +339,30 @@ def test_gather_matches_oracle(room_ref):
     assert 500 < n <= 131072
     assert st["last_count"] == n
     assert np.array_equal(smp[:n].view(np.uint32), ref.view(np.uint32))
+
+
+def test_frozen_guide_is_invariant_to_batch_size(room_ref):
+    """With fixed weights, the guided render of the same samples does not
+    depend on how many samples of each pixel share a batch (counter-based
+    RNG, one network): counts exact, sums to float64 order."""
+    desc, it, _ = room_ref
+    try:
+        it.guideReset(31)
+        it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0, samples_per_batch=1)
+        it.setGuide(True, bsdf_fraction=0.5, train=1, record_paths=desc.width * desc.height, batch_size=8192,
+                    learning_rate=3e-3, steps_per_batch=2)
+        it.resetIterationIndex()
+        it.render(16)  # a trained, non-trivial network
+        it.setGuide(True, bsdf_fraction=0.5, train=0)
+        out = []
+        for spb in (1, 16, 5):
+            it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0, samples_per_batch=spb)
+            it.resetIterationIndex()
+            it.render(16)
+            out.append((it.readbackSums(), it.counters()["rays"]))
+    finally:
+        it.setGuide(False)
+        it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0)
+    for sums, rays in out[1:]:
+        assert np.array_equal(rays, out[0][1])
+        assert np.allclose(sums, out[0][0], rtol=1e-9, atol=1e-12)
