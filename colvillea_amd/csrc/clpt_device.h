@@ -683,10 +683,13 @@ CLPT_D void ray_setup(RayW& r)
     r.Sx           = comp3(r.d, kx) / dz;
     r.Sy           = comp3(r.d, ky) / dz;
     r.Sz           = 1.0f / dz;
+    /* box-slab reciprocals: only for culling, whose bounds are widened by
+     * 1e-5 relative, so the hardware reciprocal (1 ulp) serves; the shear
+     * constants above decide hits and stay correctly rounded */
     const float tiny = 1e-20f;
-    r.inv = f3{1.0f / (fabsf(r.d.x) > tiny ? r.d.x : (r.d.x < 0.0f ? -tiny : tiny)),
-               1.0f / (fabsf(r.d.y) > tiny ? r.d.y : (r.d.y < 0.0f ? -tiny : tiny)),
-               1.0f / (fabsf(r.d.z) > tiny ? r.d.z : (r.d.z < 0.0f ? -tiny : tiny))};
+    r.inv = f3{__builtin_amdgcn_rcpf(fabsf(r.d.x) > tiny ? r.d.x : (r.d.x < 0.0f ? -tiny : tiny)),
+               __builtin_amdgcn_rcpf(fabsf(r.d.y) > tiny ? r.d.y : (r.d.y < 0.0f ? -tiny : tiny)),
+               __builtin_amdgcn_rcpf(fabsf(r.d.z) > tiny ? r.d.z : (r.d.z < 0.0f ? -tiny : tiny))};
 }
 
 /* Watertight two-sided triangle test; hit iff tmin <= t <= tmax. */

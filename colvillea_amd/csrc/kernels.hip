@@ -122,6 +122,24 @@ struct TraverseStack
     st.gstride = gridDim.x * blockDim.x;                                                                               \
     st.sp      = 0
 
+/* The rays of a wave's share of a persistent launch -- rays
+ * w + k * stride + j (w = the wave's first global thread, j < 64) -- handed
+ * to whichever lanes finish first: one ballot per refill and a wave-uniform
+ * cursor, no atomics.  A lane that draws short rays takes more of them, so
+ * the wave's lanes finish together instead of waiting for the lane with the
+ * most work. */
+struct WavePool
+{
+    uint32_t base, stride, next;
+    __device__ __forceinline__ uint32_t index(uint32_t m) const { return base + (m >> 6) * stride + (m & 63u); }
+    /* the lanes in `mask` (ballot of the finished lanes) take the next rays */
+    __device__ __forceinline__ uint32_t take(unsigned long long mask)
+    {
+        const uint32_t m = next + (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
+        return index(m);
+    }
+};
+
 struct HitRec
 {
     float    t, b1, b2;
@@ -258,10 +276,15 @@ __device__ __forceinline__ bool visit_node<8>(const void* nodes, const RayW& r, 
     return false;
 }
 
-/* Two-level BVH traversal (W = 4 or 8 wide) as a per-lane state machine: step() does one
- * unit of work (one node, or one triangle of a leaf) and reports when the
- * ray is finished, so a lane whose ray ends can start its next ray at once
- * instead of idling until the slowest lane of its wave is done.
+/* Two-level BVH traversal (W = 4 or 8 wide) as a per-lane state machine:
+ * step() does up to one node visit and one triangle test.  A leaf found
+ * during traversal is held while the lane keeps visiting inner nodes of
+ * the same tree, and its triangles are tested one per step alongside, so
+ * a lane makes progress on both kinds of work in one pass of the wave
+ * (a second leaf, an instance boundary or the marker wait until the held
+ * leaf is done).  It reports when the ray is finished, so a lane whose ray
+ * ends can start its next ray at once.  The closest hit does not depend on
+ * the visiting order (ties are broken by (t, instance, primitive)).
  * ANY = occlusion: stop at the first hit (D15: optix.cu:106 traces shadow
  * rays with closest-hit flags).  The world ray is (wo, wd); inside an
  * instance the object-space ray replaces it until the marker pushed on entry
@@ -269,24 +292,22 @@ __device__ __forceinline__ bool visit_node<8>(const void* nodes, const RayW& r, 
 template <bool ANY, int W>
 struct Tracer
 {
-    f3       wo, wd;
-    float    tmin;
-    RayW     r;
+    RayW     rw; /* the world ray, set up once */
+    RayW     r;  /* the ray of the current tree (world, or an instance's object space) */
     uint32_t cur_inst;
     HitRec   best;
     uint32_t ref;
     uint32_t leaf_next, leaf_left;
+    bool     has_ref;
     bool     found;
 
     __device__ __forceinline__ void start(const SceneView& sc, TraverseStack& st, f3 o, f3 d, float t0, float t1)
     {
-        wo     = o;
-        wd     = d;
-        tmin   = t0;
-        r.o    = o;
-        r.d    = d;
-        r.tmin = t0;
-        ray_setup(r);
+        rw.o    = o;
+        rw.d    = d;
+        rw.tmin = t0;
+        ray_setup(rw);
+        r         = rw;
         cur_inst  = 0xffffffffu;
         best.t    = t1;
         best.b1   = 0.0f;
@@ -296,6 +317,7 @@ struct Tracer
         found     = false;
         st.sp     = 0;
         ref       = sc.tlas_root;
+        has_ref   = true;
         leaf_next = 0;
         leaf_left = 0;
     }
@@ -303,24 +325,29 @@ struct Tracer
     /* Returns true when the ray is finished. */
     __device__ __forceinline__ bool step(const SceneView& sc, TraverseStack& st, uint32_t& nnodes, uint32_t& ntris)
     {
-        bool need_pop = false;
-        if (leaf_left == 0)
+        /* node unit */
+        if (has_ref)
         {
+            const bool held = leaf_left > 0u;
             if (ref == CLPT_STACK_MARKER)
             {
-                r.o    = wo;
-                r.d    = wd;
-                r.tmin = tmin;
-                ray_setup(r);
-                cur_inst = 0xffffffffu;
-                need_pop = true;
+                if (!held)
+                {
+                    r        = rw;
+                    cur_inst = 0xffffffffu;
+                    has_ref  = false;
+                }
             }
             else if ((ref & (CLPT_LEAF_BIT | CLPT_INST_BIT)) == CLPT_LEAF_BIT)
             {
-                leaf_next = ref & (CLPT_MAX_TRIS - 1u);
-                leaf_left = ((ref >> 27) & 7u) + 1u;
+                if (!held)
+                {
+                    leaf_next = ref & (CLPT_MAX_TRIS - 1u);
+                    leaf_left = ((ref >> 27) & 7u) + 1u;
+                    has_ref   = false;
+                }
             }
-            else
+            else if ((ref & CLPT_LEAF_BIT) == 0u || !held)
             {
                 if (ref & CLPT_LEAF_BIT)
                 {
@@ -331,21 +358,17 @@ struct Tracer
 #pragma unroll
                     for (int k = 0; k < 12; ++k)
                         m[k] = ip->inv[k];
-                    RayW w;
-                    w.o    = wo;
-                    w.d    = wd;
-                    w.tmin = tmin;
-                    ray_to_object(m, w, r);
+                    ray_to_object(m, rw, r);
                     st.push(CLPT_STACK_MARKER, -CLPT_FLT_MAX);
                     ref = ip->blas_root;
                 }
                 ++nnodes;
-                need_pop = !visit_node<W>(sc.nodes, r, best.t, st, ref);
+                has_ref = visit_node<W>(sc.nodes, r, best.t, st, ref);
             }
         }
-        if (leaf_left > 0)
+        /* triangle unit: one triangle of the held leaf */
+        if (leaf_left > 0u)
         {
-            /* one triangle of the current leaf */
             const TriRec tr = sc.tris[leaf_next];
             ++ntris;
             ++leaf_next;
@@ -370,21 +393,22 @@ struct Tracer
                     found     = true;
                 }
             }
-            if (leaf_left == 0)
-                need_pop = true;
         }
-        if (need_pop)
+        if (!has_ref)
         {
             /* the next entry that can still hold a closer hit */
-            for (;;)
+            while (st.sp > 0)
             {
-                if (st.sp == 0)
-                    return true;
                 float t;
                 st.pop(ref, t);
                 if (t <= best.t)
+                {
+                    has_ref = true;
                     break;
+                }
             }
+            if (!has_ref && leaf_left == 0u)
+                return true;
         }
         return false;
     }
@@ -668,7 +692,8 @@ __global__ __launch_bounds__(256) void k_trace(SceneView sc, RayQueue q, const u
     uint32_t       nn = 0, nt = 0;
     const uint32_t n    = *count;
     const float    tmin = depth == 1 ? 0.0f : CLPT_RAY_EPS;
-    uint32_t       i    = st.gid;
+    WavePool       pool{st.gid - lane_id(), st.gstride, 64u};
+    uint32_t       i      = st.gid;
     bool           active = i < n;
     Tracer<false, W> tr;
     if (active)
@@ -676,20 +701,30 @@ __global__ __launch_bounds__(256) void k_trace(SceneView sc, RayQueue q, const u
         const float4 o = q.o[i], d = q.d[i];
         tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, tmin, CLPT_FLT_MAX);
     }
-    while (active)
+    while (__any(active))
     {
-        if (tr.step(sc, st, nn, nt))
+        bool fin = false;
+        if (active && tr.step(sc, st, nn, nt))
         {
             hb.h[i]    = make_float4(tr.found ? tr.best.t : -1.0f, tr.best.b1, tr.best.b2,
                                      __uint_as_float(tr.found ? tr.best.inst : 0xffffffffu));
             hb.prim[i] = tr.found ? tr.best.prim : 0xffffffffu;
-            i += st.gstride;
-            active = i < n;
-            if (active)
+            fin        = true;
+        }
+        const unsigned long long mask = __ballot(fin);
+        if (mask)
+        {
+            if (fin)
             {
-                const float4 o = q.o[i], d = q.d[i];
-                tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, tmin, CLPT_FLT_MAX);
+                i      = pool.take(mask);
+                active = i < n;
+                if (active)
+                {
+                    const float4 o = q.o[i], d = q.d[i];
+                    tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, tmin, CLPT_FLT_MAX);
+                }
             }
+            pool.next += (uint32_t)__popcll(mask);
         }
     }
     const unsigned long long sn = wave_sum_u64(nn), stt = wave_sum_u64(nt);
@@ -1056,6 +1091,7 @@ __global__ __launch_bounds__(256) void k_shadow(SceneView sc, ShadowQueue sq, ui
     CLPT_DECLARE_STACK(st, gstack);
     uint32_t       nn = 0, nt = 0, nvis = 0;
     const uint32_t n      = bc->shadow[depth];
+    WavePool       pool{st.gid - lane_id(), st.gstride, 64u};
     uint32_t       i      = st.gid;
     bool           active = i < n;
     Tracer<true, W> tr;
@@ -1064,9 +1100,10 @@ __global__ __launch_bounds__(256) void k_shadow(SceneView sc, ShadowQueue sq, ui
         const float4 o = sq.o[i], d = sq.d[i];
         tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_RAY_EPS, CLPT_FLT_MAX);
     }
-    while (active)
+    while (__any(active))
     {
-        if (tr.step(sc, st, nn, nt))
+        bool fin = false;
+        if (active && tr.step(sc, st, nn, nt))
         {
             if (!tr.found)
             {
@@ -1081,13 +1118,22 @@ __global__ __launch_bounds__(256) void k_shadow(SceneView sc, ShadowQueue sq, ui
                         guide_E[tg] = make_float4(o.w, d.w, x.x, 0.0f);
                 }
             }
-            i += st.gstride;
-            active = i < n;
-            if (active)
+            fin = true;
+        }
+        const unsigned long long mask = __ballot(fin);
+        if (mask)
+        {
+            if (fin)
             {
-                const float4 o = sq.o[i], d = sq.d[i];
-                tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_RAY_EPS, CLPT_FLT_MAX);
+                i      = pool.take(mask);
+                active = i < n;
+                if (active)
+                {
+                    const float4 o = sq.o[i], d = sq.d[i];
+                    tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_RAY_EPS, CLPT_FLT_MAX);
+                }
             }
+            pool.next += (uint32_t)__popcll(mask);
         }
     }
     const unsigned long long sn = wave_sum_u64(nn), stt = wave_sum_u64(nt), sv = wave_sum_u64(nvis);
