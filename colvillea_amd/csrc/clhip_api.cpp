@@ -132,7 +132,7 @@ struct clhip_ctx
     std::vector<EmitterRec> emitters;
     int32_t                 dome_index = -1, dome_tex = -1;
     uint32_t                dome_w = 0, dome_h = 0;
-    DevBuf                  d_emitters, d_pcond, d_cdfcond, d_pv, d_cdfv, d_rowint;
+    DevBuf                  d_emitters, d_pcond, d_cdfcond, d_pv, d_cdfv, d_rowint, d_cutv, d_cutcond;
 
     /* camera / parameters (the single-view film) */
     ViewRec      cam{};
@@ -279,6 +279,8 @@ SceneView scene_view(const clhip_ctx* c)
     s.cdfcond   = c->d_cdfcond.as<const float>();
     s.pv        = c->d_pv.as<const float>();
     s.cdfv      = c->d_cdfv.as<const float>();
+    s.cutv      = c->d_cutv.as<const uint32_t>();
+    s.cutcond   = c->d_cutcond.as<const uint32_t>();
     s.tlas_root = c->tlas_root;
     s.ninst     = (uint32_t)c->inst.size();
     return s;
@@ -1402,10 +1404,13 @@ int clhip_build_emitters(clhip_ctx* ctx, const clhip_emitter* emitters, uint32_t
         CLHIP_CHECK(ctx, ctx->d_pv.alloc(((size_t)t.h + 1) * sizeof(float)));
         CLHIP_CHECK(ctx, ctx->d_cdfv.alloc(((size_t)t.h + 1) * sizeof(float)));
         CLHIP_CHECK(ctx, ctx->d_rowint.alloc((size_t)t.h * sizeof(double)));
+        CLHIP_CHECK(ctx, ctx->d_cutv.alloc((CLPT_CUTS + 1u) * sizeof(uint32_t)));
+        CLHIP_CHECK(ctx, ctx->d_cutcond.alloc((size_t)t.h * (CLPT_CUTS + 1u) * sizeof(uint32_t)));
         CLHIP_CHECK(ctx, clpt::launch_dome_tables(ctx->stream, t.buf->as<const float4>(), t.w, t.h,
                                                   ctx->d_pcond.as<float>(), ctx->d_cdfcond.as<float>(),
                                                   ctx->d_pv.as<float>(), ctx->d_cdfv.as<float>(),
-                                                  ctx->d_rowint.as<double>()));
+                                                  ctx->d_rowint.as<double>(), ctx->d_cutv.as<uint32_t>(),
+                                                  ctx->d_cutcond.as<uint32_t>()));
         CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
     return CLHIP_OK;

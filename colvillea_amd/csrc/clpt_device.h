@@ -510,12 +510,23 @@ CLPT_D uint32_t upper_bound_f(const float* a, uint32_t begin, uint32_t end, floa
 }
 
 /* Distribution1D::sample1D: hdridome.h:31-60 */
+/* Cut points of a CDF search: cut[k] = upper_bound(cdf[begin, end), k / 256)
+ * for k = 0..256.  upper_bound is monotone in the value, so for u in
+ * [k/256, (k+1)/256) its result lies in [cut[k], cut[k+1]] and a search of
+ * that range returns the same index as a search of the whole CDF. */
+CLPT_D uint32_t upper_bound_cut(const float* a, const uint32_t* cut, float value)
+{
+    const uint32_t k = min((uint32_t)(value * (float)CLPT_CUTS), CLPT_CUTS - 1u); /* value in [0, 1) */
+    return upper_bound_f(a, cut[k], cut[k + 1], value);
+}
+
+/* cut: the CDF's cut points, or NULL for a search of the whole CDF */
 CLPT_D float sample1d(const float* f, const float* cdf, uint32_t N, float c, float u, uint32_t cdf_off, float* pdf,
-                      uint32_t* off)
+                      uint32_t* off, const uint32_t* cut)
 {
     if (c == 0.0f)
         return 0.0f;
-    uint32_t o = upper_bound_f(cdf, cdf_off, cdf_off + N + 1, u);
+    uint32_t o = cut ? upper_bound_cut(cdf, cut, u) : upper_bound_f(cdf, cdf_off, cdf_off + N + 1, u);
     o          = o - 1u;
     if (o < cdf_off)
         o = cdf_off;
@@ -536,9 +547,10 @@ CLPT_D void sample2d(const SceneView& sc, float ux, float uy, float* su, float* 
     const uint32_t nu = sc.dome_w, nv = sc.dome_h;
     float          pu = 0.0f, pvv = 0.0f;
     uint32_t       off = 0;
-    *sv                = sample1d(sc.pv, sc.cdfv, nv, sc.pv[nv], uy, 0, &pvv, &off);
+    *sv                = sample1d(sc.pv, sc.cdfv, nv, sc.pv[nv], uy, 0, &pvv, &off, sc.cutv);
     const float cv     = sc.pcond[(size_t)off * (nu + 1) + nu];
-    *su                = sample1d(sc.pcond, sc.cdfcond, nu, cv, ux, off * (nu + 1), &pu, &off);
+    *su = sample1d(sc.pcond, sc.cdfcond, nu, cv, ux, off * (nu + 1), &pu, &off,
+                   sc.cutcond ? sc.cutcond + (size_t)off * (CLPT_CUTS + 1u) : nullptr);
     *pdf               = pu * pvv;
 }
 

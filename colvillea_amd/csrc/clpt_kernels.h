@@ -23,7 +23,7 @@ hipError_t launch_batch_end(hipStream_t s, const BatchCounters* bc, TotalCounter
 hipError_t launch_set_cursor(hipStream_t s, uint32_t* cursor, uint32_t value);
 hipError_t launch_resolve(hipStream_t s, const double* accum, uint32_t npix, uint32_t spp, float4* out);
 hipError_t launch_dome_tables(hipStream_t s, const float4* tex, uint32_t w, uint32_t h, float* pcond, float* cdfcond,
-                              float* pv, float* cdfv, double* rowint);
+                              float* pv, float* cdfv, double* rowint, uint32_t* cutv, uint32_t* cutcond);
 hipError_t launch_rng_batch(hipStream_t s, const uint4* seeds, uint32_t n, uint32_t ndraws, uint32_t* ou, float* of);
 hipError_t launch_camera_batch(hipStream_t s, const ViewRec& fr, const uint2* ps, uint32_t n, float4* out);
 hipError_t launch_bsdf_batch(hipStream_t s, const SceneView& sc, const uint32_t* mtl, const float* wi3,

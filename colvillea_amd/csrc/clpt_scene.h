@@ -122,6 +122,9 @@ struct alignas(16) EmitterRec
     float    tan_r; /* tan(angular radius) rounded once on the host */
 };
 
+/* Buckets of the HDRI CDF search cut points (SceneView::cutv, cutcond). */
+#define CLPT_CUTS 256u
+
 /* Everything a kernel needs to find the scene; passed by value. */
 struct SceneView
 {
@@ -146,6 +149,8 @@ struct SceneView
     const float*       cdfcond;
     const float*       pv;
     const float*       cdfv;
+    const uint32_t*    cutv;    /* 257 search cut points of cdfv (CLPT_CUTS buckets) */
+    const uint32_t*    cutcond; /* 257 per row of cdfcond */
     uint32_t           tlas_root;
     uint32_t           ninst;
     uint32_t           bvh_width; /* 4 (Node4) or 8 (Node8) */

@@ -685,7 +685,7 @@ __global__ __launch_bounds__(256) void k_camera(BatchTable bt, GuideView gv, Ray
 }
 
 template <int W>
-__global__ __launch_bounds__(256) void k_trace(SceneView sc, RayQueue q, const uint32_t* count, uint32_t depth,
+__global__ __launch_bounds__(256, 6) void k_trace(SceneView sc, RayQueue q, const uint32_t* count, uint32_t depth,
                                                HitBuf hb, uint2* gstack, unsigned long long* ctr)
 {
     CLPT_DECLARE_STACK(st, gstack);
@@ -1084,7 +1084,7 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
 }
 
 template <int W>
-__global__ __launch_bounds__(256) void k_shadow(SceneView sc, ShadowQueue sq, uint32_t depth, double* accum,
+__global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq, uint32_t depth, double* accum,
                                                 BatchCounters* bc, uint2* gstack, unsigned long long* ctr,
                                                 float4* guide_E)
 {
@@ -1308,6 +1308,23 @@ __global__ __launch_bounds__(256) void k_dome_marginal(uint32_t h, float* pv, fl
         pv[h] = (float)total;
 }
 
+/* Search cut points of the row CDFs (blocks 0..h-1) and of the marginal
+ * (block h): cut[k] = upper_bound over the CDF's entries of k / 256. */
+__global__ __launch_bounds__(256) void k_dome_cuts(uint32_t w, uint32_t h, const float* cdfcond, const float* cdfv,
+                                                    uint32_t* cutv, uint32_t* cutcond)
+{
+    const uint32_t r = blockIdx.x;
+    for (uint32_t k = threadIdx.x; k <= CLPT_CUTS; k += blockDim.x)
+    {
+        const float v = (float)k / (float)CLPT_CUTS;
+        if (r < h)
+            cutcond[(size_t)r * (CLPT_CUTS + 1u) + k] =
+                upper_bound_f(cdfcond, r * (w + 1u), r * (w + 1u) + w + 1u, v);
+        else
+            cutv[k] = upper_bound_f(cdfv, 0u, h + 1u, v);
+    }
+}
+
 /* ------------------------------------------------------------ batch kernels */
 
 __global__ void k_rng_batch(const uint4* seeds, uint32_t n, uint32_t ndraws, uint32_t* out_u, float* out_f)
@@ -1517,11 +1534,12 @@ hipError_t launch_resolve(hipStream_t s, const double* accum, uint32_t npix, uin
 }
 
 hipError_t launch_dome_tables(hipStream_t s, const float4* tex, uint32_t w, uint32_t h, float* pcond, float* cdfcond,
-                              float* pv, float* cdfv, double* rowint)
+                              float* pv, float* cdfv, double* rowint, uint32_t* cutv, uint32_t* cutcond)
 {
     hipLaunchKernelGGL(k_dome_prefilter, dim3(grid_for((uint64_t)w * h, 256, 8192)), dim3(256), 0, s, tex, w, h, pcond);
     hipLaunchKernelGGL(k_dome_rows, dim3(h), dim3(256), 0, s, w, h, pcond, cdfcond, pv, rowint);
     hipLaunchKernelGGL(k_dome_marginal, dim3(1), dim3(256), 0, s, h, pv, cdfv, rowint);
+    hipLaunchKernelGGL(k_dome_cuts, dim3(h + 1), dim3(256), 0, s, w, h, cdfcond, cdfv, cutv, cutcond);
     return hipGetLastError();
 }
 
