@@ -193,6 +193,9 @@ int clhip_build_emitters(clhip_ctx* ctx, const clhip_emitter* emitters, uint32_t
  * nodebase/camera.h:8-27).  Resets the progressive accumulation when the
  * camera differs from the current one (path.h:73-80). */
 int clhip_set_camera(clhip_ctx* ctx, const float pos[3], const float d00[3], const float ddu[3], const float ddv[3]);
+/* Path depth, Russian-roulette depth, RNG view lane and batch size.  A
+ * change of depth, RR depth or lane restarts the accumulation; the batch
+ * size alone does not (it changes no sample). */
 int clhip_set_params(clhip_ctx* ctx, const clhip_params* params);
 
 /* Integrator::render (integrator.h:61; path.cpp:84-231): adds `spp`
