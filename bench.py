@@ -56,10 +56,56 @@ def cpu_baseline(desc, seconds_target: float, threads: int):
     _, cnt = sc.render_rows(rows, "f32", samples=(0, k), nthreads=threads)
     dt = time.perf_counter() - t
     samples = n * w * k
+    model = None
+    try:  # the host CPU the baseline ran on
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
+    except OSError:
+        pass
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "host_threads": os.cpu_count(),
             "sample": f"{n} of {h} rows spread evenly over the {w}x{h} frame x {k} spp ({samples} camera paths, "
                       f"{int(cnt['rays'].sum())} closest-hit rays), depth {desc.max_depth}, float32 oracle, "
                       f"{threads} threads, {dt:.1f} s"}
+
+
+def other_width(desc, args) -> dict:
+    """The same 1080p render on the other tree width (BVH8 when the main
+    line ran BVH4): one warm-up, the best of two timed renders, and the
+    closest-hit kernel's event-timed launches."""
+    from colvillea_amd import Integrator, IntegratorType, upload_scene
+
+    width = 8 if args.bvh_width == 4 else 4
+    it = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
+    try:
+        it.setBVHWidth(width)
+        upload_scene(it, desc)
+        it.render(desc.spp)
+        it.sync()
+        best = None
+        for _ in range(2):
+            it.resetIterationIndex()
+            it.sync()
+            t = time.perf_counter()
+            it.render(desc.spp)
+            it.sync()
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        it.setTiming(True)
+        it.resetIterationIndex()
+        it.render(desc.spp)
+        it.sync()
+        c = it.counters()
+        it.setTiming(False)
+        rays = int(c["rays"][1:desc.max_depth + 1].sum())
+        out = {"bvh_width": width, "value": round(desc.width * desc.height * desc.spp / best / 1e6, 2),
+               "unit": "Msamples/s", "k_trace_ms_per_frame": round(c["trace_ms"], 2),
+               "nodes_per_ray": round(c["trace_nodes"] / max(1, rays), 2),
+               "tris_per_ray": round(c["trace_tris"] / max(1, rays), 2)}
+        log(f"BVH{width}: {out}")
+        return out
+    finally:
+        it.close()
 
 
 def batched_views(integ, args) -> dict:
@@ -221,6 +267,7 @@ def main():
     ap.add_argument("--bvh-width", type=int, default=4, choices=(4, 8), help="tree width of the C1 scene's BVH")
     ap.add_argument("--samples-per-batch", type=int, default=0,
                     help="spp of every pixel per batch for the 1080p render (0 = library default)")
+    ap.add_argument("--no-other-width", action="store_true", help="skip the 1080p render on the other BVH width")
     ap.add_argument("--no-c2", action="store_true", help="skip the batched 256-view render")
     ap.add_argument("--c2-steps", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true")
@@ -285,6 +332,7 @@ def main():
     samples = desc.width * desc.height * desc.spp * args.steps
     value = samples / t_total / 1e6
     batched = None if args.no_c2 else batched_views(integ, args)
+    other = None if args.no_other_width else other_width(desc, args)
     algo_bytes = rays_traced * QUEUE_BYTES_PER_RAY + nodes * NODE_BYTES[args.bvh_width] + tris * TRI_BYTES
     achieved = algo_bytes / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
     per_launch = algo_bytes / max(1, launches)
@@ -328,7 +376,7 @@ def main():
                                   f"{desc.num_triangles()} triangles (fBm grid + {len(desc.instance_mesh) - 1} "
                                   f"instanced icospheres), 2048x1024 log-normal HDRI, seed {desc.info['seed']}",
                       "width": desc.width, "height": desc.height, "spp": desc.spp, "max_depth": desc.max_depth},
-           "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "guided": guided}
+           "roofline": roofline, "cpu_baseline": cpu, "other_bvh_width": other, "batched": batched, "guided": guided}
     if stress is not None:
         out["stress"] = stress
     if integ is not None:
