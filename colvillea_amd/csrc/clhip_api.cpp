@@ -1278,7 +1278,7 @@ int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uin
         r      = clpt::build_bvh4(lo.data(), hi.data(), num_instances, 1u, nodes4, order, leaf);
     }
     /* stack bound: each W-wide level leaves at most W - 1 entries behind,
-     * plus the instance marker and the popped entry */
+     * plus the popped entry and a margin */
     const uint32_t need = (ctx->bvh_width - 1u) * (r.max_depth + max_blas_depth) + 4u;
     if (need > 128u)
         return fail(ctx, CLHIP_E_UNSUPPORTED, "tree too deep for the traversal stack (TLAS %u + BLAS %u levels)",

@@ -661,12 +661,13 @@ CLPT_D f3 sample_emitters(const SceneView& sc, float ux, float uy, f3* dir, floa
 
 /* ------------------------------------------------------------ geometry */
 
+/* A ray set up for traversal.  The lower bound of t is not part of it: it
+ * is the same for every ray of a launch and is passed alongside. */
 struct RayW
 {
-    f3    o, d, inv;
-    float tmin;
-    int   kx, ky, kz;
-    float Sx, Sy, Sz;
+    f3       o, d, inv;
+    uint32_t k; /* the axes kx | ky << 8 | kz << 16 of the watertight test */
+    float    Sx, Sy, Sz;
 };
 
 CLPT_D float comp3(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
@@ -688,13 +689,14 @@ CLPT_D void ray_setup(RayW& r)
         kx          = ky;
         ky          = t;
     }
-    r.kx           = kx;
-    r.ky           = ky;
-    r.kz           = kz;
+    r.k            = (uint32_t)kx | ((uint32_t)ky << 8) | ((uint32_t)kz << 16);
+    /* one correctly rounded division; the shear is a per-ray linear map
+     * applied alike to every vertex, so rounding Sx, Sy through Sz keeps the
+     * test watertight */
     const float dz = comp3(r.d, kz);
-    r.Sx           = comp3(r.d, kx) / dz;
-    r.Sy           = comp3(r.d, ky) / dz;
     r.Sz           = 1.0f / dz;
+    r.Sx           = comp3(r.d, kx) * r.Sz;
+    r.Sy           = comp3(r.d, ky) * r.Sz;
     /* box-slab reciprocals: only for culling, whose bounds are widened by
      * 1e-5 relative, so the hardware reciprocal (1 ulp) serves; the shear
      * constants above decide hits and stay correctly rounded */
@@ -705,16 +707,17 @@ CLPT_D void ray_setup(RayW& r)
 }
 
 /* Watertight two-sided triangle test; hit iff tmin <= t <= tmax. */
-CLPT_D bool tri_test(const RayW& r, float tmax, f3 v0, f3 v1, f3 v2, float* t, float* b1, float* b2)
+CLPT_D bool tri_test(const RayW& r, float tmin, float tmax, f3 v0, f3 v1, f3 v2, float* t, float* b1, float* b2)
 {
     const f3    A = sub3(v0, r.o), B = sub3(v1, r.o), C = sub3(v2, r.o);
-    const float Az0 = comp3(A, r.kz), Bz0 = comp3(B, r.kz), Cz0 = comp3(C, r.kz);
-    const float Ax = comp3(A, r.kx) - r.Sx * Az0;
-    const float Ay = comp3(A, r.ky) - r.Sy * Az0;
-    const float Bx = comp3(B, r.kx) - r.Sx * Bz0;
-    const float By = comp3(B, r.ky) - r.Sy * Bz0;
-    const float Cx = comp3(C, r.kx) - r.Sx * Cz0;
-    const float Cy = comp3(C, r.ky) - r.Sy * Cz0;
+    const int   kx = (int)(r.k & 0xffu), ky = (int)((r.k >> 8) & 0xffu), kz = (int)(r.k >> 16);
+    const float Az0 = comp3(A, kz), Bz0 = comp3(B, kz), Cz0 = comp3(C, kz);
+    const float Ax = comp3(A, kx) - r.Sx * Az0;
+    const float Ay = comp3(A, ky) - r.Sy * Az0;
+    const float Bx = comp3(B, kx) - r.Sx * Bz0;
+    const float By = comp3(B, ky) - r.Sy * Bz0;
+    const float Cx = comp3(C, kx) - r.Sx * Cz0;
+    const float Cy = comp3(C, ky) - r.Sy * Cz0;
     float       U  = Cx * By - Cy * Bx;
     float       V  = Ax * Cy - Ay * Cx;
     float       W  = Bx * Ay - By * Ax;
@@ -732,7 +735,7 @@ CLPT_D bool tri_test(const RayW& r, float tmax, f3 v0, f3 v1, f3 v2, float* t, f
     const float Az = r.Sz * Az0, Bz = r.Sz * Bz0, Cz = r.Sz * Cz0;
     const float T  = U * Az + V * Bz + W * Cz;
     const float tt = T / det;
-    if (!(tt >= r.tmin && tt <= tmax))
+    if (!(tt >= tmin && tt <= tmax))
         return false;
     *t  = tt;
     *b1 = V / det;
@@ -747,7 +750,6 @@ CLPT_D void ray_to_object(const float* m, const RayW& w, RayW& o)
              m[8] * w.o.x + m[9] * w.o.y + m[10] * w.o.z + m[11]};
     o.d    = f3{m[0] * w.d.x + m[1] * w.d.y + m[2] * w.d.z, m[4] * w.d.x + m[5] * w.d.y + m[6] * w.d.z,
              m[8] * w.d.x + m[9] * w.d.y + m[10] * w.d.z};
-    o.tmin = w.tmin;
     ray_setup(o);
 }
 

@@ -43,7 +43,6 @@ static_assert(sizeof(Node8) == 96, "Node8 is six 16-byte words");
 
 /* Child reference encoding. */
 #define CLPT_CHILD_EMPTY 0xffffffffu
-#define CLPT_STACK_MARKER 0xfffffffeu /* pops back to the world-space ray */
 #define CLPT_LEAF_BIT 0x80000000u
 #define CLPT_INST_BIT 0x40000000u
 /* triangle leaf: LEAF | (count-1) << 27 | first triangle (27 bits) */

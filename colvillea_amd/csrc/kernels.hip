@@ -162,12 +162,12 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32
  * hits farther than the nearest (nearest last) with their entry distance,
  * and continue at the nearest.  Returns false when no child was hit. */
 template <int W>
-__device__ __forceinline__ bool visit_node(const void* nodes, const RayW& r, float tmax, TraverseStack& st,
-                                           uint32_t& ref);
+__device__ __forceinline__ bool visit_node(const void* nodes, const RayW& r, float tmin, float tmax,
+                                           TraverseStack& st, uint32_t& ref);
 
 template <>
-__device__ __forceinline__ bool visit_node<4>(const void* nodes, const RayW& r, float tmax, TraverseStack& st,
-                                              uint32_t& ref)
+__device__ __forceinline__ bool visit_node<4>(const void* nodes, const RayW& r, float tmin, float tmax,
+                                              TraverseStack& st, uint32_t& ref)
 {
     const float4* np  = reinterpret_cast<const float4*>(static_cast<const Node4*>(nodes) + ref);
     const float4  lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
@@ -183,7 +183,7 @@ __device__ __forceinline__ bool visit_node<4>(const void* nodes, const RayW& r, 
         const float t0x = (lo_x[c] - r.o.x) * r.inv.x, t1x = (hi_x[c] - r.o.x) * r.inv.x;
         const float t0y = (lo_y[c] - r.o.y) * r.inv.y, t1y = (hi_y[c] - r.o.y) * r.inv.y;
         const float t0z = (lo_z[c] - r.o.z) * r.inv.z, t1z = (hi_z[c] - r.o.z) * r.inv.z;
-        float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+        float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
         float tfar  = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
         tnear       = tnear * 0.99999f;
         tfar        = tfar * 1.00001f;
@@ -210,8 +210,8 @@ __device__ __forceinline__ bool visit_node<4>(const void* nodes, const RayW& r, 
 }
 
 template <>
-__device__ __forceinline__ bool visit_node<8>(const void* nodes, const RayW& r, float tmax, TraverseStack& st,
-                                              uint32_t& ref)
+__device__ __forceinline__ bool visit_node<8>(const void* nodes, const RayW& r, float tmin, float tmax,
+                                              TraverseStack& st, uint32_t& ref)
 {
     const uint4* np = reinterpret_cast<const uint4*>(static_cast<const Node8*>(nodes) + ref);
     const uint4  w0 = np[0], q0 = np[1], q1 = np[2], q2 = np[3], c0 = np[4], c1 = np[5];
@@ -238,7 +238,7 @@ __device__ __forceinline__ bool visit_node<8>(const void* nodes, const RayW& r, 
         const float t0x = (lx - r.o.x) * r.inv.x, t1x = (hx - r.o.x) * r.inv.x;
         const float t0y = (ly - r.o.y) * r.inv.y, t1y = (hy - r.o.y) * r.inv.y;
         const float t0z = (lz - r.o.z) * r.inv.z, t1z = (hz - r.o.z) * r.inv.z;
-        float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.tmin));
+        float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
         float tfar  = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
         tnear       = tnear * 0.99999f;
         tfar        = tfar * 1.00001f;
@@ -281,14 +281,16 @@ __device__ __forceinline__ bool visit_node<8>(const void* nodes, const RayW& r, 
  * during traversal is held while the lane keeps visiting inner nodes of
  * the same tree, and its triangles are tested one per step alongside, so
  * a lane makes progress on both kinds of work in one pass of the wave
- * (a second leaf, an instance boundary or the marker wait until the held
- * leaf is done).  It reports when the ray is finished, so a lane whose ray
+ * (a second leaf, an instance entry or the return to the top tree wait
+ * until the held leaf is done).  It reports when the ray is finished, so a lane whose ray
  * ends can start its next ray at once.  The closest hit does not depend on
  * the visiting order (ties are broken by (t, instance, primitive)).
  * ANY = occlusion: stop at the first hit (D15: optix.cu:106 traces shadow
- * rays with closest-hit flags).  The world ray is (wo, wd); inside an
- * instance the object-space ray replaces it until the marker pushed on entry
- * is popped. */
+ * rays with closest-hit flags).  Inside an instance the object-space ray
+ * replaces the world ray.  The stack depth at instance entry marks where the
+ * instance's entries begin: popping an entry below it leaves the instance,
+ * and the world ray is restored as soon as no leaf of the instance is held
+ * (a held leaf's triangles still need the object-space ray). */
 template <bool ANY, int W>
 struct Tracer
 {
@@ -298,14 +300,15 @@ struct Tracer
     HitRec   best;
     uint32_t ref;
     uint32_t leaf_next, leaf_left;
+    int      inst_sp; /* stack depth at instance entry, 0 in the top tree, -1 once an
+                         entry of the top tree was popped inside an instance */
     bool     has_ref;
     bool     found;
 
-    __device__ __forceinline__ void start(const SceneView& sc, TraverseStack& st, f3 o, f3 d, float t0, float t1)
+    __device__ __forceinline__ void start(const SceneView& sc, TraverseStack& st, f3 o, f3 d, float t1)
     {
-        rw.o    = o;
-        rw.d    = d;
-        rw.tmin = t0;
+        rw.o = o;
+        rw.d = d;
         ray_setup(rw);
         r         = rw;
         cur_inst  = 0xffffffffu;
@@ -320,23 +323,27 @@ struct Tracer
         has_ref   = true;
         leaf_next = 0;
         leaf_left = 0;
+        inst_sp   = 0;
     }
 
-    /* Returns true when the ray is finished. */
-    __device__ __forceinline__ bool step(const SceneView& sc, TraverseStack& st, uint32_t& nnodes, uint32_t& ntris)
+    /* Returns true when the ray is finished; hits need t >= tmin. */
+    __device__ __forceinline__ bool step(const SceneView& sc, TraverseStack& st, float tmin, uint32_t& nnodes,
+                                         uint32_t& ntris)
     {
         /* node unit */
         if (has_ref)
         {
             const bool held = leaf_left > 0u;
-            if (ref == CLPT_STACK_MARKER)
+            if (inst_sp < 0 && !held)
             {
-                if (!held)
-                {
-                    r        = rw;
-                    cur_inst = 0xffffffffu;
-                    has_ref  = false;
-                }
+                /* back in the top tree: the world ray again */
+                r        = rw;
+                cur_inst = 0xffffffffu;
+                inst_sp  = 0;
+            }
+            if (inst_sp < 0)
+            {
+                /* wait until the held leaf is done */
             }
             else if ((ref & (CLPT_LEAF_BIT | CLPT_INST_BIT)) == CLPT_LEAF_BIT)
             {
@@ -359,11 +366,11 @@ struct Tracer
                     for (int k = 0; k < 12; ++k)
                         m[k] = ip->inv[k];
                     ray_to_object(m, rw, r);
-                    st.push(CLPT_STACK_MARKER, -CLPT_FLT_MAX);
-                    ref = ip->blas_root;
+                    inst_sp = st.sp;
+                    ref     = ip->blas_root;
                 }
                 ++nnodes;
-                has_ref = visit_node<W>(sc.nodes, r, best.t, st, ref);
+                has_ref = visit_node<W>(sc.nodes, r, tmin, best.t, st, ref);
             }
         }
         /* triangle unit: one triangle of the held leaf */
@@ -374,7 +381,7 @@ struct Tracer
             ++leaf_next;
             --leaf_left;
             float tt, b1, b2;
-            if (tri_test(r, best.t, f3{tr.v0p.x, tr.v0p.y, tr.v0p.z}, f3{tr.v1.x, tr.v1.y, tr.v1.z},
+            if (tri_test(r, tmin, best.t, f3{tr.v0p.x, tr.v0p.y, tr.v0p.z}, f3{tr.v1.x, tr.v1.y, tr.v1.z},
                          f3{tr.v2.x, tr.v2.y, tr.v2.z}, &tt, &b1, &b2))
             {
                 if (ANY)
@@ -401,6 +408,8 @@ struct Tracer
             {
                 float t;
                 st.pop(ref, t);
+                if (st.sp < inst_sp)
+                    inst_sp = -1;
                 if (t <= best.t)
                 {
                     has_ref = true;
@@ -689,9 +698,9 @@ __global__ __launch_bounds__(256, 6) void k_trace(SceneView sc, RayQueue q, cons
                                                HitBuf hb, uint2* gstack, unsigned long long* ctr)
 {
     CLPT_DECLARE_STACK(st, gstack);
-    uint32_t       nn = 0, nt = 0;
-    const uint32_t n    = *count;
-    const float    tmin = depth == 1 ? 0.0f : CLPT_RAY_EPS;
+    unsigned long long sn = 0, stt = 0; /* the wave's node visits and triangle tests */
+    const uint32_t     n    = *count;
+    const float        tmin = depth == 1 ? 0.0f : CLPT_RAY_EPS;
     WavePool       pool{st.gid - lane_id(), st.gstride, 64u};
     uint32_t       i      = st.gid;
     bool           active = i < n;
@@ -699,18 +708,21 @@ __global__ __launch_bounds__(256, 6) void k_trace(SceneView sc, RayQueue q, cons
     if (active)
     {
         const float4 o = q.o[i], d = q.d[i];
-        tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, tmin, CLPT_FLT_MAX);
+        tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_FLT_MAX);
     }
     while (__any(active))
     {
-        bool fin = false;
-        if (active && tr.step(sc, st, nn, nt))
+        bool     fin = false;
+        uint32_t vn = 0, vt = 0;
+        if (active && tr.step(sc, st, tmin, vn, vt))
         {
             hb.h[i]    = make_float4(tr.found ? tr.best.t : -1.0f, tr.best.b1, tr.best.b2,
                                      __uint_as_float(tr.found ? tr.best.inst : 0xffffffffu));
             hb.prim[i] = tr.found ? tr.best.prim : 0xffffffffu;
             fin        = true;
         }
+        sn += (unsigned long long)__popcll(__ballot(vn != 0u));
+        stt += (unsigned long long)__popcll(__ballot(vt != 0u));
         const unsigned long long mask = __ballot(fin);
         if (mask)
         {
@@ -721,13 +733,12 @@ __global__ __launch_bounds__(256, 6) void k_trace(SceneView sc, RayQueue q, cons
                 if (active)
                 {
                     const float4 o = q.o[i], d = q.d[i];
-                    tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, tmin, CLPT_FLT_MAX);
+                    tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_FLT_MAX);
                 }
             }
             pool.next += (uint32_t)__popcll(mask);
         }
     }
-    const unsigned long long sn = wave_sum_u64(nn), stt = wave_sum_u64(nt);
     if (lane_id() == 0)
     {
         atomicAdd(ctr + 0, sn);
@@ -1089,8 +1100,8 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
                                                 float4* guide_E)
 {
     CLPT_DECLARE_STACK(st, gstack);
-    uint32_t       nn = 0, nt = 0, nvis = 0;
-    const uint32_t n      = bc->shadow[depth];
+    unsigned long long sn = 0, stt = 0, sv = 0; /* the wave's node visits, triangle tests, unoccluded rays */
+    const uint32_t     n      = bc->shadow[depth];
     WavePool       pool{st.gid - lane_id(), st.gstride, 64u};
     uint32_t       i      = st.gid;
     bool           active = i < n;
@@ -1098,19 +1109,20 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
     if (active)
     {
         const float4 o = sq.o[i], d = sq.d[i];
-        tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_RAY_EPS, CLPT_FLT_MAX);
+        tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_FLT_MAX);
     }
     while (__any(active))
     {
-        bool fin = false;
-        if (active && tr.step(sc, st, nn, nt))
+        bool     fin = false, vis = false;
+        uint32_t vn = 0, vt = 0;
+        if (active && tr.step(sc, st, CLPT_RAY_EPS, vn, vt))
         {
             if (!tr.found)
             {
                 const float4 o = sq.o[i], d = sq.d[i];
                 const float2 x = sq.x[i];
                 accum_add(accum, __float_as_uint(x.y), f3{o.w, d.w, x.x});
-                ++nvis;
+                vis = true;
                 if (sq.tag)
                 {
                     const uint32_t tg = sq.tag[i];
@@ -1120,6 +1132,9 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
             }
             fin = true;
         }
+        sn += (unsigned long long)__popcll(__ballot(vn != 0u));
+        stt += (unsigned long long)__popcll(__ballot(vt != 0u));
+        sv += (unsigned long long)__popcll(__ballot(vis));
         const unsigned long long mask = __ballot(fin);
         if (mask)
         {
@@ -1130,13 +1145,12 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
                 if (active)
                 {
                     const float4 o = sq.o[i], d = sq.d[i];
-                    tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_RAY_EPS, CLPT_FLT_MAX);
+                    tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_FLT_MAX);
                 }
             }
             pool.next += (uint32_t)__popcll(mask);
         }
     }
-    const unsigned long long sn = wave_sum_u64(nn), stt = wave_sum_u64(nt), sv = wave_sum_u64(nvis);
     if (lane_id() == 0)
     {
         atomicAdd(ctr + 2, sn);
@@ -1424,8 +1438,8 @@ __global__ __launch_bounds__(256) void k_trace_batch(SceneView sc, const float4*
         if (any)
         {
             Tracer<true, W> tr;
-            tr.start(sc, st, ro, rd, a.w, b.w);
-            while (!tr.step(sc, st, nn, nt))
+            tr.start(sc, st, ro, rd, b.w);
+            while (!tr.step(sc, st, a.w, nn, nt))
             {
             }
             out[i]  = make_float4(tr.found ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f);
@@ -1434,8 +1448,8 @@ __global__ __launch_bounds__(256) void k_trace_batch(SceneView sc, const float4*
         else
         {
             Tracer<false, W> tr;
-            tr.start(sc, st, ro, rd, a.w, b.w);
-            while (!tr.step(sc, st, nn, nt))
+            tr.start(sc, st, ro, rd, b.w);
+            while (!tr.step(sc, st, a.w, nn, nt))
             {
             }
             out[i]  = make_float4(tr.found ? tr.best.t : -1.0f, tr.found ? tr.best.b1 : 0.0f,

@@ -701,9 +701,9 @@ static void SFX(ray_setup)(SFX(ray) * r)
     r->ky = ky;
     r->kz = kz;
     const R dz = SFX(comp)(r->d, kz);
-    r->Sx      = SFX(comp)(r->d, kx) / dz;
-    r->Sy      = SFX(comp)(r->d, ky) / dz;
     r->Sz      = RC(1.0) / dz;
+    r->Sx      = SFX(comp)(r->d, kx) * r->Sz;
+    r->Sy      = SFX(comp)(r->d, ky) * r->Sz;
     const R tiny = RC(1e-20);
     r->inv = SFX(mk)(RC(1.0) / (RFABS(r->d.x) > tiny ? r->d.x : (r->d.x < RC(0.0) ? -tiny : tiny)),
                      RC(1.0) / (RFABS(r->d.y) > tiny ? r->d.y : (r->d.y < RC(0.0) ? -tiny : tiny)),
