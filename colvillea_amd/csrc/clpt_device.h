@@ -743,13 +743,16 @@ CLPT_D bool tri_test(const RayW& r, float tmin, float tmax, f3 v0, f3 v1, f3 v2,
     return true;
 }
 
-/* Object-space ray of an instance (affine, no renormalisation). */
+/* Object-space ray of an instance (affine, no renormalisation), with
+ * explicit fused multiply-adds in a fixed order (each correctly rounded,
+ * so a host fmaf reproduces them). */
 CLPT_D void ray_to_object(const float* m, const RayW& w, RayW& o)
 {
-    o.o    = f3{m[0] * w.o.x + m[1] * w.o.y + m[2] * w.o.z + m[3], m[4] * w.o.x + m[5] * w.o.y + m[6] * w.o.z + m[7],
-             m[8] * w.o.x + m[9] * w.o.y + m[10] * w.o.z + m[11]};
-    o.d    = f3{m[0] * w.d.x + m[1] * w.d.y + m[2] * w.d.z, m[4] * w.d.x + m[5] * w.d.y + m[6] * w.d.z,
-             m[8] * w.d.x + m[9] * w.d.y + m[10] * w.d.z};
+    o.o = f3{fmaf(m[0], w.o.x, fmaf(m[1], w.o.y, fmaf(m[2], w.o.z, m[3]))),
+             fmaf(m[4], w.o.x, fmaf(m[5], w.o.y, fmaf(m[6], w.o.z, m[7]))),
+             fmaf(m[8], w.o.x, fmaf(m[9], w.o.y, fmaf(m[10], w.o.z, m[11])))};
+    o.d = f3{fmaf(m[0], w.d.x, fmaf(m[1], w.d.y, m[2] * w.d.z)), fmaf(m[4], w.d.x, fmaf(m[5], w.d.y, m[6] * w.d.z)),
+             fmaf(m[8], w.d.x, fmaf(m[9], w.d.y, m[10] * w.d.z))};
     ray_setup(o);
 }
 

@@ -19,6 +19,7 @@ static inline void orc_sincos64(double x, double* s, double* c)
 #define RFMIN(a, b) fmin((a), (b))
 #define RFMAX(a, b) fmax((a), (b))
 #define RFLOOR(x) floor(x)
+#define RFMA(a, b, c) fma((a), (b), (c))
 #define ORC_PI 3.14159265358979323846
 #define ORC_INV_PI 0.318309886183790671538
 #define ORC_IS_F32 0

@@ -11,7 +11,7 @@
  * The instantiating file defines:
  *   R, SFX(x), RC(lit)      real type, name suffixer, literal of type R
  *   RSINCOS(x,s,c), RATAN2(y,x), RACOS(x)
- *   RSQRT, RFABS, RFMIN, RFMAX, RFLOOR
+ *   RSQRT, RFABS, RFMIN, RFMAX, RFLOOR, RFMA (fused multiply-add)
  *   ORC_PI, ORC_INV_PI      the reference's M_PIf / M_1_PIf in type R
  *   ORC_IS_F32              1 for the float32 variant
  */
@@ -789,12 +789,12 @@ static void SFX(to_object)(const orc_scene* s, uint32_t inst, const SFX(ray) * w
 #else
     const double* m = s->invd[inst];
 #endif
-    o->o = SFX(mk)((R)m[0] * w->o.x + (R)m[1] * w->o.y + (R)m[2] * w->o.z + (R)m[3],
-                   (R)m[4] * w->o.x + (R)m[5] * w->o.y + (R)m[6] * w->o.z + (R)m[7],
-                   (R)m[8] * w->o.x + (R)m[9] * w->o.y + (R)m[10] * w->o.z + (R)m[11]);
-    o->d = SFX(mk)((R)m[0] * w->d.x + (R)m[1] * w->d.y + (R)m[2] * w->d.z,
-                   (R)m[4] * w->d.x + (R)m[5] * w->d.y + (R)m[6] * w->d.z,
-                   (R)m[8] * w->d.x + (R)m[9] * w->d.y + (R)m[10] * w->d.z);
+    o->o = SFX(mk)(RFMA((R)m[0], w->o.x, RFMA((R)m[1], w->o.y, RFMA((R)m[2], w->o.z, (R)m[3]))),
+                   RFMA((R)m[4], w->o.x, RFMA((R)m[5], w->o.y, RFMA((R)m[6], w->o.z, (R)m[7]))),
+                   RFMA((R)m[8], w->o.x, RFMA((R)m[9], w->o.y, RFMA((R)m[10], w->o.z, (R)m[11]))));
+    o->d = SFX(mk)(RFMA((R)m[0], w->d.x, RFMA((R)m[1], w->d.y, (R)m[2] * w->d.z)),
+                   RFMA((R)m[4], w->d.x, RFMA((R)m[5], w->d.y, (R)m[6] * w->d.z)),
+                   RFMA((R)m[8], w->d.x, RFMA((R)m[9], w->d.y, (R)m[10] * w->d.z)));
     o->tmin = w->tmin;
     o->tmax = w->tmax;
     SFX(ray_setup)(o);
