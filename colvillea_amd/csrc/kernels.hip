@@ -754,8 +754,37 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
     const bool     dome   = sc.dome_index >= 0;
     const float    nl     = (float)sc.nemitters;
     const BatchRec b      = bt.batches[*bt.cursor];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    __shared__ uint32_t order[256]; /* blockDim.x <= 256 (launch bounds) */
+    __shared__ uint32_t ncls[8];
+    /* The block takes blockDim.x consecutive rays at a time and hands them to its
+     * threads grouped by class (escaped, or the hit material's type), so a
+     * wave mostly runs one BSDF's code instead of every type present among
+     * its lanes.  Results do not depend on the order (the RNG is keyed by
+     * path and sample); only float64 film sums and queue slots are taken in
+     * another order. */
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base = n - base > stride ? base + stride : n)
     {
+        const uint32_t i0  = base + threadIdx.x;
+        uint32_t       cls = 7u; /* past the end */
+        if (i0 < n)
+        {
+            const float4 hh = hb.h[i0];
+            cls             = hh.x >= 0.0f ? 1u + sc.mtl[sc.inst[__float_as_uint(hh.w)].material].type : 0u;
+        }
+        if (threadIdx.x < 8u)
+            ncls[threadIdx.x] = 0u;
+        __syncthreads();
+        const uint32_t slot = atomicAdd(&ncls[cls], 1u);
+        __syncthreads();
+        uint32_t first = 0u;
+        for (uint32_t c = 0; c < cls; ++c)
+            first += ncls[c];
+        order[first + slot] = i0;
+        __syncthreads();
+        const uint32_t i = order[threadIdx.x];
+        __syncthreads();
+        if (i >= n)
+            continue;
         const float4   o4 = qin.o[i], d4 = qin.d[i], T4 = qin.T[i];
         const float4   h4 = hb.h[i];
         const uint32_t path = __float_as_uint(d4.w);
