@@ -935,10 +935,35 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
     const BatchRec b      = bt.batches[*bt.cursor];
     const uint32_t tstr   = guide_tstride(gv, b.npaths);
     const float    cb     = gv.c_bsdf;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride)
+    __shared__ uint32_t order[256]; /* blockDim.x <= 256 (launch bounds) */
+    __shared__ uint32_t ncls[8];
+    /* rays handed to threads grouped by class, as in k_shade: a wave then
+     * mostly holds one material, and waves without a guided vertex skip
+     * the network */
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base = n - base > stride ? base + stride : n)
     {
-        const uint32_t i      = base + lane_id();
-        const bool     active = i < n;
+        {
+            const uint32_t i0  = base + threadIdx.x;
+            uint32_t       cls = 7u; /* past the end */
+            if (i0 < n)
+            {
+                const float4 hh = hb.h[i0];
+                cls             = hh.x >= 0.0f ? 1u + sc.mtl[sc.inst[__float_as_uint(hh.w)].material].type : 0u;
+            }
+            if (threadIdx.x < 8u)
+                ncls[threadIdx.x] = 0u;
+            __syncthreads();
+            const uint32_t slot = atomicAdd(&ncls[cls], 1u);
+            __syncthreads();
+            uint32_t first = 0u;
+            for (uint32_t c = 0; c < cls; ++c)
+                first += ncls[c];
+            order[first + slot] = i0;
+            __syncthreads();
+        }
+        const uint32_t i = order[threadIdx.x];
+        __syncthreads();
+        const bool active = i < n;
         float4         o4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d4 = o4, T4 = o4, h4 = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
         uint32_t       prim = 0u;
         if (active)
