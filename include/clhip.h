@@ -120,7 +120,7 @@ typedef struct
     uint32_t rr_depth;         /* Russian roulette from this vertex depth (0 = off) */
     uint32_t view;             /* RNG view lane: z = (view << 20) | sample index */
     uint32_t samples_per_batch;/* samples of every pixel in flight together (0 = as many whole samples
-                                  as fit the launch's paths_per_batch, default 2^25 paths) */
+                                  as fit the launch's paths_per_batch, default 2^27 paths) */
 } clhip_params;
 
 /* Exact counters of the last clhip_render (sum over its samples).  Index b
@@ -285,7 +285,7 @@ typedef struct
     uint32_t trace_blocks_per_cu;  /* persistent closest-hit grid (256-thread blocks per CU) */
     uint32_t shadow_blocks_per_cu; /* persistent any-hit grid */
     uint32_t shade_blocks_per_cu;  /* camera and shading grids */
-    uint32_t paths_per_batch;      /* paths in flight per batch (0 = 2^25; clhip_render_views raises it to the
+    uint32_t paths_per_batch;      /* paths in flight per batch (0 = 2^27; clhip_render_views raises it to the
                                       largest view; clhip_render uses whole samples of the film) */
     uint32_t graph;                /* 1: replay a captured hipGraph per batch; 2: plain launches */
     uint32_t order;                /* clhip_render_views: 1 = given order, 2 = least work first */
