@@ -14,7 +14,7 @@
  *   *_f32 : float32, -ffp-contract=off, the renderer's own elementary
  *           functions (colvillea_amd/csrc/clpt_elem.h) -> bit-comparable to
  *           the GPU for identical inputs;
- *   *_f64 : float64 with libm -> the image-level judge (relMSE).
+ *   *_f64 : float64 with libm -> the image-level reference (relMSE).
  *
  * The reference itself cannot be compiled here (it needs OWL's vec.h and
  * CUDA headers that the image lacks), so this restatement is pinned by

@@ -1,5 +1,5 @@
 /* orc_f64.c -- float64 instantiation of the oracle core (libm elementary
- * functions): the image-level judge.  TEST INFRASTRUCTURE ONLY (see oracle.h). */
+ * functions): the image-level reference.  TEST INFRASTRUCTURE ONLY (see oracle.h). */
 #include <math.h>
 
 static inline void orc_sincos64(double x, double* s, double* c)

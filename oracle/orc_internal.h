@@ -29,7 +29,7 @@ struct orc_scene
     orc_bvh*       blas; /* one per mesh */
     orc_bvh        tlas; /* over instance world boxes */
     float (*inv)[12];    /* world->object, float (the renderer's values) */
-    double (*invd)[12];  /* world->object, float64 (the f64 judge) */
+    double (*invd)[12];  /* world->object, float64 (the f64 reference) */
     float* sun_tan;      /* per emitter: tan(angular radius) as float */
 };
 
