@@ -280,8 +280,8 @@ def main():
     ap.add_argument("--autotune-log", default=None, help="write the autotuner's trials here (JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is described in "
-              "DESIGN.md and not run.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); sharding over GPUs (sample indices "
+              "split across GPUs, then one float64 film reduction) is not run.", file=sys.stderr)
         sys.exit(2)
 
     from colvillea_amd import Integrator, IntegratorType, scenes, upload_scene

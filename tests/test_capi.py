@@ -74,7 +74,7 @@ def test_create_without_device_fails_loudly():
     """On a machine without a HIP device, context creation reports it."""
     ctx = C.c_void_p()
     rc = capi.lib().clhip_create(8, 8, 2, C.byref(ctx))
-    if rc == 0:  # a GPU is present (CPU suite run on a GPU box)
+    if rc == 0:  # a GPU is present (CPU suite run on a machine with a GPU)
         capi.lib().clhip_destroy(ctx)
         pytest.skip("a HIP device is present")
     assert rc == -2
