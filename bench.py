@@ -157,7 +157,7 @@ def batched_views(integ, args) -> dict:
 
 
 GUIDE_TUNED = {"bsdf_fraction": 0.5, "train": 1, "record_paths": 1 << 20, "batch_size": 32768,
-               "learning_rate": 3e-3, "steps_per_batch": 8, "max_steps": 1024, "warm_spp": 8,
+               "learning_rate": 3e-3, "steps_per_batch": 8, "max_steps": 512, "warm_spp": 8,
                "warm_steps_per_batch": 32}
 GUIDE_SEEDS = (1, 2, 3)
 GUIDE_SPEC_LR = dict(GUIDE_TUNED, learning_rate=1e-3)
@@ -168,7 +168,7 @@ def guided_room(args) -> dict:
     the closed room lit through a window (1920x1080, depth 10).  The guided
     schedule: train on a discarded 8-spp warm-up (32 Adam steps of 2^15
     samples per batch) and then on the first batches (8 steps per batch),
-    2^20 recorded paths per batch, frozen after 1024 steps.  The ratio is
+    2^20 recorded paths per batch, frozen after 512 steps.  The ratio is
     the mean over three network initialisations.  Also the same schedule at
     the configuration's nominal learning rate 1e-3 (one initialisation)."""
     from colvillea_amd import Integrator, IntegratorType, guided_eval, scenes, upload_scene
