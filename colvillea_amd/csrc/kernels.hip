@@ -677,7 +677,8 @@ __global__ __launch_bounds__(256) void k_camera(BatchTable bt, GuideView gv, Ray
         camera_ray(v, pixel % v.width, pixel / v.width, sample, &o, &d);
         q.o[i] = make_float4(o.x, o.y, o.z, 0.0f);
         q.d[i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));
-        q.T[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        /* the throughput plane is not written: a camera path's throughput is
+         * 1, and shading at depth 1 takes it as such */
         if (guide_records(i, tstr, *bt.cursor))
         {
             const uint32_t slot = i / tstr;
@@ -785,7 +786,8 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
         __syncthreads();
         if (i >= n)
             continue;
-        const float4   o4 = qin.o[i], d4 = qin.d[i], T4 = qin.T[i];
+        const float4   o4 = qin.o[i], d4 = qin.d[i];
+        const float4   T4 = depth == 1u ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : qin.T[i];
         const float4   h4 = hb.h[i];
         const uint32_t path = __float_as_uint(d4.w);
         uint32_t       vi, vpix, sample;
@@ -970,7 +972,7 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
         {
             o4   = qin.o[i];
             d4   = qin.d[i];
-            T4   = qin.T[i];
+            T4   = depth == 1u ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : qin.T[i];
             h4   = hb.h[i];
             prim = hb.prim[i];
         }
