@@ -190,9 +190,10 @@ struct BatchTable
 /* A ray queue: 3 float4 planes + count (device). */
 struct RayQueue
 {
-    float4*   o;  /* origin.xyz, previous BSDF pdf */
+    float4*   o;  /* origin.xyz, w unused (read by traversal only) */
     float4*   d;  /* direction.xyz, path id (bits) */
-    float4*   T;  /* throughput.xyz, unused */
+    float4*   T;  /* throughput.xyz, previous BSDF pdf (read by shading only; not
+                     written for camera paths, whose throughput is 1 and pdf 0) */
     uint32_t* count;
 };
 

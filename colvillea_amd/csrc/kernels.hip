@@ -786,7 +786,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
         __syncthreads();
         if (i >= n)
             continue;
-        const float4   o4 = qin.o[i], d4 = qin.d[i];
+        const float4   d4 = qin.d[i];
+        /* throughput and, in w, the previous BSDF pdf; the origin plane is
+         * traversal's only */
         const float4   T4 = depth == 1u ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : qin.T[i];
         const float4   h4 = hb.h[i];
         const uint32_t path = __float_as_uint(d4.w);
@@ -813,7 +815,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
                 else
                 {
                     const float lp = dome_pdf(sc, dir) / nl;
-                    const float w  = o4.w / (o4.w + lp);
+                    const float w  = T4.w / (T4.w + lp);
                     accum_add(accum, pixel, mul3(T, scl3(le, w)));
                 }
             }
@@ -896,9 +898,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
         const uint32_t ns = wave_push(&bc->rays[depth + 1], want_next);
         if (want_next)
         {
-            qout.o[ns] = make_float4(p.x, p.y, p.z, npdf);
+            qout.o[ns] = make_float4(p.x, p.y, p.z, 0.0f);
             qout.d[ns] = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(path));
-            qout.T[ns] = make_float4(T.x, T.y, T.z, 0.0f);
+            qout.T[ns] = make_float4(T.x, T.y, T.z, npdf);
         }
     }
 }
@@ -966,11 +968,10 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
         const uint32_t i = order[threadIdx.x];
         __syncthreads();
         const bool active = i < n;
-        float4         o4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d4 = o4, T4 = o4, h4 = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
+        float4         d4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), T4 = d4, h4 = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
         uint32_t       prim = 0u;
         if (active)
         {
-            o4   = qin.o[i];
             d4   = qin.d[i];
             T4   = depth == 1u ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : qin.T[i];
             h4   = hb.h[i];
@@ -1039,7 +1040,7 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
                 else
                 {
                     const float lp = dome_pdf(sc, dir) / nl;
-                    const float w  = o4.w / (o4.w + lp);
+                    const float w  = T4.w / (T4.w + lp);
                     cc             = mul3(T, scl3(le, w));
                 }
                 accum_add(accum, pixel, cc);
@@ -1143,9 +1144,9 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
         const uint32_t ns = wave_push(&bc->rays[depth + 1], want_next);
         if (want_next)
         {
-            qout.o[ns] = make_float4(si.p.x, si.p.y, si.p.z, npdf);
+            qout.o[ns] = make_float4(si.p.x, si.p.y, si.p.z, 0.0f);
             qout.d[ns] = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(path));
-            qout.T[ns] = make_float4(T.x, T.y, T.z, 0.0f);
+            qout.T[ns] = make_float4(T.x, T.y, T.z, npdf);
         }
     }
 }
