@@ -31,7 +31,7 @@ def default_space(integ: Integrator) -> Dict[str, List[int]]:
     t = d["trace_blocks_per_cu"]
     return {
         "trace_blocks_per_cu": sorted({max(1, t), max(1, t - 1), max(1, t - 2), max(1, t // 2)}, reverse=True),
-        "shadow_blocks_per_cu": [d["shadow_blocks_per_cu"], 2, 4, 6],
+        "shadow_blocks_per_cu": sorted({x for x in (max(1, d["shadow_blocks_per_cu"]), 3, 4, 6) if x <= max(1, d["shadow_blocks_per_cu"])}, reverse=True),
         "shade_blocks_per_cu": [d["shade_blocks_per_cu"], 4, 16],
         "paths_per_batch": [d["paths_per_batch"], 16 << 20, 32 << 20, 64 << 20],
         "order": [1, 2],

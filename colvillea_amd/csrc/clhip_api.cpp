@@ -975,9 +975,10 @@ int clhip_create(uint32_t width, uint32_t height, uint32_t integrator_type, clhi
     {
         /* persistent grids: every lane refills its own rays, so at most the
          * blocks that fit on the chip at once.  Measured on the 1080p scene
-         * (MI355X): closest hit is fastest at full residency, any hit at 3
-         * blocks/CU (fewer lanes, more rays per lane, less cache pressure).
-         * clhip_set_launch and the environment variables override. */
+         * (MI355X, 64 spp in one batch): both traversals are fastest at full
+         * residency; any hit at 2, 3, 4 and 6 blocks/CU gives 909, 977, 1015
+         * and 1039 Msamples/s.  clhip_set_launch and the environment
+         * variables override. */
         auto env_u = [](const char* name) {
             const char* v = std::getenv(name);
             return v ? (uint32_t)std::atoi(v) : 0u;
@@ -985,7 +986,7 @@ int clhip_create(uint32_t width, uint32_t height, uint32_t integrator_type, clhi
         const uint32_t rt = clpt::resident_blocks_per_cu(0), rs = clpt::resident_blocks_per_cu(1);
         const uint32_t ot = env_u("CLPT_TRACE_BLOCKS_PER_CU"), os = env_u("CLPT_SHADOW_BLOCKS_PER_CU");
         c->def_trace  = ot ? ot : std::max(1u, rt);
-        c->def_shadow = os ? os : std::max(1u, std::min(rs, 3u));
+        c->def_shadow = os ? os : std::max(1u, rs);
         c->blocks_guided = c->num_cus * std::max(1u, clpt::resident_blocks_per_cu(3));
         apply_launch(c.get());
     }
