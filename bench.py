@@ -18,6 +18,8 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+# MI355X L2: 8 XCDs x 16 channels x 128 B per clock at 2.4 GHz
+L2_PEAK_GBS = 8 * 16 * 128 * 2.4
 QUEUE_BYTES_PER_RAY = 32 + 16 + 4  # ray o/d float4 x2 read, hit float4 + prim write
 NODE_BYTES = {4: 128, 8: 96}  # Node4: float child boxes; Node8: byte-quantised child boxes
 TRI_BYTES = 48
@@ -36,6 +38,9 @@ def cpu_baseline(desc, seconds_target: float, threads: int):
     import numpy as np
     import orc
 
+    t = time.perf_counter()
+    orc.use_native()  # built here, for this host's CPU: -O3 -march=native
+    log(f"cpu baseline: oracle built for this host in {time.perf_counter() - t:.1f} s")
     sc = orc.Scene(desc)
     w, h = desc.width, desc.height
 
@@ -56,17 +61,32 @@ def cpu_baseline(desc, seconds_target: float, threads: int):
     _, cnt = sc.render_rows(rows, "f32", samples=(0, k), nthreads=threads)
     dt = time.perf_counter() - t
     samples = n * w * k
-    model = None
-    try:  # the host CPU the baseline ran on
-        with open("/proc/cpuinfo") as fh:
-            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
-    except OSError:
-        pass
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "cpu_model": model, "host_threads": os.cpu_count(),
+            "host": host_cpu(), "build": "gcc -O3 -march=native -fno-fast-math -ffp-contract=off",
             "sample": f"{n} of {h} rows spread evenly over the {w}x{h} frame x {k} spp ({samples} camera paths, "
-                      f"{int(cnt['rays'].sum())} closest-hit rays), depth {desc.max_depth}, float32 oracle, "
-                      f"{threads} threads, {dt:.1f} s"}
+                      f"{int(cnt['rays'].sum())} closest-hit rays), depth {desc.max_depth}, float32 oracle "
+                      f"(BVH2 traversal), {threads} threads, {dt:.1f} s"}
+
+
+def host_cpu() -> dict:
+    """The host CPU the baseline ran on: lscpu's model, sockets, cores per
+    socket, threads per core and logical CPUs, and the CPUs this process may
+    run on."""
+    import subprocess
+
+    out = {"cpus_available": len(os.sched_getaffinity(0))}
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "CPU(s)": "logical_cpus"}
+    try:
+        text = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        text = ""
+    for line in text.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keys and keys[k.strip()] not in out:
+            v = v.strip()
+            out[keys[k.strip()]] = int(v) if v.isdigit() else v
+    return out
 
 
 def other_width(desc, args) -> dict:
@@ -333,23 +353,36 @@ def main():
     value = samples / t_total / 1e6
     batched = None if args.no_c2 else batched_views(integ, args)
     other = None if args.no_other_width else other_width(desc, args)
-    algo_bytes = rays_traced * QUEUE_BYTES_PER_RAY + nodes * NODE_BYTES[args.bvh_width] + tris * TRI_BYTES
-    achieved = algo_bytes / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
-    per_launch = algo_bytes / max(1, launches)
-    # HBM bytes per k_trace launch from the FETCH_SIZE / WRITE_SIZE counter
-    # runs (scaled by the copy kernel), committed with their summary
-    traffic = None
-    tpath = os.path.join(REPO, "profiles", "k_trace_c1_traffic.json")
-    if os.path.exists(tpath) and desc.width == 1920 and desc.height == 1080 and args.bvh_width == 4:
-        with open(tpath) as fh:
-            traffic = json.load(fh)["hbm_bytes_per_launch"]
+    # HBM roofline of the closest-hit kernel: its compulsory bytes per launch
+    # -- every ray read and its hit written once, every node and triangle
+    # record of the tree fetched once -- over its event-timed launches
+    per_launch_ms = trace_ms / max(1, launches)
+    compulsory = rays_traced * QUEUE_BYTES_PER_RAY + launches * c["scene_bytes"]
+    achieved = compulsory / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
+    # the bytes the kernel requests from the cache hierarchy: each node visit
+    # and triangle test fetches its whole record (mostly L2/MALL hits)
+    requested = rays_traced * QUEUE_BYTES_PER_RAY + nodes * NODE_BYTES[args.bvh_width] + tris * TRI_BYTES
+    req_gbs = requested / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
+    # measured per-launch counters of the same kernel and workload (rocprofv3
+    # --pmc runs, summarised with their commands under profiles/)
+    meas = None
+    cpath = os.path.join(REPO, "profiles", "k_trace_c1_counters.json")
+    if os.path.exists(cpath) and desc.width == 1920 and desc.height == 1080 and args.bvh_width == 4:
+        with open(cpath) as fh:
+            meas = json.load(fh)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": "profiles/k_trace_c1_1080p.md" if traffic else None,
-                "kernel": "k_trace", "bvh_width": args.bvh_width, "avg_launch_ms": round(trace_ms / max(1, launches), 4),
-                "algorithmic_bytes_per_launch": int(per_launch),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": meas["hbm_bytes_per_launch"] if meas else None,
+                "kernel": "k_trace", "bvh_width": args.bvh_width, "avg_launch_ms": round(per_launch_ms, 4),
+                "algorithmic_bytes_per_launch": int(compulsory / max(1, launches)),
+                "algorithmic_bytes": f"{QUEUE_BYTES_PER_RAY} B per closest-hit ray (ray read, hit written) + the "
+                                     f"tree's node and triangle records once per launch ({c['scene_bytes']} B)",
                 "nodes_per_ray": round(nodes / max(1, rays_traced), 2),
-                "tris_per_ray": round(tris / max(1, rays_traced), 2)}
+                "tris_per_ray": round(tris / max(1, rays_traced), 2),
+                "l2": {"requested_bytes_per_launch": int(requested / max(1, launches)), "achieved": round(req_gbs, 1),
+                       "peak": L2_PEAK_GBS, "unit": "GB/s", "frac": round(req_gbs / L2_PEAK_GBS, 4),
+                       "measured": meas.get("l2") if meas else None},
+                "limiter": meas.get("limiter") if meas else None,
+                "counters_source": "profiles/k_trace_c1_1080p.md" if meas else None}
     import ctypes as C
 
     from colvillea_amd import _capi as capi
@@ -359,7 +392,6 @@ def main():
     capi.check(capi.lib().clhip_copy_probe(integ.ctx, copy_bytes, args.copy_iters, C.byref(copy_ms)), integ.ctx)
     copy_gbs = 2 * copy_bytes / (copy_ms.value * 1e-3) / 1e9
     roofline["copy_ceiling_GBs"] = round(copy_gbs, 1)
-    roofline["frac_of_copy_ceiling"] = round(achieved / copy_gbs, 4)
     log(f"copy probe: {args.copy_mib} MiB in {copy_ms.value:.3f} ms -> {copy_gbs:.0f} GB/s (read + write)")
     integ.close()
     integ = None

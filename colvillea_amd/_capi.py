@@ -96,6 +96,8 @@ class clhip_stats(C.Structure):
         ("batches", C.c_uint64),
         ("graph_replays", C.c_uint64),
         ("graph_captures", C.c_uint64),
+        ("queue_paths", C.c_uint64),
+        ("scene_bytes", C.c_uint64),
     ]
 
 
@@ -164,6 +166,7 @@ SIGNATURES = {
     "clhip_destroy": (C.c_int, [P]),
     "clhip_last_error": (C.c_char_p, [P]),
     "clhip_create_texture": (C.c_int, [P, f32p, u32, u32, u32, C.POINTER(i32)]),
+    "clhip_create_texture_u8": (C.c_int, [P, C.POINTER(C.c_uint8), u32, u32, u32, C.c_int, C.POINTER(i32)]),
     "clhip_set_bvh_width": (C.c_int, [P, u32]),
     "clhip_build_blas": (C.c_int, [P, C.POINTER(clhip_mesh), u32]),
     "clhip_build_tlas": (C.c_int, [P, u32p, u32p, f32p, u32]),
@@ -178,6 +181,8 @@ SIGNATURES = {
     "clhip_reset": (C.c_int, [P]),
     "clhip_get_film_size": (C.c_int, [P, u32p, u32p, u32p]),
     "clhip_readback": (C.c_int, [P, f32p, C.c_size_t]),
+    "clhip_readback_srgb": (C.c_int, [P, f32p, C.c_size_t]),
+    "clhip_srgb_batch": (C.c_int, [P, f32p, u32, C.c_int, f32p]),
     "clhip_readback_sums": (C.c_int, [P, C.POINTER(C.c_double), C.c_size_t]),
     "clhip_counters": (C.c_int, [P, C.POINTER(clhip_stats)]),
     "clhip_set_timing": (C.c_int, [P, C.c_int]),
@@ -213,6 +218,8 @@ SIGNATURES = {
     "clhip_guide_infer": (C.c_int, [P, f32p, u32, f32p]),
     "clhip_guide_fp8_scales": (C.c_int, [P, f32p]),
     "clhip_guide_train": (C.c_int, [P, f32p, u32, f32p, f32p]),
+    "clhip_guide_mixture_batch": (C.c_int, [P, f32p, f32p, f32p, u32, f32p, f32p, f32p]),
+    "clhip_guide_query_batch": (C.c_int, [P, f32p, f32p, f32p, u32, f32p, f32p]),
     "clhip_guide_records": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), u32,
                                        C.POINTER(C.c_uint32)]),
 }

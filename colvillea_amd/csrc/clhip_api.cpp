@@ -117,6 +117,7 @@ struct clhip_ctx
     std::vector<MeshRec>     mesh_recs;
     bool                     have_blas = false, have_tlas = false;
     DevBuf                   d_nodes, d_tris, d_meshes, d_pos, d_nrm, d_tan, d_uv, d_idx, d_inst;
+    size_t                   node_bytes = 0, tri_bytes = 0; /* in use in d_nodes, d_tris */
     std::vector<InstanceRec> inst;
     std::vector<uint32_t>    inst_entity;
     uint32_t                 tlas_root = 0, tlas_depth = 0;
@@ -1040,6 +1041,24 @@ int clhip_destroy(clhip_ctx* ctx)
     return CLHIP_OK;
 }
 
+/* Appends a texture and re-uploads the device table of all textures, which
+ * material lookups index. */
+static int add_texture(clhip_ctx* ctx, TexHost&& t, int32_t* texture_id)
+{
+    ctx->textures.push_back(std::move(t));
+    *texture_id = (int32_t)ctx->textures.size() - 1;
+    std::vector<TexRec> recs(ctx->textures.size());
+    for (size_t i = 0; i < recs.size(); ++i)
+    {
+        recs[i].data = ctx->textures[i].buf->as<const float4>();
+        recs[i].w    = ctx->textures[i].w;
+        recs[i].h    = ctx->textures[i].h;
+    }
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream)); /* the old table may be in use */
+    CLHIP_CHECK(ctx, upload(ctx->d_texrec, recs));
+    return CLHIP_OK;
+}
+
 int clhip_create_texture(clhip_ctx* ctx, const float* data, uint32_t width, uint32_t height, uint32_t channels,
                          int32_t* texture_id)
 {
@@ -1059,19 +1078,36 @@ int clhip_create_texture(clhip_ctx* ctx, const float* data, uint32_t width, uint
     t.w   = width;
     t.h   = height;
     CLHIP_CHECK(ctx, upload(*t.buf, rgba));
-    ctx->textures.push_back(std::move(t));
-    *texture_id = (int32_t)ctx->textures.size() - 1;
-    /* the device table of all textures, for material lookups */
-    std::vector<TexRec> recs(ctx->textures.size());
-    for (size_t i = 0; i < recs.size(); ++i)
+    return add_texture(ctx, std::move(t), texture_id);
+}
+
+int clhip_create_texture_u8(clhip_ctx* ctx, const uint8_t* data, uint32_t width, uint32_t height, uint32_t channels,
+                            int srgb, int32_t* texture_id)
+{
+    if (!ctx || !data || !texture_id || width == 0 || height == 0 || (channels != 3 && channels != 4))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_create_texture_u8: bad arguments");
+    if ((uint64_t)width * height > (1ull << 30))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_create_texture_u8: %ux%u texels, more than 2^30", width, height);
+    const size_t         n = (size_t)width * height;
+    std::vector<uint8_t> rgba(4 * n);
+    for (size_t i = 0; i < n; ++i)
     {
-        recs[i].data = ctx->textures[i].buf->as<const float4>();
-        recs[i].w    = ctx->textures[i].w;
-        recs[i].h    = ctx->textures[i].h;
+        rgba[4 * i]     = data[channels * i];
+        rgba[4 * i + 1] = data[channels * i + 1];
+        rgba[4 * i + 2] = data[channels * i + 2];
+        rgba[4 * i + 3] = channels == 4 ? data[4 * i + 3] : 255u;
     }
-    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream)); /* the old table may be in use */
-    CLHIP_CHECK(ctx, upload(ctx->d_texrec, recs));
-    return CLHIP_OK;
+    Tmp raw;
+    CLHIP_CHECK(ctx, raw.from_host(rgba.data(), rgba.size()));
+    TexHost t;
+    t.buf = std::make_unique<DevBuf>();
+    t.w   = width;
+    t.h   = height;
+    CLHIP_CHECK(ctx, t.buf->alloc(n * sizeof(float4)));
+    CLHIP_CHECK(ctx, clpt::launch_texture_u8(ctx->stream, raw.b.as<const uchar4>(), (uint32_t)n, srgb ? 1 : 0,
+                                             t.buf->as<float4>()));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return add_texture(ctx, std::move(t), texture_id);
 }
 
 int clhip_set_bvh_width(clhip_ctx* ctx, uint32_t width)
@@ -1193,6 +1229,7 @@ int clhip_build_blas(clhip_ctx* ctx, const clhip_mesh* meshes, uint32_t num_mesh
         idx.insert(idx.end(), me.indices, me.indices + 3 * (size_t)nt);
     }
     CLHIP_CHECK(ctx, upload(ctx->d_tris, tris));
+    ctx->tri_bytes = tris.size() * sizeof(TriRec);
     CLHIP_CHECK(ctx, upload(ctx->d_meshes, recs));
     CLHIP_CHECK(ctx, upload(ctx->d_pos, pos));
     CLHIP_CHECK(ctx, upload(ctx->d_nrm, nrm));
@@ -1290,6 +1327,7 @@ int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uin
         CLHIP_CHECK(ctx, upload(ctx->d_nodes, nodes8));
     else
         CLHIP_CHECK(ctx, upload(ctx->d_nodes, nodes4));
+    ctx->node_bytes = ctx->bvh_width == 8 ? nodes8.size() * sizeof(Node8) : nodes4.size() * sizeof(Node4);
     ctx->inst        = std::move(inst);
     ctx->inst_entity = std::move(ent);
     ctx->tlas_root   = r.root;
@@ -1326,7 +1364,7 @@ int clhip_build_materials(clhip_ctx* ctx, const clhip_material* materials, uint3
             r.k[k]   = m.k[k];
         }
         const float rough = m.roughness;
-        r.alpha           = std::fmax(rough * rough, 1e-4f); /* metal.h:131, microfacet.h:27 */
+        r.alpha           = std::fmax(rough * rough, 1e-4f); /* metal.h:52, microfacet.h:27 */
         r.ior             = m.interior_ior;
         const uint32_t nt = (uint32_t)ctx->textures.size();
         if (m.reflectance_tex > nt || m.roughness_tex > nt || m.normal_tex > nt)
@@ -1677,7 +1715,12 @@ int clhip_render_views(clhip_ctx* ctx, const clhip_view* views, uint32_t nviews)
         }
     }
 
-    rc = ensure_render_buffers(ctx, cap);
+    /* queues for the largest batch actually built, not for `cap`: a few
+     * small views need only their own paths */
+    uint32_t most = 1;
+    for (const BatchRec& b : batches)
+        most = std::max(most, b.npaths);
+    rc = ensure_render_buffers(ctx, most);
     if (rc)
         return rc;
     if (3 * (size_t)film * sizeof(double) > ctx->d_vaccum.bytes)
@@ -1722,7 +1765,7 @@ int clhip_view_readback(clhip_ctx* ctx, uint32_t view, float* rgba, size_t bytes
     DevBuf out;
     CLHIP_CHECK(ctx, out.alloc((size_t)v.npix * 16));
     CLHIP_CHECK(ctx, clpt::launch_resolve(ctx->stream, ctx->d_vaccum.as<double>() + 3 * (size_t)v.film_offset, v.npix,
-                                          ctx->vspp[view], out.as<float4>()));
+                                          ctx->vspp[view], 0, out.as<float4>()));
     CLHIP_CHECK(ctx, hipMemcpyAsync(rgba, out.p, (size_t)v.npix * 16, hipMemcpyDeviceToHost, ctx->stream));
     CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return CLHIP_OK;
@@ -1754,24 +1797,47 @@ int clhip_sync(clhip_ctx* ctx)
     return CLHIP_OK;
 }
 
-int clhip_readback(clhip_ctx* ctx, float* rgba, size_t bytes)
+static int readback_film(clhip_ctx* ctx, float* rgba, size_t bytes, int srgb, const char* who)
 {
     if (!ctx || !rgba)
-        return fail(ctx, CLHIP_E_INVALID, "clhip_readback: bad arguments");
+        return fail(ctx, CLHIP_E_INVALID, "%s: bad arguments", who);
     const size_t npix = (size_t)ctx->width * ctx->height;
     if (bytes < npix * 4 * sizeof(float))
-        return fail(ctx, CLHIP_E_INVALID, "clhip_readback: buffer of %zu bytes, need %zu", bytes, npix * 16);
+        return fail(ctx, CLHIP_E_INVALID, "%s: buffer of %zu bytes, need %zu", who, bytes, npix * 16);
     if (!ctx->d_accum.p)
     {
+        /* nothing rendered: black, which both encodings map to 0 */
         std::memset(rgba, 0, npix * 16);
         for (size_t i = 0; i < npix; ++i)
             rgba[4 * i + 3] = 1.0f;
         return CLHIP_OK;
     }
-    CLHIP_CHECK(ctx, clpt::launch_resolve(ctx->stream, ctx->d_accum.as<double>(), (uint32_t)npix, ctx->spp,
+    CLHIP_CHECK(ctx, clpt::launch_resolve(ctx->stream, ctx->d_accum.as<double>(), (uint32_t)npix, ctx->spp, srgb,
                                           ctx->d_out.as<float4>()));
     CLHIP_CHECK(ctx, hipMemcpyAsync(rgba, ctx->d_out.p, npix * 16, hipMemcpyDeviceToHost, ctx->stream));
     CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return CLHIP_OK;
+}
+
+int clhip_readback(clhip_ctx* ctx, float* rgba, size_t bytes) { return readback_film(ctx, rgba, bytes, 0, "clhip_readback"); }
+
+int clhip_readback_srgb(clhip_ctx* ctx, float* rgba, size_t bytes)
+{
+    return readback_film(ctx, rgba, bytes, 1, "clhip_readback_srgb");
+}
+
+int clhip_srgb_batch(clhip_ctx* ctx, const float* in, uint32_t n, int decode, float* out)
+{
+    if (!ctx || (n && (!in || !out)))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_srgb_batch: bad arguments");
+    if (n == 0)
+        return CLHIP_OK;
+    Tmp a, b;
+    CLHIP_CHECK(ctx, a.from_host(in, n));
+    CLHIP_CHECK(ctx, b.zeros((size_t)n * sizeof(float)));
+    CLHIP_CHECK(ctx, clpt::launch_srgb_batch(ctx->stream, a.b.as<const float>(), n, decode ? 1 : 0, b.b.as<float>()));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, b.to_host(out, n));
     return CLHIP_OK;
 }
 
@@ -1821,6 +1887,8 @@ int clhip_counters(clhip_ctx* ctx, clhip_stats* out)
     out->batches         = ctx->nbatches;
     out->graph_replays   = ctx->graph_replays;
     out->graph_captures  = ctx->graph_captures;
+    out->queue_paths     = ctx->qcap;
+    out->scene_bytes     = ctx->node_bytes + ctx->tri_bytes;
     for (auto& tl : ctx->timed)
     {
         float ms = 0.0f;
@@ -2042,7 +2110,7 @@ int clhip_copy_probe(clhip_ctx* ctx, size_t bytes, uint32_t iters, double* ms_pe
     hipEvent_t e0, e1;
     CLHIP_CHECK(ctx, hipEventCreate(&e0));
     CLHIP_CHECK(ctx, hipEventCreate(&e1));
-    const uint32_t blocks = ctx->num_cus * 16;
+    const uint32_t blocks = ctx->num_cus * 8;
     CLHIP_CHECK(ctx, clpt::launch_copy_probe(ctx->stream, a.as<const float4>(), b.as<float4>(), n, blocks));
     CLHIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
     for (uint32_t i = 0; i < iters; ++i)
@@ -2253,6 +2321,63 @@ int clhip_guide_infer(clhip_ctx* ctx, const float* q9, uint32_t n, float* out40)
                                               blocks));
     CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     CLHIP_CHECK(ctx, o.to_host(out40, (size_t)n * CLPT_G_NOUT));
+    return CLHIP_OK;
+}
+
+int clhip_guide_mixture_batch(clhip_ctx* ctx, const float* out40, const float* dir3, const float* u3, uint32_t n,
+                              float* pdf, float* smp_dir3, float* smp_pdf)
+{
+    if (!ctx || (n && (!out40 || !dir3 || !u3 || !pdf || !smp_dir3 || !smp_pdf)))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_guide_mixture_batch: bad arguments");
+    if (n == 0)
+        return CLHIP_OK;
+    Tmp o, d, u, p, sd, sp;
+    CLHIP_CHECK(ctx, o.from_host(out40, (size_t)n * CLPT_G_NOUT));
+    CLHIP_CHECK(ctx, d.from_host(dir3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, u.from_host(u3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, p.zeros((size_t)n * sizeof(float)));
+    CLHIP_CHECK(ctx, sd.zeros(3 * (size_t)n * sizeof(float)));
+    CLHIP_CHECK(ctx, sp.zeros((size_t)n * sizeof(float)));
+    CLHIP_CHECK(ctx, clpt::launch_guide_mixture_batch(ctx->stream, o.b.as<const float>(), d.b.as<const float>(),
+                                                      u.b.as<const float>(), n, p.b.as<float>(), sd.b.as<float>(),
+                                                      sp.b.as<float>()));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, p.to_host(pdf, n));
+    CLHIP_CHECK(ctx, sd.to_host(smp_dir3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, sp.to_host(smp_pdf, n));
+    return CLHIP_OK;
+}
+
+int clhip_guide_query_batch(clhip_ctx* ctx, const float* p3, const float* n3, const float* wo3, uint32_t n, float* q9,
+                            float box6[6])
+{
+    if (!ctx || (n && (!p3 || !n3 || !wo3 || !q9)))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_guide_query_batch: bad arguments");
+    if (!ctx->have_tlas)
+        return fail(ctx, CLHIP_E_STATE, "no TLAS: the query box is the scene's bounds");
+    float lo[3], inv[3];
+    for (int k = 0; k < 3; ++k)
+    {
+        const float ext = ctx->world_hi[k] - ctx->world_lo[k];
+        lo[k]           = ctx->world_lo[k];
+        inv[k]          = ext > 1e-20f ? 1.0f / ext : 1.0f;
+        if (box6)
+        {
+            box6[k]     = ctx->world_lo[k];
+            box6[3 + k] = ctx->world_hi[k];
+        }
+    }
+    if (n == 0)
+        return CLHIP_OK;
+    Tmp a, b, c, q;
+    CLHIP_CHECK(ctx, a.from_host(p3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, b.from_host(n3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, c.from_host(wo3, 3 * (size_t)n));
+    CLHIP_CHECK(ctx, q.zeros(9 * (size_t)n * sizeof(float)));
+    CLHIP_CHECK(ctx, clpt::launch_guide_query_batch(ctx->stream, lo, inv, a.b.as<const float>(), b.b.as<const float>(),
+                                                    c.b.as<const float>(), n, q.b.as<float>()));
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CLHIP_CHECK(ctx, q.to_host(q9, 9 * (size_t)n));
     return CLHIP_OK;
 }
 

@@ -38,6 +38,25 @@ CLPT_D f3   cross3(f3 a, f3 b) { return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.
 CLPT_D f3   norm3(f3 v) { return scl3(v, 1.0f / sqrtf(dot3(v, v))); }
 CLPT_D bool nonzero3(f3 v) { return v.x > 0.0f || v.y > 0.0f || v.z > 0.0f; }
 
+/* ------------------------------------------------------------ display encoding */
+
+/* convertFromLinearTosRGB (libkernel/base/math.h:114-122), per channel: no
+ * clamp, so HDR values above 1 keep rising; the exponent is the reference's
+ * 0.41666 (not 1/2.4).  powf is the device library's. */
+CLPT_D float srgb_encode(float x) { return x < 0.0031308f ? x * 12.92f : 1.055f * powf(x, 0.41666f) - 0.055f; }
+
+/* convertsRGBToLinear (math.h:131-140): clamped to [0, 1]. */
+CLPT_D float srgb_decode(float s)
+{
+    if (s <= 0.0f)
+        return 0.0f;
+    if (s >= 1.0f)
+        return 1.0f;
+    if (s <= 0.04045f)
+        return s / 12.92f;
+    return powf((s + 0.055f) / 1.055f, 2.4f);
+}
+
 /* ------------------------------------------------------------ RNG */
 
 /* pcg4d: libkernel/base/sampler.h:16-36 */
@@ -80,7 +99,7 @@ CLPT_D float rng_draw(uint32_t px, uint32_t py, uint32_t z, uint32_t k) { return
 
 /* ------------------------------------------------------------ warps */
 
-/* warp::squareToUniformConcentricDisk: warp.h:110-134 (phi in double). */
+/* warp::squareToUniformConcentricDisk: warp.h:25-50 (phi in double, from M_PI). */
 CLPT_D void concentric_disk(float ux, float uy, float* ox, float* oy)
 {
     const float r1 = 2.0f * ux - 1.0f;
@@ -106,7 +125,7 @@ CLPT_D void concentric_disk(float ux, float uy, float* ox, float* oy)
     *oy = r * s;
 }
 
-/* warp::squareToCosineHemisphere: warp.h:143-149 (radicand clamped at 0). */
+/* warp::squareToCosineHemisphere: warp.h:58-64 (radicand clamped at 0). */
 CLPT_D f3 cosine_hemisphere(float ux, float uy)
 {
     float px, py;
@@ -170,7 +189,7 @@ CLPT_D float ggx_D(float alpha, f3 m)
     return 1.0f / (CLPT_PI_F * alpha * alpha * root * root);
 }
 
-/* Frame::sinTheta: frame.h:411-415 */
+/* Frame::sinTheta: frame.h:80-84 */
 CLPT_D float sin_theta(f3 v)
 {
     const float t = 1.0f - v.z * v.z;
@@ -745,7 +764,7 @@ CLPT_D bool tri_test(const RayW& r, float tmin, float tmax, f3 v0, f3 v1, f3 v2,
 
 /* Object-space ray of an instance (affine, no renormalisation), with
  * explicit fused multiply-adds in a fixed order (each correctly rounded,
- * so a host fmaf reproduces them). */
+ * so a host fmaf reproduces them).  The caller runs ray_setup on it. */
 CLPT_D void ray_to_object(const float* m, const RayW& w, RayW& o)
 {
     o.o = f3{fmaf(m[0], w.o.x, fmaf(m[1], w.o.y, fmaf(m[2], w.o.z, m[3]))),
@@ -753,7 +772,6 @@ CLPT_D void ray_to_object(const float* m, const RayW& w, RayW& o)
              fmaf(m[8], w.o.x, fmaf(m[9], w.o.y, fmaf(m[10], w.o.z, m[11])))};
     o.d = f3{fmaf(m[0], w.d.x, fmaf(m[1], w.d.y, m[2] * w.d.z)), fmaf(m[4], w.d.x, fmaf(m[5], w.d.y, m[6] * w.d.z)),
              fmaf(m[8], w.d.x, fmaf(m[9], w.d.y, m[10] * w.d.z))};
-    ray_setup(o);
 }
 
 /* Total order of hits: t, then instance, then primitive. */

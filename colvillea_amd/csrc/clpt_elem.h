@@ -7,7 +7,7 @@
  * The renderer needs sin/cos (GGX azimuth, concentric disk, HDRI direction
  * from uv), atan2/acos (HDRI uv from direction) and, for the learned
  * proposal, exp/log.  The reference calls CUDA fast-math versions of these
- * (include/colvillea/libkernel/base/warp.h:131,
+ * (include/colvillea/libkernel/base/warp.h:46,
  * include/colvillea/libkernel/emitters/hdridome.h:210-211,452,474-475);
  * those carry no reproducible bit pattern, so the renderer defines its own.
  *

@@ -21,7 +21,9 @@ hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& 
                          BatchCounters* bc, uint2* gstack, unsigned long long* ctr, float4* guide_E, uint32_t blocks);
 hipError_t launch_batch_end(hipStream_t s, const BatchCounters* bc, TotalCounters* tot, uint32_t* cursor);
 hipError_t launch_set_cursor(hipStream_t s, uint32_t* cursor, uint32_t value);
-hipError_t launch_resolve(hipStream_t s, const double* accum, uint32_t npix, uint32_t spp, float4* out);
+hipError_t launch_resolve(hipStream_t s, const double* accum, uint32_t npix, uint32_t spp, int srgb, float4* out);
+hipError_t launch_srgb_batch(hipStream_t s, const float* in, uint32_t n, int decode, float* out);
+hipError_t launch_texture_u8(hipStream_t s, const uchar4* in, uint32_t n, int srgb, float4* out);
 hipError_t launch_dome_tables(hipStream_t s, const float4* tex, uint32_t w, uint32_t h, float* pcond, float* cdfcond,
                               float* pv, float* cdfv, double* rowint, uint32_t* cutv, uint32_t* cutcond);
 hipError_t launch_rng_batch(hipStream_t s, const uint4* seeds, uint32_t n, uint32_t ndraws, uint32_t* ou, float* of);
@@ -63,6 +65,10 @@ struct GuideTrainBufs
     float*         scl;
 };
 /* bf16 inference, or FP8 when img8 is not NULL (scales scl) */
+hipError_t launch_guide_mixture_batch(hipStream_t s, const float* out40, const float* dir3, const float* u3, uint32_t n,
+                                      float* pdf, float* smp3, float* smp_pdf);
+hipError_t launch_guide_query_batch(hipStream_t s, const float lo[3], const float inv_ext[3], const float* p3,
+                                    const float* n3, const float* wo3, uint32_t n, float* q9);
 hipError_t launch_guide_infer(hipStream_t s, const __bf16* img, const uint8_t* img8, const float* scl, const float* q9,
                               uint32_t n, float* out40, uint32_t blocks);
 hipError_t launch_guide_pack_fp8(hipStream_t s, const float* params, const GuideCounters* gc, uint8_t* img8,

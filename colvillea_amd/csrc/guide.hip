@@ -69,6 +69,43 @@ __global__ __launch_bounds__(256) void k_guide_infer(const void* img, const floa
     }
 }
 
+/* The mixture of the learned proposal on given network outputs (parity
+ * surface of the functions k_shade_guided uses): density at dir3, a sample
+ * from u3 = (lobe, u1, u2) and the density at that sample. */
+__global__ __launch_bounds__(256) void k_guide_mixture_batch(const float* out40, const float* dir3, const float* u3,
+                                                             uint32_t n, float* pdf, float* smp3, float* smp_pdf)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        float o[CLPT_G_NOUT];
+#pragma unroll
+        for (int k = 0; k < CLPT_G_NOUT; ++k)
+            o[k] = out40[CLPT_G_NOUT * (size_t)i + k];
+        VmfMix m;
+        vmf_mixture(o, m);
+        pdf[i]      = vmf_mixture_pdf(m, f3{dir3[3 * (size_t)i], dir3[3 * (size_t)i + 1], dir3[3 * (size_t)i + 2]});
+        const f3 w  = vmf_mixture_sample(m, u3[3 * (size_t)i], u3[3 * (size_t)i + 1], u3[3 * (size_t)i + 2]);
+        smp3[3 * (size_t)i]     = w.x;
+        smp3[3 * (size_t)i + 1] = w.y;
+        smp3[3 * (size_t)i + 2] = w.z;
+        smp_pdf[i]              = vmf_mixture_pdf(m, w);
+    }
+}
+
+/* guide_query for given surface points (position, shading normal, wo). */
+__global__ __launch_bounds__(256) void k_guide_query_batch(GuideBox box, const float* p3, const float* n3,
+                                                           const float* wo3, uint32_t n, float* q9)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        float q[9];
+        guide_query(box, ld3(p3 + 3 * (size_t)i), ld3(n3 + 3 * (size_t)i), ld3(wo3 + 3 * (size_t)i), q);
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            q9[9 * (size_t)i + k] = q[k];
+    }
+}
+
 /* ------------------------------------------------------------ training */
 
 /* Also the step's bookkeeping: per-block sums of the sample weights (fixed
@@ -644,6 +681,28 @@ hipError_t launch_guide_infer(hipStream_t s, const __bf16* img, const uint8_t* i
         hipLaunchKernelGGL(k_guide_infer<true>, dim3(blocks), dim3(256), 0, s, (const void*)img8, scl, q9, n, out40);
     else
         hipLaunchKernelGGL(k_guide_infer<false>, dim3(blocks), dim3(256), 0, s, (const void*)img, scl, q9, n, out40);
+    return hipGetLastError();
+}
+
+hipError_t launch_guide_mixture_batch(hipStream_t s, const float* out40, const float* dir3, const float* u3, uint32_t n,
+                                      float* pdf, float* smp3, float* smp_pdf)
+{
+    const uint32_t blocks = std::max(1u, std::min(4096u, (n + 255u) / 256u));
+    hipLaunchKernelGGL(k_guide_mixture_batch, dim3(blocks), dim3(256), 0, s, out40, dir3, u3, n, pdf, smp3, smp_pdf);
+    return hipGetLastError();
+}
+
+hipError_t launch_guide_query_batch(hipStream_t s, const float lo[3], const float inv_ext[3], const float* p3,
+                                    const float* n3, const float* wo3, uint32_t n, float* q9)
+{
+    GuideBox box;
+    for (int k = 0; k < 3; ++k)
+    {
+        box.lo[k]      = lo[k];
+        box.inv_ext[k] = inv_ext[k];
+    }
+    const uint32_t blocks = std::max(1u, std::min(4096u, (n + 255u) / 256u));
+    hipLaunchKernelGGL(k_guide_query_batch, dim3(blocks), dim3(256), 0, s, box, p3, n3, wo3, n, q9);
     return hipGetLastError();
 }
 

@@ -290,27 +290,32 @@ __device__ __forceinline__ bool visit_node<8>(const void* nodes, const RayW& r, 
  * replaces the world ray.  The stack depth at instance entry marks where the
  * instance's entries begin: popping an entry below it leaves the instance,
  * and the world ray is restored as soon as no leaf of the instance is held
- * (a held leaf's triangles still need the object-space ray). */
+ * (a held leaf's triangles still need the object-space ray).
+ *
+ * A new ray's set-up (reciprocals and the watertight shear) is not done
+ * when the ray is taken but at its first step, in the same block as an
+ * instance's object-space ray: both are rare per lane and few lanes of a
+ * wave need either in one pass, so the wave issues the shared set-up once
+ * per pass instead of once for each kind. */
 template <bool ANY, int W>
 struct Tracer
 {
-    RayW     rw; /* the world ray, set up once */
+    RayW     rw; /* the world ray */
     RayW     r;  /* the ray of the current tree (world, or an instance's object space) */
     uint32_t cur_inst;
     HitRec   best;
     uint32_t ref;
     uint32_t leaf_next, leaf_left;
     int      inst_sp; /* stack depth at instance entry, 0 in the top tree, -1 once an
-                         entry of the top tree was popped inside an instance */
+                         entry of the top tree was popped inside an instance, -2 until
+                         the world ray is set up */
     bool     has_ref;
     bool     found;
 
     __device__ __forceinline__ void start(const SceneView& sc, TraverseStack& st, f3 o, f3 d, float t1)
     {
-        rw.o = o;
-        rw.d = d;
-        ray_setup(rw);
-        r         = rw;
+        rw.o      = o;
+        rw.d      = d;
         cur_inst  = 0xffffffffu;
         best.t    = t1;
         best.b1   = 0.0f;
@@ -323,7 +328,7 @@ struct Tracer
         has_ref   = true;
         leaf_next = 0;
         leaf_left = 0;
-        inst_sp   = 0;
+        inst_sp   = -2;
     }
 
     /* Returns true when the ray is finished; hits need t >= tmin. */
@@ -333,13 +338,42 @@ struct Tracer
         /* node unit */
         if (has_ref)
         {
-            const bool held = leaf_left > 0u;
-            if (inst_sp < 0 && !held)
+            const bool held  = leaf_left > 0u;
+            const bool enter = inst_sp >= -1 && !held && (ref & (CLPT_LEAF_BIT | CLPT_INST_BIT)) ==
+                                                             (CLPT_LEAF_BIT | CLPT_INST_BIT);
+            if (inst_sp == -1 && !held && !enter)
             {
                 /* back in the top tree: the world ray again */
                 r        = rw;
                 cur_inst = 0xffffffffu;
                 inst_sp  = 0;
+            }
+            if (enter || inst_sp == -2)
+            {
+                if (enter)
+                {
+                    /* instance leaf: continue in the instance's BLAS */
+                    cur_inst              = ref & ~(CLPT_LEAF_BIT | CLPT_INST_BIT);
+                    const InstanceRec* ip = sc.inst + cur_inst;
+                    float              m[12];
+#pragma unroll
+                    for (int k = 0; k < 12; ++k)
+                        m[k] = ip->inv[k];
+                    ray_to_object(m, rw, r);
+                    inst_sp = st.sp;
+                    ref     = ip->blas_root;
+                }
+                else
+                {
+                    r.o = rw.o;
+                    r.d = rw.d;
+                }
+                ray_setup(r);
+                if (inst_sp == -2)
+                {
+                    rw      = r;
+                    inst_sp = 0;
+                }
             }
             if (inst_sp < 0)
             {
@@ -354,24 +388,13 @@ struct Tracer
                     has_ref   = false;
                 }
             }
-            else if ((ref & CLPT_LEAF_BIT) == 0u || !held)
+            else if ((ref & CLPT_LEAF_BIT) == 0u)
             {
-                if (ref & CLPT_LEAF_BIT)
-                {
-                    /* instance leaf: continue in the instance's BLAS */
-                    cur_inst              = ref & ~(CLPT_LEAF_BIT | CLPT_INST_BIT);
-                    const InstanceRec* ip = sc.inst + cur_inst;
-                    float              m[12];
-#pragma unroll
-                    for (int k = 0; k < 12; ++k)
-                        m[k] = ip->inv[k];
-                    ray_to_object(m, rw, r);
-                    inst_sp = st.sp;
-                    ref     = ip->blas_root;
-                }
                 ++nnodes;
                 has_ref = visit_node<W>(sc.nodes, r, tmin, best.t, st, ref);
             }
+            /* else an instance leaf waiting for the held leaf (or, at a
+             * ray's first step, for the next pass) */
         }
         /* triangle unit: one triangle of the held leaf */
         if (leaf_left > 0u)
@@ -1239,13 +1262,50 @@ __global__ void k_set_cursor(uint32_t* cursor, uint32_t value)
         *cursor = value;
 }
 
-__global__ void k_resolve(const double* accum, uint32_t npix, uint32_t spp, float4* out)
+/* sums / spp per pixel; with srgb the display copy is encoded as the
+ * reference's postprocessing kernel does (raygeneration.cu:498-506), while
+ * the float64 film stays linear. */
+__global__ void k_resolve(const double* accum, uint32_t npix, uint32_t spp, int srgb, float4* out)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x)
     {
         const double inv = spp ? 1.0 / (double)spp : 0.0;
-        out[i] = make_float4((float)(accum[3 * (size_t)i] * inv), (float)(accum[3 * (size_t)i + 1] * inv),
-                             (float)(accum[3 * (size_t)i + 2] * inv), 1.0f);
+        float        r = (float)(accum[3 * (size_t)i] * inv), g = (float)(accum[3 * (size_t)i + 1] * inv),
+              b = (float)(accum[3 * (size_t)i + 2] * inv);
+        if (srgb)
+        {
+            r = srgb_encode(r);
+            g = srgb_encode(g);
+            b = srgb_encode(b);
+        }
+        out[i] = make_float4(r, g, b, 1.0f);
+    }
+}
+
+/* The display functions on n values (parity surface). */
+__global__ void k_srgb_batch(const float* in, uint32_t n, int decode, float* out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = decode ? srgb_decode(in[i]) : srgb_encode(in[i]);
+}
+
+/* An 8-bit texture as the texture unit reads it with
+ * cudaReadModeNormalizedFloat and, for sRGB images, sRGB decoding of the
+ * colour channels (imagetex2d.cpp:50-66): c / 255, then convertsRGBToLinear
+ * before any filtering; alpha is not decoded. */
+__global__ void k_texture_u8(const uchar4* in, uint32_t n, int srgb, float4* out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uchar4 c = in[i];
+        float        r = (float)c.x / 255.0f, g = (float)c.y / 255.0f, b = (float)c.z / 255.0f;
+        if (srgb)
+        {
+            r = srgb_decode(r);
+            g = srgb_decode(g);
+            b = srgb_decode(b);
+        }
+        out[i] = make_float4(r, g, b, (float)c.w / 255.0f);
     }
 }
 
@@ -1516,11 +1576,22 @@ __global__ __launch_bounds__(256) void k_trace_batch(SceneView sc, const float4*
     }
 }
 
-/* Plain streaming copy (16-byte loads and stores): the reference kernel for
- * HBM byte counters and the measured copy ceiling. */
+/* Streaming copy (16-byte loads and stores, four loads in flight per lane
+ * before their stores): the known-traffic kernel for HBM byte counters and
+ * the measured copy ceiling. */
 __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t       i      = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride)
+    {
+        const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i]              = a;
+        dst[i + stride]     = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride)
         dst[i] = src[i];
 }
 
@@ -1598,9 +1669,21 @@ hipError_t launch_set_cursor(hipStream_t s, uint32_t* cursor, uint32_t value)
     return hipGetLastError();
 }
 
-hipError_t launch_resolve(hipStream_t s, const double* accum, uint32_t npix, uint32_t spp, float4* out)
+hipError_t launch_resolve(hipStream_t s, const double* accum, uint32_t npix, uint32_t spp, int srgb, float4* out)
 {
-    hipLaunchKernelGGL(k_resolve, dim3(grid_for(npix, 256, 4096)), dim3(256), 0, s, accum, npix, spp, out);
+    hipLaunchKernelGGL(k_resolve, dim3(grid_for(npix, 256, 4096)), dim3(256), 0, s, accum, npix, spp, srgb, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_srgb_batch(hipStream_t s, const float* in, uint32_t n, int decode, float* out)
+{
+    hipLaunchKernelGGL(k_srgb_batch, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, in, n, decode, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_texture_u8(hipStream_t s, const uchar4* in, uint32_t n, int srgb, float4* out)
+{
+    hipLaunchKernelGGL(k_texture_u8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, in, n, srgb, out);
     return hipGetLastError();
 }
 

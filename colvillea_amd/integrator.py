@@ -96,8 +96,12 @@ class Entity:
 
 @dataclass
 class ImageTexture2D:
-    """RGB(A) float32 image, (h, w, c); row 0 is v = 0."""
+    """RGB(A) image, (h, w, c); row 0 is v = 0.  float32 data is used as
+    is (RGBA32F); uint8 data is read as c / 255 (RGBAU8), and isSRGB marks
+    an 8-bit sRGB image whose colour channels are decoded to linear
+    (imagetex2d.cpp:13-15, 50-66)."""
     image: np.ndarray
+    isSRGB: bool = False
 
 
 @dataclass
@@ -260,7 +264,8 @@ class Integrator:
         self._check(self._lib.clhip_build_entities(self._ctx, capi.uptr(idx), len(idx)))
 
     def _texture(self, tex: ImageTexture2D) -> int:
-        img = np.asarray(tex.image, np.float32)
+        u8 = np.asarray(tex.image).dtype == np.uint8
+        img = np.asarray(tex.image, np.uint8 if u8 else np.float32)
         if img.ndim == 2:  # one channel (e.g. a roughness map): replicated to rgb
             img = img[..., None]
         if img.shape[2] == 1:
@@ -268,7 +273,13 @@ class Integrator:
         img = np.ascontiguousarray(img)
         h, w, ch = img.shape
         tid = C.c_int32()
-        self._check(self._lib.clhip_create_texture(self._ctx, capi.fptr(img), w, h, ch, C.byref(tid)))
+        if u8:
+            self._check(self._lib.clhip_create_texture_u8(self._ctx, img.ctypes.data_as(C.POINTER(C.c_uint8)), w, h,
+                                                          ch, int(bool(tex.isSRGB)), C.byref(tid)))
+        else:
+            if tex.isSRGB:
+                raise ValueError("sRGB decoding applies to 8-bit images only (float images are linear)")
+            self._check(self._lib.clhip_create_texture(self._ctx, capi.fptr(img), w, h, ch, C.byref(tid)))
         return tid.value
 
     def buildEmitters(self, emitters, domeEmitter=None, domeEmitterTextureResolution=None):
@@ -326,6 +337,16 @@ class Integrator:
         return s.value
 
     def readbackFramebuffer(self) -> np.ndarray:
+        """(h, w, 4) float32 as the reference returns it: sum / spp encoded by
+        convertFromLinearTosRGB (math.h:114-122, raygeneration.cu:498-506),
+        alpha 1.  The film itself stays linear."""
+        w, h = self.getFilmSize()
+        out = np.empty((h, w, 4), np.float32)
+        self._check(self._lib.clhip_readback_srgb(self._ctx, capi.fptr(out), out.nbytes))
+        return out
+
+    def readbackFramebufferLinear(self) -> np.ndarray:
+        """(h, w, 4) float32 linear HDR: sum / spp, alpha 1."""
         w, h = self.getFilmSize()
         out = np.empty((h, w, 4), np.float32)
         self._check(self._lib.clhip_readback(self._ctx, capi.fptr(out), out.nbytes))
@@ -342,7 +363,8 @@ class Integrator:
         self._check(self._lib.clhip_counters(self._ctx, C.byref(st)))
         d = {k: np.array(getattr(st, k)[:], np.uint64) for k in ("rays", "hits", "shadow", "visible")}
         for k in ("trace_nodes", "trace_tris", "shadow_nodes", "shadow_tris", "trace_launches", "shadow_launches",
-                  "trace_ms", "shade_ms", "shadow_ms", "render_ms", "batches", "graph_replays", "graph_captures"):
+                  "trace_ms", "shade_ms", "shadow_ms", "render_ms", "batches", "graph_replays", "graph_captures",
+                  "queue_paths", "scene_bytes"):
             d[k] = getattr(st, k)
         return d
 

@@ -71,8 +71,8 @@ enum
 };
 
 /* kernel::Material payloads: DiffuseMtl (materials/diffusemtl.h:13-17),
- * MetalMtl (metal.h:93-113), GlassMtl (glass.h:53-56).  GGX alpha is
- * roughness^2 as in the reference (metal.h:131, glass.h:66).
+ * MetalMtl (metal.h:60-65), GlassMtl (glass.h:34-37).  GGX alpha is
+ * roughness^2 as in the reference (metal.h:52, glass.h:26).
  * Textures (clhip_create_texture ids, stored as id + 1 so that 0 = none)
  * replace the constant where set, as Material::getBSDF does
  * (diffusemtl.h:22-31, metal.h:36-55): reflectance_tex for the diffuse
@@ -102,7 +102,7 @@ enum
     CLHIP_EMITTER_HDRIDOME    = 1
 };
 
-/* DirectionalEmitter (emitters/directional.h:225-228); HDRIDome takes a
+/* DirectionalEmitter (emitters/directional.h:70-73); HDRIDome takes a
  * texture created with clhip_create_texture (hdridome.h:163-175). */
 typedef struct
 {
@@ -145,6 +145,8 @@ typedef struct
     uint64_t batches;                      /* queue batches of the last render */
     uint64_t graph_replays;                /* of which replayed from a captured hipGraph */
     uint64_t graph_captures;               /* hipGraph captures made by the last render */
+    uint64_t queue_paths;                  /* paths the context's queues hold (the largest batch so far) */
+    uint64_t scene_bytes;                  /* bytes of the tree's node and triangle records (TLAS and BLASes) */
 } clhip_stats;
 
 /* ---- context: Integrator::createIntegrator (integrator.cpp:10-22) and
@@ -157,6 +159,15 @@ const char* clhip_last_error(const clhip_ctx* ctx); /* ctx may be NULL: last glo
  * RGB or RGBA float32 data, stored as RGBA32F, wrap + bilinear. */
 int clhip_create_texture(clhip_ctx* ctx, const float* data, uint32_t width, uint32_t height, uint32_t channels,
                          int32_t* texture_id);
+
+/* core::ImageTexture2D ctor for ImageTextureChannelFormat::RGBAU8 images
+ * (imagetex2d.cpp:13-15, 50-66): 8-bit RGB or RGBA read as c / 255
+ * (cudaReadModeNormalizedFloat); srgb != 0 marks an sRGB image
+ * (Image::getIsSRGB, texDesc.sRGB at :65), whose colour channels are
+ * decoded to linear per texel, before bilinear filtering, with
+ * convertsRGBToLinear (libkernel/base/math.h:131-140).  Stored as RGBA32F. */
+int clhip_create_texture_u8(clhip_ctx* ctx, const uint8_t* data, uint32_t width, uint32_t height, uint32_t channels,
+                            int srgb, int32_t* texture_id);
 
 /* Tree width of the acceleration structures built from now on: 4 (default;
  * 128-byte nodes with float boxes) or 8 (96-byte nodes with child boxes
@@ -213,6 +224,13 @@ int clhip_get_film_size(const clhip_ctx* ctx, uint32_t* width, uint32_t* height,
  * linear HDR RGBA float32 = per-pixel float64 sum / spp (replaces the
  * running mean and sRGB round trip, raygeneration.cu:75-81, 498-506). */
 int clhip_readback(clhip_ctx* ctx, float* rgba, size_t bytes);
+/* Integrator::readbackFramebuffer as the reference returns it: the film
+ * resolved as above and then encoded per channel by convertFromLinearTosRGB
+ * (math.h:114-122, applied by postprocessing, raygeneration.cu:498-506),
+ * alpha 1.  Only this display copy is encoded; the float64 film stays
+ * linear, so progressive rendering continues unaffected (the reference
+ * decodes its buffer again at raygeneration.cu:80-81). */
+int clhip_readback_srgb(clhip_ctx* ctx, float* rgba, size_t bytes);
 /* The raw float64 RGB sums (3 * width * height doubles). */
 int clhip_readback_sums(clhip_ctx* ctx, double* rgb, size_t bytes);
 
@@ -247,6 +265,10 @@ int clhip_emitter_batch(clhip_ctx* ctx, const float* u2, uint32_t n, float* dir3
 
 /* HDRIDome::evalEnvironment + pdfDirect (hdridome.h:248-297). */
 int clhip_dome_eval_batch(clhip_ctx* ctx, const float* dir3, uint32_t n, float* le3, float* pdf);
+
+/* convertFromLinearTosRGB (decode = 0) or convertsRGBToLinear (decode != 0)
+ * (math.h:114-140) on n single channels. */
+int clhip_srgb_batch(clhip_ctx* ctx, const float* in, uint32_t n, int decode, float* out);
 
 /* The HDRI tables built by clhip_build_emitters: (w+1)*h, (w+1)*h, h+1, h+1. */
 int clhip_dome_tables(clhip_ctx* ctx, float* pcond, float* cdfcond, float* pv, float* cdfv, uint32_t* w, uint32_t* h);
@@ -353,6 +375,16 @@ int clhip_guide_fp8_scales(clhip_ctx* ctx, float* scales10);
 /* Network outputs (40 per query) for n queries of 9 floats, at the
  * inference precision of clhip_set_guide. */
 int clhip_guide_infer(clhip_ctx* ctx, const float* q9, uint32_t n, float* out40);
+/* Test surfaces of the functions the guided shading kernel uses.  The
+ * mixture (softmax weights, normalised means, kappa = exp(clamp(s, -4, 8)))
+ * of n rows of 40 network outputs: its density at dir3, a direction drawn
+ * with u3 = (lobe draw, u1, u2) and the density there.  And the network
+ * query of n surface points: (position - lo) / (hi - lo) per axis over the
+ * scene's bounds (box6 = lo[3], hi[3], optional), shading normal, wo. */
+int clhip_guide_mixture_batch(clhip_ctx* ctx, const float* out40, const float* dir3, const float* u3, uint32_t n,
+                              float* pdf, float* smp_dir3, float* smp_pdf);
+int clhip_guide_query_batch(clhip_ctx* ctx, const float* p3, const float* n3, const float* wo3, uint32_t n, float* q9,
+                            float box6[6]);
 /* One Adam step on n samples of 13 floats (query 9, direction 3, weight);
  * grad (32768 floats, optional) receives the summed gradient. */
 int clhip_guide_train(clhip_ctx* ctx, const float* samples13, uint32_t n, float* grad, float* loss);

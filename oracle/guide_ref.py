@@ -140,13 +140,19 @@ def mix_pdf(out, d):
 
 
 def frame(n):
-    """Orthonormal t, b around unit n (any fixed completion)."""
+    """Orthonormal t, b around unit n, completed as makeFrame
+    (libkernel/base/math.h:33-48) completes it -- c in the plane of n and
+    the larger of its x, y axes, t = c x n, b = c -- so that a sample drawn
+    here and one drawn by the renderer from the same numbers agree."""
     n = np.asarray(n, np.float64)
-    a = np.where(np.abs(n[..., :1]) > 0.9, np.array([0.0, 1.0, 0.0]), np.array([1.0, 0.0, 0.0]))
-    t = np.cross(n, a)
-    t /= np.linalg.norm(t, axis=-1, keepdims=True)
-    b = np.cross(n, t)
-    return t, b
+    x, y, z = n[..., 0], n[..., 1], n[..., 2]
+    use_x = np.abs(x) > np.abs(y)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ix = 1.0 / np.sqrt(x * x + z * z)
+        iy = 1.0 / np.sqrt(y * y + z * z)
+        zero = np.zeros_like(x)
+        c = np.where(use_x[..., None], np.stack([z * ix, zero, -x * ix], -1), np.stack([zero, z * iy, -y * iy], -1))
+    return np.cross(c, n), c
 
 
 def mix_sample(out, usel, u1, u2):

@@ -162,6 +162,8 @@ void       orc_scene_destroy(orc_scene* s);
     void orc_fresnel_batch_##SFX(const float* cos_i, uint32_t n, const float* eta3, const float* k3, float ior,       \
                                  float* fc3, float* fd, float* cos_t);                                                \
     void orc_tex_eval_##SFX(const float* rgba, uint32_t w, uint32_t h, const float* uv, uint32_t n, float* out4);    \
+    void orc_srgb_##SFX(const float* in, uint32_t n, int decode, float* out);                                        \
+    void orc_pull_up_##SFX(const float* frame9, const float* ng3, const float* wo3, uint32_t n, float* out9);        \
     void orc_emitter_sample_##SFX(const orc_scene* s, const float* u2, uint32_t n, float* dir3, float* value3,        \
                                   float* pdf, uint32_t* which);                                                       \
     void orc_dome_eval_##SFX(const orc_scene* s, const float* dir3, uint32_t n, float* le3, float* pdf);             \

@@ -72,6 +72,19 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def use_native():
+    """Build (for this host's CPU: -O3 -march=native, the same
+    floating-point flags) and use oracle/_build/liborc_native.so instead of
+    the portable build.  Must precede the first oracle call in the process."""
+    global LIB_PATH
+    path = os.path.join(HERE, "_build", "liborc_native.so")
+    if _LIB is not None and LIB_PATH != path:
+        raise RuntimeError("the portable oracle library is already loaded")
+    subprocess.run(["make", "-s", "-C", HERE, "native"], check=True)
+    LIB_PATH = path
+    return path
+
+
 _LIB = None
 
 
@@ -103,6 +116,8 @@ def lib():
             g("ggx_batch").argtypes = [C.c_float, f32p, f32p, f32p, C.c_uint32, f32p, f32p, f32p, f32p]
             g("fresnel_batch").argtypes = [f32p, C.c_uint32, f32p, f32p, C.c_float, f32p, f32p, f32p]
             g("tex_eval").argtypes = [f32p, C.c_uint32, C.c_uint32, f32p, C.c_uint32, f32p]
+            g("srgb").argtypes = [f32p, C.c_uint32, C.c_int, f32p]
+            g("pull_up").argtypes = [f32p, f32p, f32p, C.c_uint32, f32p]
             g("emitter_sample").argtypes = [P, f32p, C.c_uint32, f32p, f32p, f32p, u32p]
             g("dome_eval").argtypes = [P, f32p, C.c_uint32, f32p, f32p]
             g("trace").argtypes = [P, f32p, C.c_uint32, C.c_int, f32p, f32p, u32p, u32p]
@@ -129,8 +144,17 @@ def _up(a):
 
 def texture_rgba(t) -> np.ndarray:
     """An ImageTexture2D as contiguous RGBA float32 (1 or 3 channels are
-    widened as the renderer widens them: grey -> rgb, alpha 1)."""
-    img = np.asarray(t.image, np.float32)
+    widened as the renderer widens them: grey -> rgb, alpha 1).  8-bit
+    images read as c / 255, sRGB ones decoded by the oracle's decoder."""
+    if np.asarray(t.image).dtype == np.uint8:
+        img = np.asarray(t.image).astype(np.float32) / np.float32(255.0)
+        if getattr(t, "isSRGB", False):
+            if img.ndim == 2:
+                img = srgb(img, decode=True)
+            else:
+                img = np.concatenate([srgb(img[..., :3], decode=True), img[..., 3:]], -1)
+    else:
+        img = np.asarray(t.image, np.float32)
     if img.ndim == 2:
         img = img[..., None]
     if img.shape[2] == 1:
@@ -425,6 +449,26 @@ def tex_eval(rgba, uv, variant="f32"):
     n = uv.shape[0]
     out = np.zeros((n, 4), np.float32)
     getattr(lib(), f"orc_tex_eval_{variant}")(_fp(rgba), rgba.shape[1], rgba.shape[0], _fp(uv), n, _fp(out))
+    return out
+
+
+def pull_up(s, t, n, ng, wo, variant="f32"):
+    """Material::fixNormalPerturbingShadingFrame (material.h:236-285) on
+    frames (s, t, n), geometric normals and wo: the fixed (s, t, n)."""
+    fr = np.ascontiguousarray(np.concatenate([s, t, n], 1), np.float32)
+    ng = np.ascontiguousarray(ng, np.float32)
+    wo = np.ascontiguousarray(wo, np.float32)
+    out = np.zeros_like(fr)
+    getattr(lib(), f"orc_pull_up_{variant}")(_fp(fr), _fp(ng), _fp(wo), len(fr), _fp(out))
+    return out[:, 0:3], out[:, 3:6], out[:, 6:9]
+
+
+def srgb(x, decode=False, variant="f32"):
+    """convertFromLinearTosRGB (decode=False) or convertsRGBToLinear
+    (math.h:114-140) per value."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    getattr(lib(), f"orc_srgb_{variant}")(_fp(x), x.size, int(bool(decode)), _fp(out))
     return out
 
 

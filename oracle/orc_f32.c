@@ -14,6 +14,7 @@
 #define RFMAX(a, b) fmaxf((a), (b))
 #define RFLOOR(x) floorf(x)
 #define RFMA(a, b, c) fmaf((a), (b), (c))
+#define RPOW(x, y) powf((x), (y)) /* libm: the display encoding only */
 #define ORC_PI 3.14159265358979323846f
 #define ORC_INV_PI 0.318309886183790671538f
 #define ORC_IS_F32 1

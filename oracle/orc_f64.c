@@ -20,6 +20,7 @@ static inline void orc_sincos64(double x, double* s, double* c)
 #define RFMAX(a, b) fmax((a), (b))
 #define RFLOOR(x) floor(x)
 #define RFMA(a, b, c) fma((a), (b), (c))
+#define RPOW(x, y) pow((x), (y))
 #define ORC_PI 3.14159265358979323846
 #define ORC_INV_PI 0.318309886183790671538
 #define ORC_IS_F32 0

@@ -11,7 +11,7 @@
  * The instantiating file defines:
  *   R, SFX(x), RC(lit)      real type, name suffixer, literal of type R
  *   RSINCOS(x,s,c), RATAN2(y,x), RACOS(x)
- *   RSQRT, RFABS, RFMIN, RFMAX, RFLOOR, RFMA (fused multiply-add)
+ *   RSQRT, RFABS, RFMIN, RFMAX, RFLOOR, RFMA (fused multiply-add), RPOW
  *   ORC_PI, ORC_INV_PI      the reference's M_PIf / M_1_PIf in type R
  *   ORC_IS_F32              1 for the float32 variant
  */
@@ -54,7 +54,7 @@ static inline int SFX(nonzero)(V3 v) { return v.x > RC(0.0) || v.y > RC(0.0) || 
 
 /* ------------------------------------------------------------ warps */
 
-/* warp::squareToUniformConcentricDisk: warp.h:110-134.  The reference
+/* warp::squareToUniformConcentricDisk: warp.h:25-50.  The reference
  * evaluates phi in double (M_PI / 4.0f is a double). */
 static void SFX(concentric_disk)(R ux, R uy, R* ox, R* oy)
 {
@@ -81,7 +81,7 @@ static void SFX(concentric_disk)(R ux, R uy, R* ox, R* oy)
     *oy = r * s;
 }
 
-/* warp::squareToCosineHemisphere: warp.h:143-149.  The radicand is clamped
+/* warp::squareToCosineHemisphere: warp.h:58-64.  The radicand is clamped
  * at 0 (the reference can take sqrt of a rounding-negative value at |p| = 1). */
 static V3 SFX(cosine_hemisphere)(R ux, R uy)
 {
@@ -156,7 +156,7 @@ static R SFX(ggx_D)(R alpha, V3 m)
     return RC(1.0) / (ORC_PI * alpha * alpha * root * root);
 }
 
-/* Frame::sinTheta/tanTheta: frame.h:411-420 */
+/* Frame::sinTheta/tanTheta: frame.h:80-89 */
 static inline R SFX(sin_theta)(V3 v)
 {
     const R t = RC(1.0) - v.z * v.z;
@@ -254,8 +254,8 @@ static inline V3 SFX(refract)(V3 wi, V3 n, R eta, R cos_t)
 /* ------------------------------------------------------------ BSDFs */
 
 /* Material -> BSDF parameters: DiffuseMtl/MetalMtl/GlassMtl::getBSDF
- * (materials/diffusemtl.h:20-29, metal.h:116-134 with roughness^2 at :131,
- * glass.h:59-69 with roughness^2 at :66). */
+ * (materials/diffusemtl.h:20-29, metal.h:37-55 with roughness^2 at :52,
+ * glass.h:19-29 with roughness^2 at :26). */
 static inline R SFX(mtl_alpha)(const orc_material* m)
 {
     const R r = (R)m->roughness;
@@ -1384,6 +1384,51 @@ void SFX(orc_fresnel_batch)(const float* cos_i, uint32_t n, const float* eta3, c
         fd[i]    = (float)SFX(fresnel_dielectric_ext)(c, &ct, (R)ior);
         cos_t[i] = (float)ct;
     }
+}
+
+/* convertFromLinearTosRGB (libkernel/base/math.h:114-122), one channel: no
+ * clamp; the reference's exponent 0.41666. */
+static R SFX(srgb_encode)(R x)
+{
+    return x < RC(0.0031308) ? x * RC(12.92) : RC(1.055) * RPOW(x, RC(0.41666)) - RC(0.055);
+}
+
+/* convertsRGBToLinear (math.h:131-140), clamped to [0, 1]. */
+static R SFX(srgb_decode)(R s)
+{
+    if (s <= RC(0.0))
+        return RC(0.0);
+    if (s >= RC(1.0))
+        return RC(1.0);
+    if (s <= RC(0.04045))
+        return s / RC(12.92);
+    return RPOW((s + RC(0.055)) / RC(1.055), RC(2.4));
+}
+
+/* pull_up on n frames: frame9 = s, t, n per item (in and out). */
+void SFX(orc_pull_up)(const float* frame9, const float* ng3, const float* wo3, uint32_t n, float* out9)
+{
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        const float* a = frame9 + 9 * (size_t)i;
+        SFX(frame) f;
+        f.s     = (V3){(R)a[0], (R)a[1], (R)a[2]};
+        f.t     = (V3){(R)a[3], (R)a[4], (R)a[5]};
+        f.n     = (V3){(R)a[6], (R)a[7], (R)a[8]};
+        V3 ng   = {(R)ng3[3 * (size_t)i], (R)ng3[3 * (size_t)i + 1], (R)ng3[3 * (size_t)i + 2]};
+        V3 wo   = {(R)wo3[3 * (size_t)i], (R)wo3[3 * (size_t)i + 1], (R)wo3[3 * (size_t)i + 2]};
+        SFX(pull_up)(&f, ng, wo);
+        float* o = out9 + 9 * (size_t)i;
+        o[0] = (float)f.s.x, o[1] = (float)f.s.y, o[2] = (float)f.s.z;
+        o[3] = (float)f.t.x, o[4] = (float)f.t.y, o[5] = (float)f.t.z;
+        o[6] = (float)f.n.x, o[7] = (float)f.n.y, o[8] = (float)f.n.z;
+    }
+}
+
+void SFX(orc_srgb)(const float* in, uint32_t n, int decode, float* out)
+{
+    for (uint32_t i = 0; i < n; ++i)
+        out[i] = (float)(decode ? SFX(srgb_decode)((R)in[i]) : SFX(srgb_encode)((R)in[i]));
 }
 
 void SFX(orc_tex_eval)(const float* rgba, uint32_t w, uint32_t h, const float* uv, uint32_t n, float* out4)

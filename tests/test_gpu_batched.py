@@ -222,3 +222,20 @@ def test_autotuner_logs_and_reapplies(c1_small, tmp_path):
             assert got[k] == res["choice"][k], k
     finally:
         restore(integ, desc)
+
+
+def test_small_views_allocate_small_queues():
+    """At the default launch (2^27 paths per batch) a call with a few small
+    views sizes its queues by the batch it built, not by the launch's
+    ceiling."""
+    desc = scenes.make_c1(width=64, height=48, spp=1, grid=(96, 48), n_spheres=30, sphere_subdiv=2, hdri_w=128,
+                          hdri_h=64)
+    views = small_views(4, seed=9)
+    with Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, 8, 8) as integ:
+        upload_scene(integ, desc)
+        assert integ.getLaunch()["paths_per_batch"] == 1 << 27
+        integ.renderViews(views)
+        c = integ.counters()
+    need = sum(v.width * v.height * v.spp for v in views)
+    assert c["batches"] == 1 and c["rays"][1] == need
+    assert c["queue_paths"] == need

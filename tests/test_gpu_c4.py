@@ -85,7 +85,7 @@ def test_c4_full_frame_one_spp(c4):
         upload_scene(integ, c4)
         integ.render(1)
         cnt = integ.counters()
-        img = integ.readbackFramebuffer()
+        img = integ.readbackFramebufferLinear()
     assert cnt["rays"][1] == c4.width * c4.height
     assert cnt["hits"][1] > 0.6 * cnt["rays"][1]  # 71% of the camera rays hit (oracle, 384x216)
     assert np.isfinite(img).all()

@@ -69,7 +69,7 @@ def _render(desc, width):
         integ.setBVHWidth(width)
         upload_scene(integ, desc)
         integ.render(desc.spp)
-        return integ.readbackSums(), integ.counters(), integ.readbackFramebuffer(), _gpu_tables(integ, desc)
+        return integ.readbackSums(), integ.counters(), integ.readbackFramebufferLinear(), _gpu_tables(integ, desc)
 
 
 @pytest.mark.parametrize("width", [4, 8], ids=["bvh4", "bvh8"])

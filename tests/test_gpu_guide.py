@@ -366,3 +366,147 @@ def test_frozen_guide_is_invariant_to_batch_size(room_ref):
     for sums, rays in out[1:]:
         assert np.array_equal(rays, out[0][1])
         assert np.allclose(sums, out[0][0], rtol=1e-9, atol=1e-12)
+
+
+def mixture_rows(rng, n):
+    """Network outputs covering the mixture's range: weight logits in
+    [-3, 3], unnormalised means (a few zero), log-kappa in [-6, 10] so that
+    both clamps (kappa = e^-4 and e^8) are hit."""
+    out = np.zeros((n, 40))
+    out[:, 0::5] = rng.uniform(-3, 3, (n, 8))
+    for j in range(8):
+        out[:, 5 * j + 1:5 * j + 4] = rng.normal(size=(n, 3)) * rng.uniform(0.1, 3.0, (n, 1))
+        out[:, 5 * j + 4] = rng.uniform(-6, 10, n)
+    out[:8, 1:4] = 0.0  # zero mean vector: the renderer's fallback (0, 0, 1)
+    out[8:16, 4::5] = 8.0  # every lobe at the largest kappa
+    out[16:24, 4::5] = -4.0  # every lobe at the smallest
+    return out.astype(np.float32)
+
+
+def test_mixture_pdf_and_sampling_match_oracle(integ):
+    """vmf_mixture / vmf_mixture_pdf / vmf_mixture_sample as the guided
+    shading kernel runs them, on given network outputs, against the float64
+    oracle: densities, and samples drawn from the same numbers (lobe, the
+    Jakob (2012) inversion, the makeFrame completion)."""
+    from colvillea_amd import _capi as capi
+
+    rng = np.random.default_rng(41)
+    n = 200000
+    out = mixture_rows(rng, n)
+    d = rand_dirs(rng, n).astype(np.float32)
+    u = rng.uniform(size=(n, 3)).astype(np.float32)
+    pdf = np.zeros(n, np.float32)
+    sd = np.zeros((n, 3), np.float32)
+    sp = np.zeros(n, np.float32)
+    capi.check(capi.lib().clhip_guide_mixture_batch(integ.ctx, capi.fptr(out), capi.fptr(d), capi.fptr(u), n,
+                                                     capi.fptr(pdf), capi.fptr(sd), capi.fptr(sp)), integ.ctx)
+    o64 = out.astype(np.float64)
+    ref = G.mix_pdf(o64, d.astype(np.float64))
+    _, _, kappa, _ = G.mixture(o64)
+    kmax = kappa.max(axis=1)
+    # float32 rounding of cos(mu, d) - 1 is amplified by kappa in the exponent
+    tol = 2e-5 + 2e-6 * kmax
+    rel = np.abs(pdf - ref) / ref
+    big = ref > 1e-30
+    assert np.all(rel[big] <= tol[big]), (rel[big].max(), np.argmax(rel[big] / tol[big]))
+    # samples from the same numbers, away from the lobe-selection boundaries
+    w, _, _, _ = G.mixture(o64)
+    cw = np.cumsum(w, axis=1)[:, :-1]
+    clear = np.min(np.abs(cw - u[:, :1].astype(np.float64)), axis=1) > 1e-5
+    sref = G.mix_sample(o64, u[:, 0].astype(np.float64), u[:, 1].astype(np.float64), u[:, 2].astype(np.float64))
+    err = np.linalg.norm(sd.astype(np.float64) - sref, axis=1)
+    assert clear.mean() > 0.99
+    assert np.all(err[clear] < 2e-4), err[clear].max()
+    assert np.allclose(np.linalg.norm(sd, axis=1), 1.0, atol=1e-6)
+    # the density reported at the sample is the mixture's density there
+    sref_pdf = G.mix_pdf(o64, sd.astype(np.float64))
+    assert np.all(np.abs(sp - sref_pdf) <= tol * sref_pdf + 1e-30)
+
+
+@pytest.mark.parametrize("logk", [-4.0, 0.0, 3.0, 8.0])
+def test_mixture_sampling_chi_square_on_gpu(integ, logk):
+    """A chi-square test of the renderer's mixture samples against the
+    oracle's density integrated over (z, phi) cells, for lobes of one
+    concentration from the smallest to the largest kappa."""
+    from scipy import stats
+
+    from colvillea_amd import _capi as capi
+
+    rng = np.random.default_rng(43)
+    row = np.zeros(40)
+    row[0::5] = rng.uniform(-1.0, 1.0, 8)
+    for j in range(8):
+        v = rng.normal(size=3)
+        row[5 * j + 1:5 * j + 4] = v / np.linalg.norm(v)
+        row[5 * j + 4] = logk
+    n = 1 << 20
+    outs = np.repeat(row[None].astype(np.float32), n, 0)
+    u = rng.uniform(size=(n, 3)).astype(np.float32)
+    dummy = np.tile(np.array([[0, 0, 1]], np.float32), (n, 1))
+    pdf, sp = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    d = np.zeros((n, 3), np.float32)
+    capi.check(capi.lib().clhip_guide_mixture_batch(integ.ctx, capi.fptr(outs), capi.fptr(dummy), capi.fptr(u), n,
+                                                     capi.fptr(pdf), capi.fptr(d), capi.fptr(sp)), integ.ctx)
+    d = d.astype(np.float64)
+    # cells in (z, phi) about each lobe's frame would be needed for very
+    # sharp lobes; a fine global grid with adaptive sub-sampling suffices
+    nz, nph, sub = 64, 128, 16 if logk < 8 else 48
+    zi = np.minimum(((d[:, 2] + 1) / 2 * nz).astype(int), nz - 1)
+    pi_ = np.minimum(((np.arctan2(d[:, 1], d[:, 0]) + np.pi) / (2 * np.pi) * nph).astype(int), nph - 1)
+    obs = np.bincount(zi * nph + pi_, minlength=nz * nph).astype(np.float64)
+    exp = np.zeros(nz * nph)
+    zz = -1 + (np.arange(nz * sub) + 0.5) * (2 / (nz * sub))
+    pp = -np.pi + (np.arange(nph * sub) + 0.5) * (2 * np.pi / (nph * sub))
+    cell = (2 / (nz * sub)) * (2 * np.pi / (nph * sub))
+    _, mu, kappa, norm = G.mixture(row[None])
+    band = sub * max(1, 256 // sub)  # rows per band (whole cells), to bound memory
+    for r0 in range(0, nz * sub, band):
+        Z, P = np.meshgrid(zz[r0:r0 + band], pp, indexing="ij")
+        R = np.sqrt(1 - Z ** 2)
+        dirs = np.stack([R * np.cos(P), R * np.sin(P), Z], -1).reshape(-1, 3)
+        p = (norm[0] * np.exp(kappa[0] * (dirs @ mu[0].T - 1.0))).sum(1).reshape(Z.shape) * cell
+        nzb = Z.shape[0] // sub
+        exp[(r0 // sub) * nph:(r0 // sub + nzb) * nph] += p.reshape(nzb, sub, nph, sub).sum(axis=(1, 3)).ravel()
+    assert abs(exp.sum() - 1.0) < 2e-3, exp.sum()
+    exp *= n
+    m = exp > 5
+    # cells expecting fewer than 5 samples are pooled into one
+    o = np.append(obs[m], obs[~m].sum())
+    e = np.append(exp[m], exp[~m].sum())
+    chi2 = np.sum((o - e) ** 2 / e)
+    dof = len(e) - 1
+    assert stats.chi2.sf(chi2, dof) > 1e-4, (chi2, dof)
+
+
+def test_guide_query_over_scene_bounds():
+    """guide_query: (p - lo) / (hi - lo) per axis over the scene box (the
+    union of the instances' world-space boxes, each rounded outward to
+    float32), then the shading normal and wo unchanged."""
+    from colvillea_amd import _capi as capi
+
+    desc = scenes.make_c0(width=8, height=8, spp=1, hdri_w=32, hdri_h=16)
+    with Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, 8, 8) as it:
+        upload_scene(it, desc)
+        rng = np.random.default_rng(44)
+        n = 4096
+        p = rng.uniform(-6, 6, (n, 3)).astype(np.float32)
+        nn = rand_dirs(rng, n).astype(np.float32)
+        wo = rand_dirs(rng, n).astype(np.float32)
+        q = np.zeros((n, 9), np.float32)
+        box = np.zeros(6, np.float32)
+        capi.check(capi.lib().clhip_guide_query_batch(it.ctx, capi.fptr(p), capi.fptr(nn), capi.fptr(wo), n,
+                                                       capi.fptr(q), capi.fptr(box)), it.ctx)
+    # the scene box from the meshes and transforms, in float64, rounded out
+    lo, hi = np.full(3, np.inf), np.full(3, -np.inf)
+    for mi, xf in zip(desc.instance_mesh, desc.instance_xform):
+        v = desc.meshes[mi].vertices.astype(np.float64)
+        corners = np.array([[a, b, c] for a in (v[:, 0].min(), v[:, 0].max()) for b in (v[:, 1].min(), v[:, 1].max())
+                            for c in (v[:, 2].min(), v[:, 2].max())])
+        w = corners @ xf[:, :3].astype(np.float64).T + xf[:, 3].astype(np.float64)
+        lo = np.minimum(lo, w.min(0))
+        hi = np.maximum(hi, w.max(0))
+    assert np.all(box[:3] <= lo) and np.all(box[3:] >= hi)
+    assert np.allclose(box[:3], lo, rtol=1e-6, atol=1e-6) and np.allclose(box[3:], hi, rtol=1e-6, atol=1e-6)
+    inv = np.float32(1.0) / (box[3:] - box[:3])
+    assert np.array_equal(q[:, :3], (p - box[:3]) * inv)
+    assert np.array_equal(q[:, 3:6], nn) and np.array_equal(q[:, 6:9], wo)

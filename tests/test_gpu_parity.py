@@ -14,6 +14,7 @@ import pytest
 
 import orc
 from colvillea_amd import GlassMtl, Integrator, IntegratorType, MetalMtl, DiffuseMtl, scenes, upload_scene
+from colvillea_amd.integrator import ImageTexture2D
 from colvillea_amd import _capi as capi
 
 pytestmark = pytest.mark.gpu
@@ -256,8 +257,13 @@ def test_render_counts_and_image_match_oracle(c0_small):
     # float64 sums of identical float32 contributions, different order
     rel = np.mean((sums - osums) ** 2 / (osums ** 2 + 1e-2))
     assert rel < 1e-12, rel
-    img = integ.readbackFramebuffer()
+    img = integ.readbackFramebufferLinear()
     assert np.allclose(img[..., :3], (sums / desc.spp).astype(f32), rtol=1e-6, atol=1e-7)
+    # the reference's readbackFramebuffer: the same resolve, sRGB-encoded
+    # (raygeneration.cu:498-506), against the oracle's encoder (<= 4 ulp)
+    disp = integ.readbackFramebuffer()
+    assert np.all(disp[..., 3] == 1.0)
+    assert ulps(disp[..., :3], orc.srgb(img[..., :3], decode=False)).max() <= 4
 
 
 def test_render_matches_f64_oracle():
@@ -331,3 +337,58 @@ def test_textured_materials_match_oracle():
     assert rel < 1e-12, rel
     assert sums.mean() > 0
 
+
+
+def test_srgb_display_functions_within_4_ulp(c0_small):
+    """convertFromLinearTosRGB / convertsRGBToLinear (math.h:114-140) on the
+    GPU against the oracle: HDR values up to 1e4, the linear segment, the
+    threshold, negatives and the decoder's clamps."""
+    _, integ = c0_small
+    rng = np.random.default_rng(31)
+    x = np.concatenate([np.geomspace(1e-8, 1e4, 200001), rng.uniform(0, 1, 100000), -np.geomspace(1e-6, 1, 101),
+                        [0.0, 0.0031308, np.nextafter(f32(0.0031308), f32(0)), 1.0]]).astype(f32)
+    s = np.concatenate([np.linspace(-0.5, 1.5, 200001), [0.04045, np.nextafter(f32(0.04045), f32(1)), 0.0, 1.0]]
+                       ).astype(f32)
+    for vals, dec in ((x, 0), (s, 1)):
+        out = np.zeros_like(vals)
+        capi.check(capi.lib().clhip_srgb_batch(integ.ctx, capi.fptr(vals), len(vals), dec, capi.fptr(out)), integ.ctx)
+        ref = orc.srgb(vals, decode=bool(dec))
+        assert ulps(out, ref).max() <= 4, dec
+    # 1.055f * 1 - 0.055f: the reference's float32 result at 1
+    out = np.zeros(1, f32)
+    capi.check(capi.lib().clhip_srgb_batch(integ.ctx, capi.fptr(np.ones(1, f32)), 1, 0, capi.fptr(out)), integ.ctx)
+    assert out[0] == f32(0.99999994)
+
+
+def test_u8_srgb_texture_equals_decoded_float_texture():
+    """An 8-bit sRGB reflectance texture renders exactly as the float
+    texture of its texels decoded (c / 255, then convertsRGBToLinear, per
+    texel, before filtering) by the renderer's own decoder."""
+    import dataclasses
+
+    base = scenes.make_textured(width=96, height=64, spp=4)
+    rng = np.random.default_rng(32)
+    c8 = rng.integers(0, 256, size=(32, 32, 4), dtype=np.uint8)
+    renders = []
+    dec = None
+    for kind in ("u8", "float"):
+        with Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, base.width, base.height) as integ:
+            if kind == "u8":
+                tex = ImageTexture2D(c8, isSRGB=True)
+            else:
+                v = (c8[..., :3].astype(f32) / f32(255)).ravel()
+                d = np.zeros_like(v)
+                capi.check(capi.lib().clhip_srgb_batch(integ.ctx, capi.fptr(v), len(v), 1, capi.fptr(d)), integ.ctx)
+                dec = np.concatenate([d.reshape(32, 32, 3), c8[..., 3:].astype(f32) / f32(255)], -1)
+                tex = ImageTexture2D(dec)
+            mats = list(base.materials)
+            mats[1] = DiffuseMtl(tex)
+            desc = dataclasses.replace(base, materials=mats)
+            upload_scene(integ, desc)
+            integ.render(desc.spp)
+            renders.append((integ.readbackSums(), integ.counters()))
+    (sa, ca), (sb, cb) = renders
+    for k in ("rays", "hits", "shadow", "visible"):
+        assert np.array_equal(ca[k], cb[k]), k
+    assert np.array_equal(sa, sb) or np.allclose(sa, sb, rtol=1e-12, atol=1e-12)
+    assert np.allclose(dec[..., :3], orc.srgb(c8[..., :3].astype(f32) / f32(255), decode=True), rtol=5e-7, atol=1e-9)

@@ -215,3 +215,16 @@ def test_gather_known_paths_and_loop_form():
     assert out.shape == ref.shape
     assert np.allclose(out[:, :12], ref[:, :12])
     assert np.allclose(out[:, 12], ref[:, 12], rtol=1e-5)
+
+
+def test_frame_is_orthonormal_and_right_handed():
+    rng = np.random.default_rng(9)
+    n = rng.normal(size=(20000, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    n[:6] = [[1, 0, 0], [0, 1, 0], [0, 0, 1], [-1, 0, 0], [0, -1, 0], [0, 0, -1]]
+    t, b = G.frame(n)
+    for v in (t, b):
+        assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-12)
+    assert np.allclose(np.sum(t * b, 1), 0, atol=1e-12)
+    assert np.allclose(np.sum(t * n, 1), 0, atol=1e-12) and np.allclose(np.sum(b * n, 1), 0, atol=1e-12)
+    assert np.allclose(np.cross(t, b), n, atol=1e-12)

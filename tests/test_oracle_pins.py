@@ -429,6 +429,66 @@ def test_bilinear_texel_centres_and_ramp():
     assert abs(out[0, 0] - (w - 1) / 2) < 1e-6
 
 
+# ------------------------------------------------------------------ display encoding
+
+def srgb_standard_encode(x):
+    """The sRGB transfer function of IEC 61966-2-1 (float64): linear below
+    0.0031308, else 1.055 x^(1/2.4) - 0.055."""
+    x = np.asarray(x, np.float64)
+    return np.where(x <= 0.0031308, 12.92 * x, 1.055 * np.abs(x) ** (1 / 2.4) - 0.055)
+
+
+def srgb_standard_decode(s):
+    s = np.clip(np.asarray(s, np.float64), 0.0, 1.0)
+    return np.where(s <= 0.04045, s / 12.92, ((s + 0.055) / 1.055) ** 2.4)
+
+
+def test_srgb_functions_vs_standard():
+    # tabulated values of the standard transfer function
+    for x, s in ((0.5, 0.735356983052449), (0.18, 0.461356129500442), (0.0031308, 0.0404500),
+                 (1.0, 1.0), (0.0, 0.0)):
+        assert abs(srgb_standard_encode(x) - s) < 1e-6, x
+    assert abs(srgb_standard_decode(0.5) - 0.214041140482232) < 1e-12
+    x = np.concatenate([np.geomspace(1e-6, 1e4, 4001), np.linspace(0, 1, 1001)]).astype(f32)
+    for variant in ("f32", "f64"):
+        enc = orc.srgb(x, decode=False, variant=variant).astype(np.float64)
+        std = srgb_standard_encode(x)
+        # the reference's exponent 0.41666 instead of 1/2.4 moves the power
+        # by the factor x^(-6.67e-6): a relative change of 6.67e-6 |ln x|
+        pw = np.where(x > 0.0031308, 1.055 * x.astype(np.float64) ** (1 / 2.4), 0.0)
+        bound = pw * 7e-6 * np.abs(np.log(np.maximum(x, 1e-30))) + 4e-7 * (1.0 + np.abs(std))
+        assert np.all(np.abs(enc - std) <= bound), variant
+        dec = orc.srgb(np.linspace(-0.5, 1.5, 20001).astype(f32), decode=True, variant=variant).astype(np.float64)
+        sd = srgb_standard_decode(np.linspace(-0.5, 1.5, 20001).astype(f32))
+        assert np.allclose(dec, sd, rtol=1e-6, atol=1e-9), variant
+    # the float32 expression order of the reference: at 1 the encoder gives
+    # 1.055f * 1 - 0.055f, one ulp below 1
+    assert orc.srgb(np.array([1.0], f32))[0] == f32(1.055) - f32(0.055) == f32(0.99999994)
+    # the linear segment, and both branches meeting at the threshold
+    lo = np.nextafter(f32(0.0031308), f32(0))
+    assert orc.srgb(np.array([lo], f32))[0] == lo * f32(12.92)
+    hi = orc.srgb(np.array([f32(0.0031308)], f32))[0]
+    assert abs(float(hi) - 0.04045) < 2e-5
+    # decode clamps; decode(encode(x)) = x up to the exponent mismatch
+    assert np.array_equal(orc.srgb(np.array([-1.0, 0.0, 1.0, 7.0], f32), decode=True), np.array([0, 0, 1, 1], f32))
+    y = np.geomspace(1e-4, 1.0, 500).astype(f32)
+    back = orc.srgb(orc.srgb(y), decode=True).astype(np.float64)
+    assert np.allclose(back, y, rtol=3e-4, atol=1e-7)
+    # float32 and float64 variants agree to a few ulp (the float32 rounding of
+    # the power and the product, amplified up to 2.4x by the subtraction of
+    # 0.055 near the threshold)
+    assert ulps(orc.srgb(x, variant="f32"), orc.srgb(x, variant="f64")).max() <= 8
+
+
+def test_u8_srgb_texture_reads_as_decoded_float():
+    c = np.arange(256, dtype=np.uint8).reshape(16, 16, 1).repeat(3, 2)
+    lin = orc.texture_rgba(ImageTexture2D(c))
+    assert np.array_equal(lin[..., 0], (np.arange(256, dtype=f32) / f32(255)).reshape(16, 16))
+    s = orc.texture_rgba(ImageTexture2D(c, isSRGB=True))
+    assert np.allclose(s[..., 1].ravel(), srgb_standard_decode(np.arange(256) / 255.0), rtol=1e-6, atol=1e-9)
+    assert np.all(s[..., 3] == 1.0)
+
+
 # ------------------------------------------------------------------ geometry
 
 def _single_triangle_scene(v):
@@ -663,3 +723,73 @@ def test_f32_and_f64_oracles_agree():
     rel = np.mean((a - b) ** 2 / (b ** 2 + 1e-2))
     assert rel < 1e-6
     assert ca["rays"][1] == cb["rays"][1]
+
+
+def _pull_up_cases(rng, n):
+    """Random geometric normals, shading normals tilted from them by up to
+    80 degrees (a normal map), tangents orthogonal to the shading normal,
+    and wo on the geometric side near grazing, where the mirror of wo about
+    the shading normal falls below the surface -- the regime of the fix."""
+    ng = rng.normal(size=(n, 3))
+    ng /= np.linalg.norm(ng, axis=1, keepdims=True)
+    a = rng.normal(size=(n, 3))
+    a -= np.sum(a * ng, 1, keepdims=True) * ng
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    tilt = rng.uniform(0.2, 1.4, (n, 1))
+    ns = np.cos(tilt) * ng + np.sin(tilt) * a
+    t = rng.normal(size=(n, 3))
+    t -= np.sum(t * ns, 1, keepdims=True) * ns
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    s = np.cross(ns, t)
+    b = np.cross(ng, a)
+    elev = rng.uniform(0.02, 0.6, (n, 1))
+    az = rng.uniform(0, 2 * np.pi, (n, 1))
+    wo = np.cos(elev) * (np.cos(az) * a + np.sin(az) * b) + np.sin(elev) * ng
+    # half of the cases seen from the other side (the fix flips n and ng)
+    flip = rng.uniform(size=(n, 1)) < 0.5
+    wo = np.where(flip, -wo, wo)
+    f = lambda v: v.astype(np.float32)
+    return f(s), f(t), f(ns), f(ng), f(wo)
+
+
+def test_normal_map_pull_up_invariants():
+    """After the pull-up (Material::fixNormalPerturbingShadingFrame,
+    material.h:236-285) the mirror of wo about the new shading normal lies
+    above the geometric surface, lifted to 0.1 of the old mirror's depth
+    (the lift 1.1, config.h:9); the new normal is unit length, on the
+    geometric side, the half vector of wo and that mirror; the frame is
+    orthonormal and right-handed.  Cases where the mirror is already above
+    the surface are left as they are."""
+    rng = np.random.default_rng(51)
+    s, t, ns, ng, wo = _pull_up_cases(rng, 50000)
+    S, T, N = (x.astype(np.float64) for x in orc.pull_up(s, t, ns, ng, wo, "f64"))
+    ng64, wo64, ns64 = ng.astype(np.float64), wo.astype(np.float64), ns.astype(np.float64)
+    side = np.where(np.sum(wo64 * ng64, 1) < 0, -1.0, 1.0)[:, None]
+    g, n0 = ng64 * side, ns64 * side  # the side of wo
+    r0 = 2 * np.sum(wo64 * n0, 1, keepdims=True) * n0 - wo64
+    depth = np.sum(r0 * g, 1)
+    fixed = (depth <= 0) & (np.sum(n0 * g, 1) > 0)
+    assert fixed.mean() > 0.3 and (~fixed).mean() > 0.1
+    r1 = 2 * np.sum(wo64 * N, 1, keepdims=True) * N - wo64
+    # the new mirror direction: above the surface by 0.1 of the old depth,
+    # measured before normalisation (r0 + 1.1 (a / b) n0 has height 0.1 a)
+    lift = r0 + 1.1 * (-depth / np.sum(n0 * g, 1))[:, None] * n0
+    want = 0.1 * (-depth) / np.linalg.norm(lift, axis=1)
+    h = np.sum(r1 * g, 1)
+    assert np.all(h[fixed] > 0)
+    # (the float64 variant's outputs are rounded to float32, and h is a
+    # difference of terms of order 1: a few 1e-7 absolute)
+    assert np.allclose(h[fixed], want[fixed], rtol=1e-5, atol=4e-6)
+    assert np.median(np.abs(h - want)[fixed]) < 2e-7
+    assert np.allclose(np.linalg.norm(N, axis=1), 1.0, atol=2e-7)
+    assert np.all(np.sum(N * g, 1)[fixed] >= 0)
+    for u, v in ((S, T), (S, N), (T, N)):
+        assert np.allclose(np.sum(u[fixed] * v[fixed], 1), 0, atol=5e-7)
+    assert np.allclose(np.cross(N[fixed], T[fixed]), S[fixed], atol=5e-7)
+    # untouched cases keep the side-flipped normal and their tangents
+    keep = ~fixed
+    assert np.allclose(N[keep], n0[keep], atol=1e-7)
+    assert np.allclose(T[keep], t[keep].astype(np.float64), atol=1e-7)
+    # the float32 variant agrees with float64 to float32 rounding
+    S32, T32, N32 = orc.pull_up(s, t, ns, ng, wo, "f32")
+    assert np.allclose(N32, N, atol=2e-5) and np.allclose(T32, T, atol=2e-5)
