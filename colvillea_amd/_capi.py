@@ -218,6 +218,7 @@ SIGNATURES = {
     "clhip_guide_infer": (C.c_int, [P, f32p, u32, f32p]),
     "clhip_guide_fp8_scales": (C.c_int, [P, f32p]),
     "clhip_guide_train": (C.c_int, [P, f32p, u32, f32p, f32p]),
+    "clhip_trace_rays_device": (C.c_int, [P, C.c_void_p, u32, C.c_int, C.c_void_p, C.c_void_p]),
     "clhip_guide_mixture_batch": (C.c_int, [P, f32p, f32p, f32p, u32, f32p, f32p, f32p]),
     "clhip_guide_query_batch": (C.c_int, [P, f32p, f32p, f32p, u32, f32p, f32p]),
     "clhip_guide_records": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), u32,

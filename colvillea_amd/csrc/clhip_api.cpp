@@ -118,6 +118,7 @@ struct clhip_ctx
     bool                     have_blas = false, have_tlas = false;
     DevBuf                   d_nodes, d_tris, d_meshes, d_pos, d_nrm, d_tan, d_uv, d_idx, d_inst;
     size_t                   node_bytes = 0, tri_bytes = 0; /* in use in d_nodes, d_tris */
+    uint2*                   noalloc_stack = nullptr; /* this device's scratch of clhip_trace_rays_device */
     std::vector<InstanceRec> inst;
     std::vector<uint32_t>    inst_entity;
     uint32_t                 tlas_root = 0, tlas_depth = 0;
@@ -994,6 +995,14 @@ int clhip_create(uint32_t width, uint32_t height, uint32_t integrator_type, clhi
     e          = hipStreamCreate(&c->stream);
     if (e != hipSuccess)
         return fail(nullptr, CLHIP_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    /* the device address of clhip_trace_rays_device's scratch, looked up
+     * once per device with that device current */
+    e = clpt::noalloc_scratch(c->device, &c->noalloc_stack);
+    if (e != hipSuccess)
+    {
+        (void)hipStreamDestroy(c->stream);
+        return fail(nullptr, CLHIP_E_HIP, "scratch address on device %d: %s", c->device, hipGetErrorString(e));
+    }
     c->cam.width  = width;
     c->cam.height = height;
     c->cam.npix   = width * height;
@@ -1328,6 +1337,9 @@ int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uin
     else
         CLHIP_CHECK(ctx, upload(ctx->d_nodes, nodes4));
     ctx->node_bytes = ctx->bvh_width == 8 ? nodes8.size() * sizeof(Node8) : nodes4.size() * sizeof(Node4);
+    /* the records go up now, so traversal alone (clhip_trace_rays_device)
+     * needs no further upload; sync_instances fills in the materials */
+    CLHIP_CHECK(ctx, upload(ctx->d_inst, inst));
     ctx->inst        = std::move(inst);
     ctx->inst_entity = std::move(ent);
     ctx->tlas_root   = r.root;
@@ -2095,6 +2107,29 @@ int clhip_occlusion_batch(clhip_ctx* ctx, const float* rays8, uint32_t n, uint8_
         return rc;
     for (uint32_t i = 0; i < n; ++i)
         occluded[i] = out[i].x != 0.0f ? 1u : 0u;
+    return CLHIP_OK;
+}
+
+int clhip_trace_rays_device(clhip_ctx* ctx, const float* rays8, uint32_t n, int any, float* hit4, uint32_t* prim)
+{
+    if (!ctx || (n && (!rays8 || !hit4 || !prim)))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_trace_rays_device: bad arguments");
+    if (n > CLHIP_TRACE_RAYS_DEVICE_MAX)
+        return fail(ctx, CLHIP_E_INVALID, "clhip_trace_rays_device: %u rays, at most %u per call", n,
+                    (uint32_t)CLHIP_TRACE_RAYS_DEVICE_MAX);
+    if (!ctx->have_tlas)
+        return fail(ctx, CLHIP_E_STATE, "no TLAS: call clhip_build_tlas first");
+    if (!ctx->noalloc_stack)
+        return fail(ctx, CLHIP_E_STATE, "no scratch address for device %d", ctx->device);
+    if (n == 0)
+        return CLHIP_OK;
+    int dev = -1;
+    CLHIP_CHECK(ctx, hipGetDevice(&dev));
+    if (dev != ctx->device)
+        return fail(ctx, CLHIP_E_STATE, "device %d is current, the context's is %d", dev, ctx->device);
+    CLHIP_CHECK(ctx, clpt::launch_trace_noalloc(ctx->stream, scene_view(ctx), reinterpret_cast<const float4*>(rays8),
+                                                n, any ? 1 : 0, reinterpret_cast<float4*>(hit4), prim,
+                                                ctx->noalloc_stack));
     return CLHIP_OK;
 }
 

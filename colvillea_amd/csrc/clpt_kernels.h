@@ -65,6 +65,13 @@ struct GuideTrainBufs
     float*         scl;
 };
 /* bf16 inference, or FP8 when img8 is not NULL (scales scl) */
+/* The allocation-free trace entry point: its fixed grid, the per-device
+ * address of its scratch (the stacks' overflow area), and the launch. */
+#define CLPT_NOALLOC_BLOCKS 256u
+#define CLPT_MAX_DEVICES 64
+hipError_t noalloc_scratch(int device, uint2** out);
+hipError_t launch_trace_noalloc(hipStream_t s, const SceneView& sc, const float4* rays, uint32_t n, int any,
+                                float4* out, uint32_t* prim, uint2* scratch_addr);
 hipError_t launch_guide_mixture_batch(hipStream_t s, const float* out40, const float* dir3, const float* u3, uint32_t n,
                                       float* pdf, float* smp3, float* smp_pdf);
 hipError_t launch_guide_query_batch(hipStream_t s, const float lo[3], const float inv_ext[3], const float* p3,

@@ -20,6 +20,8 @@
 #include "clpt_guide.h"
 #include "clpt_kernels.h"
 
+#include <mutex>
+
 
 /* ------------------------------------------------------------ wave utils */
 
@@ -1594,6 +1596,54 @@ __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ s
     for (; i < n; i += stride)
         dst[i] = src[i];
 }
+
+/* ------------------------------------------------------------ allocation-free tracing */
+
+/* The traversal stacks' overflow area of the allocation-free trace entry
+ * point: CLPT_NOALLOC_BLOCKS persistent blocks of 256 lanes, each lane
+ * CLPT_GSTACK_ENTRIES (ref, t) pairs, laid out [entry][global lane] as the
+ * trace kernel addresses it (58 MiB).  Each device holds its own copy for
+ * the life of the process; every call on a device shares it, and it needs
+ * no clearing (an entry is always pushed before it is popped). */
+static __device__ __align__(256) uint2 scratch[(size_t)CLPT_NOALLOC_BLOCKS * 256u * CLPT_GSTACK_ENTRIES];
+
+namespace clpt
+{
+/* The device address of `scratch` on `device` (which must be current),
+ * looked up once per device.  Host code uses this address, never the
+ * array's name, which in host code is only a stand-in. */
+hipError_t noalloc_scratch(int device, uint2** out)
+{
+    static std::mutex mu;
+    static uint2*     addr[CLPT_MAX_DEVICES] = {};
+    if (device < 0 || device >= CLPT_MAX_DEVICES)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device])
+    {
+        void*      p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        addr[device] = static_cast<uint2*>(p);
+    }
+    *out = addr[device];
+    return hipSuccess;
+}
+
+/* The batch trace kernel on the fixed grid the scratch is laid out for. */
+hipError_t launch_trace_noalloc(hipStream_t s, const SceneView& sc, const float4* rays, uint32_t n, int any,
+                                float4* out, uint32_t* prim, uint2* scratch_addr)
+{
+    if (sc.bvh_width == 8)
+        hipLaunchKernelGGL(k_trace_batch<8>, dim3(CLPT_NOALLOC_BLOCKS), dim3(256), 0, s, sc, rays, n, any, out, prim,
+                           scratch_addr);
+    else
+        hipLaunchKernelGGL(k_trace_batch<4>, dim3(CLPT_NOALLOC_BLOCKS), dim3(256), 0, s, sc, rays, n, any, out, prim,
+                           scratch_addr);
+    return hipGetLastError();
+}
+} // namespace clpt
 
 /* ------------------------------------------------------------ launchers */
 

@@ -281,6 +281,23 @@ int clhip_trace_batch(clhip_ctx* ctx, const float* rays8, uint32_t n, float* t, 
 /* Any hit (optix.cu:77-110 with terminate-on-first-hit). */
 int clhip_occlusion_batch(clhip_ctx* ctx, const float* rays8, uint32_t n, uint8_t* occluded);
 
+/* Closest hit (any = 0) or any hit (any != 0) for n rays already in device
+ * memory, writing device memory, with no allocation of any kind: for
+ * callers that cannot allocate (code under HIP graph capture, serving loops
+ * that trace on their own schedule, hosts whose allocator policy forbids
+ * hipMalloc).  No counterpart in the reference.  rays8 = n x (o.xyz, tmin,
+ * d.xyz, tmax); hit4 = n x (t or -1, b1, b2, instance id as uint bits) for
+ * closest hit, or (1 if occluded else 0, 0, 0, 0) for any hit; prim = n
+ * primitive ids (0xffffffff on a miss; 0 for any hit).  Results equal
+ * clhip_trace_batch / clhip_occlusion_batch.  Asynchronous on the context
+ * stream (clhip_sync waits); the buffers must stay valid until then.
+ * Traversal stacks spill into a scratch array fixed in the library (58 MiB
+ * per device, held until the process ends), shared by every call on that
+ * device: calls that overlap on one GPU race.  At most
+ * CLHIP_TRACE_RAYS_DEVICE_MAX rays per call; the TLAS must be built. */
+#define CLHIP_TRACE_RAYS_DEVICE_MAX 0x80000000u
+int clhip_trace_rays_device(clhip_ctx* ctx, const float* rays8, uint32_t n, int any, float* hit4, uint32_t* prim);
+
 /* World-to-object matrices the context computed for its instances
  * (num_instances * 12 floats). */
 int clhip_instance_inverses(clhip_ctx* ctx, float* out12, uint32_t n);
