@@ -98,6 +98,7 @@ class clhip_stats(C.Structure):
         ("graph_captures", C.c_uint64),
         ("queue_paths", C.c_uint64),
         ("scene_bytes", C.c_uint64),
+        ("blas_builds", C.c_uint64),
     ]
 
 
@@ -169,6 +170,7 @@ SIGNATURES = {
     "clhip_create_texture_u8": (C.c_int, [P, C.POINTER(C.c_uint8), u32, u32, u32, C.c_int, C.POINTER(i32)]),
     "clhip_set_bvh_width": (C.c_int, [P, u32]),
     "clhip_build_blas": (C.c_int, [P, C.POINTER(clhip_mesh), u32]),
+    "clhip_build_blas_subset": (C.c_int, [P, C.POINTER(clhip_mesh), u32p, u32]),
     "clhip_build_tlas": (C.c_int, [P, u32p, u32p, f32p, u32]),
     "clhip_build_materials": (C.c_int, [P, C.POINTER(clhip_material), u32]),
     "clhip_build_entities": (C.c_int, [P, u32p, u32]),

@@ -72,6 +72,18 @@ struct MeshInfo
     float    lo[3]    = {0, 0, 0}, hi[3] = {0, 0, 0};
 };
 
+/* One mesh's tree (its own node and triangle numbering) and records. */
+struct MeshBuild
+{
+    std::vector<Node4>    nodes4;
+    std::vector<Node8>    nodes8;
+    std::vector<TriRec>   tris;
+    std::vector<float>    pos, nrm, tan, uv;
+    std::vector<uint32_t> idx;
+    uint32_t              flags = 0; /* MESH_HAS_* */
+    MeshInfo              info;
+};
+
 struct TimedLaunch
 {
     hipEvent_t a, b;
@@ -115,6 +127,8 @@ struct clhip_ctx
     std::vector<Node8>       blas_nodes8;
     std::vector<MeshInfo>    meshes;
     std::vector<MeshRec>     mesh_recs;
+    std::vector<MeshBuild>   mesh_builds; /* per-mesh trees, reassembled on every BLAS (re)build */
+    uint64_t                 blas_builds = 0; /* mesh trees built since creation */
     bool                     have_blas = false, have_tlas = false;
     DevBuf                   d_nodes, d_tris, d_meshes, d_pos, d_nrm, d_tan, d_uv, d_idx, d_inst;
     size_t                   node_bytes = 0, tri_bytes = 0; /* in use in d_nodes, d_tris */
@@ -1130,113 +1144,165 @@ int clhip_set_bvh_width(clhip_ctx* ctx, uint32_t width)
         ctx->have_blas = ctx->have_tlas = false; /* the trees must be built again */
         ctx->blas_nodes4.clear();
         ctx->blas_nodes8.clear();
+        ctx->mesh_builds.clear();
     }
     return CLHIP_OK;
 }
 
-int clhip_build_blas(clhip_ctx* ctx, const clhip_mesh* meshes, uint32_t num_meshes)
+extern "C++" {
+/* One mesh's tree built on its own: inner references count from its first
+ * node, leaves from its first triangle; assemble_blas relocates them into
+ * the shared arrays.  Kept per mesh so that an edit rebuilds one tree. */
+static int build_mesh(clhip_ctx* ctx, const clhip_mesh& me, uint32_t m, MeshBuild& out)
 {
-    if (!ctx || (!meshes && num_meshes))
-        return fail(ctx, CLHIP_E_INVALID, "clhip_build_blas: bad arguments");
+    if (!me.vertices || (!me.indices && me.num_triangles))
+        return fail(ctx, CLHIP_E_INVALID, "mesh %u: missing vertices or indices", m);
+    const uint32_t nt = me.num_triangles, nv = me.num_vertices;
+    if (nt >= CLPT_MAX_TRIS)
+        return fail(ctx, CLHIP_E_UNSUPPORTED, "mesh %u: %u triangles exceed the 2^27 limit", m, nt);
+    for (size_t i = 0; i < 3 * (size_t)nt; ++i)
+        if (me.indices[i] >= nv)
+            return fail(ctx, CLHIP_E_INVALID, "mesh %u: index %u out of range (%u vertices)", m, me.indices[i], nv);
+    std::vector<float> lo(3 * (size_t)nt), hi(3 * (size_t)nt);
+    for (uint32_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k)
+        {
+            float a = me.vertices[3 * (size_t)me.indices[3 * (size_t)t] + k];
+            float b = me.vertices[3 * (size_t)me.indices[3 * (size_t)t + 1] + k];
+            float c = me.vertices[3 * (size_t)me.indices[3 * (size_t)t + 2] + k];
+            lo[3 * (size_t)t + k] = std::min(a, std::min(b, c));
+            hi[3 * (size_t)t + k] = std::max(a, std::max(b, c));
+        }
+    MeshBuild             mb;
+    std::vector<uint32_t> order;
+    const auto            leaf = [](uint32_t first, uint32_t count) { return CLPT_TRI_LEAF(first, count); };
+    const clpt::BvhResult r    = ctx->bvh_width == 8
+                                     ? clpt::build_bvh8(lo.data(), hi.data(), nt, CLPT_MAX_LEAF_TRIS, mb.nodes8, order, leaf)
+                                     : clpt::build_bvh4(lo.data(), hi.data(), nt, CLPT_MAX_LEAF_TRIS, mb.nodes4, order, leaf);
+    mb.tris.resize(nt);
+    for (uint32_t i = 0; i < nt; ++i)
+    {
+        const uint32_t p  = order[i];
+        const float*   v0 = me.vertices + 3 * (size_t)me.indices[3 * (size_t)p];
+        const float*   v1 = me.vertices + 3 * (size_t)me.indices[3 * (size_t)p + 1];
+        const float*   v2 = me.vertices + 3 * (size_t)me.indices[3 * (size_t)p + 2];
+        float          pf;
+        std::memcpy(&pf, &p, 4);
+        mb.tris[i].v0p = make_float4(v0[0], v0[1], v0[2], pf);
+        mb.tris[i].v1  = make_float4(v1[0], v1[1], v1[2], 0.0f);
+        mb.tris[i].v2  = make_float4(v2[0], v2[1], v2[2], 0.0f);
+    }
+    mb.info.ntris  = nt;
+    mb.info.nverts = nv;
+    mb.info.root   = r.root;
+    mb.info.depth  = r.max_depth;
+    std::memcpy(mb.info.lo, r.lo, sizeof(mb.info.lo));
+    std::memcpy(mb.info.hi, r.hi, sizeof(mb.info.hi));
+    mb.flags = (me.normals ? MESH_HAS_NORMALS : 0u) | (me.tangents ? MESH_HAS_TANGENTS : 0u) |
+        (me.uvs ? MESH_HAS_UVS : 0u);
+    mb.pos.assign(me.vertices, me.vertices + 3 * (size_t)nv);
+    if (me.normals)
+        mb.nrm.assign(me.normals, me.normals + 3 * (size_t)nv);
+    if (me.tangents)
+        mb.tan.assign(me.tangents, me.tangents + 3 * (size_t)nv);
+    if (me.uvs)
+        mb.uv.assign(me.uvs, me.uvs + 2 * (size_t)nv);
+    mb.idx.assign(me.indices, me.indices + 3 * (size_t)nt);
+    out = std::move(mb);
+    ++ctx->blas_builds;
+    return CLHIP_OK;
+}
+
+/* A child reference of a mesh's own tree, moved to where its nodes and
+ * triangles land in the shared arrays. */
+static inline uint32_t relocate_ref(uint32_t c, uint32_t node_off, uint32_t tri_off)
+{
+    if (c == CLPT_CHILD_EMPTY)
+        return c;
+    if (c & CLPT_LEAF_BIT)
+        return (c & ~(CLPT_MAX_TRIS - 1u)) | ((c & (CLPT_MAX_TRIS - 1u)) + tri_off);
+    return c + node_off;
+}
+
+template <typename NodeT>
+static void append_nodes(std::vector<NodeT>& dst, const std::vector<NodeT>& src, uint32_t tri_off)
+{
+    const uint32_t node_off = (uint32_t)dst.size();
+    for (NodeT nd : src)
+    {
+        for (uint32_t& c : nd.child)
+            c = relocate_ref(c, node_off, tri_off);
+        dst.push_back(nd);
+    }
+}
+
+/* The shared node, triangle and attribute arrays from the per-mesh builds,
+ * uploaded; the TLAS must be built again (the roots moved). */
+static int assemble_blas(clhip_ctx* ctx)
+{
+    const std::vector<MeshBuild>& mbs = ctx->mesh_builds;
+    uint64_t                      total_tris = 0;
+    bool                          any_n = false, any_t = false, any_uv = false;
+    for (const MeshBuild& mb : mbs)
+    {
+        total_tris += mb.info.ntris;
+        any_n |= (mb.flags & MESH_HAS_NORMALS) != 0u;
+        any_t |= (mb.flags & MESH_HAS_TANGENTS) != 0u;
+        any_uv |= (mb.flags & MESH_HAS_UVS) != 0u;
+    }
+    if (total_tris >= CLPT_MAX_TRIS)
+        return fail(ctx, CLHIP_E_UNSUPPORTED, "%llu triangles exceed the 2^27 limit", (unsigned long long)total_tris);
     std::vector<Node4>    nodes4;
     std::vector<Node8>    nodes8;
     std::vector<TriRec>   tris;
-    std::vector<MeshInfo> infos(num_meshes);
-    std::vector<MeshRec>  recs(num_meshes);
+    std::vector<MeshInfo> infos(mbs.size());
+    std::vector<MeshRec>  recs(mbs.size());
     std::vector<float>    pos, nrm, tan, uv;
     std::vector<uint32_t> idx;
-    bool                  any_n = false, any_t = false, any_uv = false;
-    for (uint32_t m = 0; m < num_meshes; ++m)
+    tris.reserve(total_tris);
+    for (size_t m = 0; m < mbs.size(); ++m)
     {
-        any_n |= meshes[m].normals != nullptr;
-        any_t |= meshes[m].tangents != nullptr;
-        any_uv |= meshes[m].uvs != nullptr;
-    }
-    uint64_t total_tris = 0;
-    for (uint32_t m = 0; m < num_meshes; ++m)
-        total_tris += meshes[m].num_triangles;
-    if (total_tris >= CLPT_MAX_TRIS)
-        return fail(ctx, CLHIP_E_UNSUPPORTED, "%llu triangles exceed the 2^27 limit", (unsigned long long)total_tris);
-    for (uint32_t m = 0; m < num_meshes; ++m)
-    {
-        const clhip_mesh& me = meshes[m];
-        if (!me.vertices || (!me.indices && me.num_triangles))
-            return fail(ctx, CLHIP_E_INVALID, "mesh %u: missing vertices or indices", m);
-        const uint32_t nt = me.num_triangles, nv = me.num_vertices;
-        for (size_t i = 0; i < 3 * (size_t)nt; ++i)
-            if (me.indices[i] >= nv)
-                return fail(ctx, CLHIP_E_INVALID, "mesh %u: index %u out of range (%u vertices)", m, me.indices[i], nv);
-        std::vector<float> lo(3 * (size_t)nt), hi(3 * (size_t)nt);
-        for (uint32_t t = 0; t < nt; ++t)
-            for (int k = 0; k < 3; ++k)
-            {
-                float a = me.vertices[3 * (size_t)me.indices[3 * (size_t)t] + k];
-                float b = me.vertices[3 * (size_t)me.indices[3 * (size_t)t + 1] + k];
-                float c = me.vertices[3 * (size_t)me.indices[3 * (size_t)t + 2] + k];
-                lo[3 * (size_t)t + k] = std::min(a, std::min(b, c));
-                hi[3 * (size_t)t + k] = std::max(a, std::max(b, c));
-            }
-        const uint32_t        tri_base = (uint32_t)tris.size();
-        std::vector<uint32_t> order;
-        const auto           leaf = [tri_base](uint32_t first, uint32_t count) {
-            return CLPT_TRI_LEAF(tri_base + first, count);
-        };
-        const clpt::BvhResult r = ctx->bvh_width == 8
-                                      ? clpt::build_bvh8(lo.data(), hi.data(), nt, CLPT_MAX_LEAF_TRIS, nodes8, order, leaf)
-                                      : clpt::build_bvh4(lo.data(), hi.data(), nt, CLPT_MAX_LEAF_TRIS, nodes4, order, leaf);
-        for (uint32_t i = 0; i < nt; ++i)
-        {
-            const uint32_t p  = order[i];
-            const float*   v0 = me.vertices + 3 * (size_t)me.indices[3 * (size_t)p];
-            const float*   v1 = me.vertices + 3 * (size_t)me.indices[3 * (size_t)p + 1];
-            const float*   v2 = me.vertices + 3 * (size_t)me.indices[3 * (size_t)p + 2];
-            TriRec         tr;
-            uint32_t       pu = p;
-            float          pf;
-            std::memcpy(&pf, &pu, 4);
-            tr.v0p = make_float4(v0[0], v0[1], v0[2], pf);
-            tr.v1  = make_float4(v1[0], v1[1], v1[2], 0.0f);
-            tr.v2  = make_float4(v2[0], v2[1], v2[2], 0.0f);
-            tris.push_back(tr);
-        }
-        MeshInfo& info = infos[m];
-        info.tri_base  = tri_base;
-        info.ntris     = nt;
-        info.nverts    = nv;
-        info.root      = r.root;
-        info.depth     = r.max_depth;
-        std::memcpy(info.lo, r.lo, sizeof(info.lo));
-        std::memcpy(info.hi, r.hi, sizeof(info.hi));
-        MeshRec& rec = recs[m];
-        rec.idx_off  = (uint32_t)(idx.size() / 3);
-        rec.vtx_off  = (uint32_t)(pos.size() / 3);
-        rec.ntris    = nt;
-        rec.flags    = (me.normals ? MESH_HAS_NORMALS : 0u) | (me.tangents ? MESH_HAS_TANGENTS : 0u) |
-            (me.uvs ? MESH_HAS_UVS : 0u);
-        pos.insert(pos.end(), me.vertices, me.vertices + 3 * (size_t)nv);
+        const MeshBuild& mb       = mbs[m];
+        const uint32_t   tri_base = (uint32_t)tris.size();
+        const uint32_t   node_off = (uint32_t)(ctx->bvh_width == 8 ? nodes8.size() : nodes4.size());
+        if (ctx->bvh_width == 8)
+            append_nodes(nodes8, mb.nodes8, tri_base);
+        else
+            append_nodes(nodes4, mb.nodes4, tri_base);
+        tris.insert(tris.end(), mb.tris.begin(), mb.tris.end());
+        infos[m]          = mb.info;
+        infos[m].tri_base = tri_base;
+        infos[m].root     = relocate_ref(mb.info.root, node_off, tri_base);
+        const uint32_t nv = mb.info.nverts;
+        recs[m].idx_off   = (uint32_t)(idx.size() / 3);
+        recs[m].vtx_off   = (uint32_t)(pos.size() / 3);
+        recs[m].ntris     = mb.info.ntris;
+        recs[m].flags     = mb.flags;
+        pos.insert(pos.end(), mb.pos.begin(), mb.pos.end());
         if (any_n)
         {
-            if (me.normals)
-                nrm.insert(nrm.end(), me.normals, me.normals + 3 * (size_t)nv);
+            if (mb.flags & MESH_HAS_NORMALS)
+                nrm.insert(nrm.end(), mb.nrm.begin(), mb.nrm.end());
             else
                 nrm.resize(nrm.size() + 3 * (size_t)nv, 0.0f);
         }
         if (any_t)
         {
-            if (me.tangents)
-                tan.insert(tan.end(), me.tangents, me.tangents + 3 * (size_t)nv);
+            if (mb.flags & MESH_HAS_TANGENTS)
+                tan.insert(tan.end(), mb.tan.begin(), mb.tan.end());
             else
                 tan.resize(tan.size() + 3 * (size_t)nv, 0.0f);
         }
         if (any_uv)
         {
-            if (me.uvs)
-                uv.insert(uv.end(), me.uvs, me.uvs + 2 * (size_t)nv);
+            if (mb.flags & MESH_HAS_UVS)
+                uv.insert(uv.end(), mb.uv.begin(), mb.uv.end());
             else
                 uv.resize(uv.size() + 2 * (size_t)nv, 0.0f);
         }
-        idx.insert(idx.end(), me.indices, me.indices + 3 * (size_t)nt);
+        idx.insert(idx.end(), mb.idx.begin(), mb.idx.end());
     }
+    CLHIP_CHECK(ctx, hipStreamSynchronize(ctx->stream)); /* the old arrays may be in use */
     CLHIP_CHECK(ctx, upload(ctx->d_tris, tris));
     ctx->tri_bytes = tris.size() * sizeof(TriRec);
     CLHIP_CHECK(ctx, upload(ctx->d_meshes, recs));
@@ -1247,11 +1313,50 @@ int clhip_build_blas(clhip_ctx* ctx, const clhip_mesh* meshes, uint32_t num_mesh
     CLHIP_CHECK(ctx, upload(ctx->d_idx, idx));
     ctx->blas_nodes4 = std::move(nodes4);
     ctx->blas_nodes8 = std::move(nodes8);
-    ctx->meshes     = std::move(infos);
-    ctx->mesh_recs  = std::move(recs);
-    ctx->have_blas  = true;
-    ctx->have_tlas  = false; /* roots moved: the TLAS must be rebuilt */
+    ctx->meshes      = std::move(infos);
+    ctx->mesh_recs   = std::move(recs);
+    ctx->have_blas   = true;
+    ctx->have_tlas   = false; /* roots moved: the TLAS must be rebuilt */
     return CLHIP_OK;
+}
+
+} // extern "C++"
+
+int clhip_build_blas(clhip_ctx* ctx, const clhip_mesh* meshes, uint32_t num_meshes)
+{
+    if (!ctx || (!meshes && num_meshes))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_build_blas: bad arguments");
+    std::vector<MeshBuild> builds(num_meshes);
+    for (uint32_t m = 0; m < num_meshes; ++m)
+    {
+        const int rc = build_mesh(ctx, meshes[m], m, builds[m]);
+        if (rc)
+            return rc;
+    }
+    ctx->mesh_builds = std::move(builds);
+    return assemble_blas(ctx);
+}
+
+int clhip_build_blas_subset(clhip_ctx* ctx, const clhip_mesh* meshes, const uint32_t* mesh_index, uint32_t n)
+{
+    if (!ctx || (n && (!meshes || !mesh_index)))
+        return fail(ctx, CLHIP_E_INVALID, "clhip_build_blas_subset: bad arguments");
+    std::vector<MeshBuild> builds(n);
+    size_t                 count = ctx->mesh_builds.size();
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        if (mesh_index[k] > count)
+            return fail(ctx, CLHIP_E_INVALID, "mesh index %u: %zu meshes (and the next) exist", mesh_index[k], count);
+        count = std::max<size_t>(count, (size_t)mesh_index[k] + 1);
+        const int rc = build_mesh(ctx, meshes[k], mesh_index[k], builds[k]);
+        if (rc)
+            return rc;
+    }
+    if (count > ctx->mesh_builds.size())
+        ctx->mesh_builds.resize(count);
+    for (uint32_t k = 0; k < n; ++k)
+        ctx->mesh_builds[mesh_index[k]] = std::move(builds[k]);
+    return assemble_blas(ctx);
 }
 
 int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uint32_t* instance_ids,
@@ -1901,6 +2006,7 @@ int clhip_counters(clhip_ctx* ctx, clhip_stats* out)
     out->graph_captures  = ctx->graph_captures;
     out->queue_paths     = ctx->qcap;
     out->scene_bytes     = ctx->node_bytes + ctx->tri_bytes;
+    out->blas_builds     = ctx->blas_builds;
     for (auto& tl : ctx->timed)
     {
         float ms = 0.0f;

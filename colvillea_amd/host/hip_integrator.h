@@ -88,6 +88,21 @@ public:
         check(clhip_build_blas(m_ctx, m.data(), (uint32_t)m.size()));
     }
 
+    /* buildBLAS for the meshes whose BLAS needs a rebuild, as RenderEngine
+     * passes them (renderengine.cpp:64-80): trimeshes[k] is mesh number
+     * meshIndex[k]; the other meshes keep their trees. */
+    void buildBLAS(const std::vector<uint32_t>& meshIndex, const std::vector<TriangleMeshView>& trimeshes)
+    {
+        if (meshIndex.size() != trimeshes.size())
+            throw Error(CLHIP_E_INVALID, "buildBLAS: mismatched array sizes");
+        std::vector<clhip_mesh> m(trimeshes.size());
+        for (size_t i = 0; i < m.size(); ++i)
+            m[i] = clhip_mesh{trimeshes[i].vertices, trimeshes[i].indices, trimeshes[i].normals,
+                              trimeshes[i].tangents, trimeshes[i].uvs,     trimeshes[i].numVertices,
+                              trimeshes[i].numTriangles};
+        check(clhip_build_blas_subset(m_ctx, m.data(), meshIndex.data(), (uint32_t)m.size()));
+    }
+
     /* buildTLAS (integrator.h:49-50): meshOfInstance indexes the BLAS list;
      * transforms (3x4 row-major per instance) may be empty for identity. */
     void buildTLAS(const std::vector<uint32_t>& meshOfInstance, const std::vector<uint32_t>& instanceIDs,

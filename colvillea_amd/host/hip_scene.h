@@ -16,8 +16,9 @@
  *   - entities carry an optional 3x4 object-to-world transform (the
  *     reference has none, its defect D14); editing it marks EntityEdited and
  *     rebuilds the TLAS only;
- *   - the device BLAS build takes the whole mesh list, so any shape change
- *     rebuilds every BLAS (the reference tracks per-mesh BLAS validity);
+ *   - BLAS validity is tracked per mesh, as the reference's TriMeshBLAS
+ *     does (asdataset.h:16-124): a shape edit rebuilds that mesh's tree
+ *     only (clhip_build_blas_subset), then the TLAS;
  *   - textures are uploaded by the engine when TexturesAdded is pending, and
  *     the HDRI dome refers to its texture by the integrator's texture id.
  */
@@ -204,9 +205,13 @@ public:
     }
     /* ShapeEdited: new vertex positions (same count) */
     void setVertices(std::vector<float> verts);
+    /* TriMeshBLAS::needRebuildBLAS (asdataset.h:16-124) */
+    bool needRebuildBLAS() const noexcept { return !m_blasValid; }
 
 private:
+    friend class Scene;
     Scene*                m_scene;
+    bool                  m_blasValid = false;
     std::vector<float>    m_verts;
     std::vector<uint32_t> m_tris;
     std::vector<float>    m_normals, m_tangents, m_uvs;
@@ -239,6 +244,12 @@ struct CompiledEmitterResult
 {
     std::vector<clhip_emitter> emitters;
     int32_t                    domeIndex = -1;
+};
+
+struct BLASBuildSource
+{
+    std::vector<uint32_t>         meshIndex; /* position in the scene's mesh list */
+    std::vector<TriangleMeshView> meshes;
 };
 
 struct TLASBuildSource
@@ -364,17 +375,26 @@ public:
     void setCamera(const Camera& c) { m_camera = c; }
     const Camera& collectCamera() const noexcept { return m_camera; }
 
-    /* collectTriangleMeshForBLASBuilding (scene.cpp:186-210): every mesh
-     * when a shape was added or edited */
-    std::optional<std::vector<TriangleMeshView>> collectTriangleMeshForBLASBuilding() const
+    /* collectTriangleMeshForBLASBuilding (scene.cpp:186-210): when a shape
+     * was added or edited, the meshes whose BLAS needs a rebuild */
+    std::optional<BLASBuildSource> collectTriangleMeshForBLASBuilding() const
     {
-        if (!m_actions.hasAction(Action::ShapeEdited) && !m_actions.hasAction(Action::ShapeAdded) &&
-            !m_actions.hasAction(Action::ShapeRemoved))
+        if (!m_actions.hasAction(Action::ShapeEdited) && !m_actions.hasAction(Action::ShapeAdded))
             return std::nullopt;
-        std::vector<TriangleMeshView> v;
-        for (const auto& m : m_trimeshes)
-            v.push_back(m->view());
-        return v;
+        BLASBuildSource b;
+        for (size_t i = 0; i < m_trimeshes.size(); ++i)
+            if (m_trimeshes[i]->needRebuildBLAS())
+            {
+                b.meshIndex.push_back((uint32_t)i);
+                b.meshes.push_back(m_trimeshes[i]->view());
+            }
+        return b;
+    }
+    /* the BLASes of these meshes are built (TriMeshBLAS state) */
+    void markBLASBuilt(const std::vector<uint32_t>& meshIndex)
+    {
+        for (uint32_t i : meshIndex)
+            m_trimeshes.at(i)->m_blasValid = true;
     }
     /* collectTriangleMeshForTLASBuilding (scene.cpp:212-242): one instance
      * per entity, instance id = entity index; also on entity edits (new
@@ -485,7 +505,8 @@ inline void TriangleMesh::setVertices(std::vector<float> verts)
 {
     if (verts.size() != m_verts.size())
         throw Error(CLHIP_E_INVALID, "setVertices: vertex count changed");
-    m_verts = std::move(verts);
+    m_verts     = std::move(verts);
+    m_blasValid = false;
     m_scene->markEdited(Action::ShapeEdited);
 }
 inline void Entity::setMaterial(std::shared_ptr<Material> m)
@@ -567,7 +588,9 @@ private:
     {
         if (auto b = m_scene->collectTriangleMeshForBLASBuilding())
         {
-            m_integ->buildBLAS(*b);
+            if (!b->meshes.empty())
+                m_integ->buildBLAS(b->meshIndex, b->meshes);
+            m_scene->markBLASBuilt(b->meshIndex);
             m_integ->resetIterationIndex();
         }
         if (auto t = m_scene->collectTriangleMeshForTLASBuilding())

@@ -72,9 +72,10 @@ Camera    camera(uint32_t w, uint32_t h)
 
 struct Built
 {
-    std::shared_ptr<Scene>    scene;
-    std::shared_ptr<Material> floor;
-    std::shared_ptr<Entity>   prop;
+    std::shared_ptr<Scene>        scene;
+    std::shared_ptr<Material>     floor;
+    std::shared_ptr<Entity>       prop;
+    std::shared_ptr<TriangleMesh> prop_mesh;
 };
 
 Built build_scene(uint32_t w, uint32_t h)
@@ -84,6 +85,7 @@ Built build_scene(uint32_t w, uint32_t h)
     Scene& s  = *b.scene;
     auto   g  = ground(s, 5.0f);
     auto   o  = octahedron(s);
+    b.prop_mesh = o;
     b.floor   = s.createDiffuseMaterial({0.6f, 0.6f, 0.55f});
     auto m1   = s.createMetalMaterial({1, 1, 1}, 0.3f, {0.2f, 0.9f, 1.1f}, {3.9f, 2.4f, 2.2f});
     auto m2   = s.createGlassMaterial(0.1f, 1.5f);
@@ -111,7 +113,10 @@ int selftest()
     auto ents = s.compileEntity();
     auto mats = s.compileMaterials();
     auto emit = s.compileEmitters();
-    expect(blas && blas->size() == 2 && (*blas)[1].numTriangles == 8, "BLAS source: both meshes");
+    expect(blas && blas->meshes.size() == 2 && blas->meshes[1].numTriangles == 8 &&
+               blas->meshIndex == std::vector<uint32_t>({0, 1}),
+           "BLAS source: both meshes");
+    s.markBLASBuilt(blas->meshIndex);
     expect(tlas && tlas->meshOfInstance == std::vector<uint32_t>({0, 1, 1}) &&
                tlas->instanceIDs == std::vector<uint32_t>({0, 1, 2}) && tlas->transforms.size() == 36 &&
                tlas->transforms[12 + 3] == -1.0f,
@@ -131,6 +136,11 @@ int selftest()
     auto t2 = s.collectTriangleMeshForTLASBuilding();
     expect(t2 && t2->transforms[12 + 7] == 2.0f && !s.collectTriangleMeshForBLASBuilding(),
            "a transform edit rebuilds the TLAS only");
+    s.resetSceneEditActions();
+    b.prop_mesh->setVertices(std::vector<float>(18, 0.5f));
+    auto b3 = s.collectTriangleMeshForBLASBuilding();
+    expect(b3 && b3->meshIndex == std::vector<uint32_t>({1}) && s.collectTriangleMeshForTLASBuilding(),
+           "a shape edit rebuilds that mesh's BLAS only, then the TLAS");
     std::printf("selftest %s\n", g_fail ? "FAILED" : "ok");
     return g_fail ? 1 : 0;
 }
@@ -174,7 +184,7 @@ int render(int frames)
         const int32_t                 tid = direct->createTexture(tex->data().data(), 64, 32, 3);
         direct->buildMaterials(*s2.compileMaterials());
         direct->buildGeometryEntities(*s2.compileEntity());
-        direct->buildBLAS(*s2.collectTriangleMeshForBLASBuilding());
+        direct->buildBLAS(s2.collectTriangleMeshForBLASBuilding()->meshes);
         auto t = *s2.collectTriangleMeshForTLASBuilding();
         direct->buildTLAS(t.meshOfInstance, t.instanceIDs, t.transforms);
         auto e = *s2.compileEmitters();
@@ -207,6 +217,40 @@ int render(int frames)
     engine->endRendering();
     clhip_get_film_size(integ->native(), nullptr, nullptr, &spp);
     expect(spp == 1, "a transform edit restarts the accumulation");
+
+    /* a shape edit rebuilds one BLAS: the octahedron, flattened, renders as
+     * a fresh context given the edited scene renders it */
+    clhip_stats st{};
+    clhip_counters(integ->native(), &st);
+    const uint64_t builds0 = st.blas_builds;
+    std::vector<float> flat = {1, 0, 0, -1, 0, 0, 0, 0.2f, 0, 0, -0.2f, 0, 0, 0, 1, 0, 0, -1};
+    b.prop_mesh->setVertices(flat);
+    engine->startRendering();
+    engine->endRendering();
+    clhip_counters(integ->native(), &st);
+    expect(st.blas_builds == builds0 + 1, "a shape edit rebuilds exactly one BLAS");
+    std::vector<double> e1(3 * (size_t)W * H), e2(3 * (size_t)W * H);
+    clhip_readback_sums(integ->native(), e1.data(), e1.size() * sizeof(double));
+    {
+        Built b3 = build_scene(W, H);
+        b3.prop->setTransform(place(-1.0f, 3.0f, 0.0f, 0.8f));
+        b3.prop_mesh->setVertices(flat);
+        std::shared_ptr<HipWavefrontPathTracingIntegrator> fresh =
+            HipWavefrontPathTracingIntegrator::createIntegrator(IntegratorType::WavefrontPathTracing, W, H);
+        fresh->setParams(clhip_params{5u, 0u, 0u, 0u});
+        auto e3 = RenderEngine::createRenderEngine(fresh, b3.scene);
+        b3.floor->setReflectance({0.05f, 0.05f, 0.05f});
+        e3->startRendering();
+        e3->endRendering();
+        clhip_readback_sums(fresh->native(), e2.data(), e2.size() * sizeof(double));
+    }
+    double dn = 0, dd = 0;
+    for (size_t i = 0; i < e1.size(); ++i)
+    {
+        dn += (e1[i] - e2[i]) * (e1[i] - e2[i]);
+        dd += e2[i] * e2[i];
+    }
+    expect(dd > 0 && dn <= 1e-20 * dd, "after one BLAS rebuild the image equals a full build's");
     std::printf("frames %d mean %.6f after-edit %.6f\n", frames, m0, m1);
     return g_fail ? 1 : 0;
 }

@@ -189,9 +189,9 @@ class Integrator:
         """4 or 8; call before buildBLAS (no reference counterpart)."""
         self._check(self._lib.clhip_set_bvh_width(self._ctx, int(width)))
 
-    def buildBLAS(self, trimeshes: Sequence[TriangleMesh]):
+    @staticmethod
+    def _mesh_array(trimeshes):
         arr = (capi.clhip_mesh * max(1, len(trimeshes)))()
-        keep = []
         for i, m in enumerate(trimeshes):
             arr[i].vertices = capi.fptr(m.vertices)
             arr[i].indices = capi.uptr(m.triangles)
@@ -200,9 +200,30 @@ class Integrator:
             arr[i].uvs = capi.fptr(m.uvs)
             arr[i].num_vertices = m.vertices.shape[0]
             arr[i].num_triangles = m.triangles.shape[0]
-            keep.append(m)
-        self._check(self._lib.clhip_build_blas(self._ctx, arr, len(trimeshes)))
+        return arr
+
+    def buildBLAS(self, trimeshes: Sequence[TriangleMesh]):
+        """Every mesh's tree (the mesh list of the scene, in order)."""
+        self._check(self._lib.clhip_build_blas(self._ctx, self._mesh_array(trimeshes), len(trimeshes)))
         self._meshes = list(trimeshes)
+
+    def rebuildBLAS(self, meshIndex: Sequence[int], trimeshes: Sequence[TriangleMesh]):
+        """The trees of the meshes whose BLAS needs a rebuild, as RenderEngine
+        passes them (scene.cpp:186-210): trimeshes[k] replaces mesh number
+        meshIndex[k] (or appends it, one past the last); the other meshes keep
+        their trees.  Build the TLAS again afterwards."""
+        idx = np.ascontiguousarray(meshIndex, np.uint32)
+        if len(idx) != len(trimeshes):
+            raise ValueError("rebuildBLAS: one index per mesh")
+        self._check(self._lib.clhip_build_blas_subset(self._ctx, self._mesh_array(trimeshes), capi.uptr(idx),
+                                                      len(idx)))
+        meshes = list(getattr(self, "_meshes", []))
+        for i, m in zip(idx.tolist(), trimeshes):
+            if i == len(meshes):
+                meshes.append(m)
+            else:
+                meshes[i] = m
+        self._meshes = meshes
 
     def buildTLAS(self, trimeshes: Sequence[TriangleMesh], instanceIDs: Sequence[int],
                   transforms: Optional[np.ndarray] = None):
@@ -364,7 +385,7 @@ class Integrator:
         d = {k: np.array(getattr(st, k)[:], np.uint64) for k in ("rays", "hits", "shadow", "visible")}
         for k in ("trace_nodes", "trace_tris", "shadow_nodes", "shadow_tris", "trace_launches", "shadow_launches",
                   "trace_ms", "shade_ms", "shadow_ms", "render_ms", "batches", "graph_replays", "graph_captures",
-                  "queue_paths", "scene_bytes"):
+                  "queue_paths", "scene_bytes", "blas_builds"):
             d[k] = getattr(st, k)
         return d
 

@@ -147,6 +147,7 @@ typedef struct
     uint64_t graph_captures;               /* hipGraph captures made by the last render */
     uint64_t queue_paths;                  /* paths the context's queues hold (the largest batch so far) */
     uint64_t scene_bytes;                  /* bytes of the tree's node and triangle records (TLAS and BLASes) */
+    uint64_t blas_builds;                  /* mesh trees (BLASes) built since the context was created */
 } clhip_stats;
 
 /* ---- context: Integrator::createIntegrator (integrator.cpp:10-22) and
@@ -179,6 +180,14 @@ int clhip_set_bvh_width(clhip_ctx* ctx, uint32_t width);
 /* Integrator::buildBLAS (integrator.h:47; OptiXDevice::buildOptiXAccelBLASes,
  * optixdevice.cpp:136-192): one BVH4 per mesh. */
 int clhip_build_blas(clhip_ctx* ctx, const clhip_mesh* meshes, uint32_t num_meshes);
+
+/* The per-mesh form of Integrator::buildBLAS as RenderEngine drives it
+ * (renderengine.cpp:64-80 with Scene::collectTriangleMeshForBLASBuilding,
+ * scene.cpp:186-210): only the meshes whose BLAS needs a rebuild are passed,
+ * meshes[k] being mesh number mesh_index[k]; the other meshes keep their
+ * trees.  An index one past the current meshes appends a mesh.  The TLAS
+ * must then be built again (clhip_build_tlas), as after clhip_build_blas. */
+int clhip_build_blas_subset(clhip_ctx* ctx, const clhip_mesh* meshes, const uint32_t* mesh_index, uint32_t n);
 
 /* Integrator::buildTLAS (integrator.h:49-50; optixdevice.cpp:194-233): one
  * instance per entity.  instance_ids[i] is the entity index of instance i

@@ -143,3 +143,39 @@ def test_render_before_scene_is_built_reports_state_error():
         with pytest.raises(capi.ClhipError, match="TLAS"):
             integ.render(1)
         assert integ.iterationIndex() == 0
+
+
+@pytest.mark.parametrize("width", [4, 8], ids=["bvh4", "bvh8"])
+def test_rebuilding_one_blas_equals_a_full_build(width):
+    """An edit to one mesh rebuilds that mesh's tree only (per-mesh BLAS
+    validity, scene.cpp:186-210 / asdataset.h:16-124); the render then equals
+    a fresh context given the edited scene, and the other trees were not
+    built again."""
+    import dataclasses
+
+    from colvillea_amd.integrator import TriangleMesh
+
+    desc = scenes.make_c0(width=48, height=32, spp=2, hdri_w=64, hdri_h=32)
+    s = desc.meshes[1]
+    squashed = TriangleMesh((s.vertices * np.array([1.0, 0.4, 1.0], np.float32)).astype(np.float32), s.triangles,
+                            normals=s.normals)
+    edited = dataclasses.replace(desc, meshes=[desc.meshes[0], squashed])
+    with Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height) as a:
+        a.setBVHWidth(width)
+        upload_scene(a, desc)
+        before = a.counters()["blas_builds"]
+        a.rebuildBLAS([1], [squashed])
+        a.buildTLAS([edited.meshes[m] for m in edited.instance_mesh], list(range(len(edited.instance_mesh))),
+                    edited.instance_xform)
+        a.resetIterationIndex()
+        a.render(desc.spp)
+        ca, sa = a.counters(), a.readbackSums()
+    with Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height) as b:
+        b.setBVHWidth(width)
+        upload_scene(b, edited)
+        b.render(desc.spp)
+        cb, sb = b.counters(), b.readbackSums()
+    assert before == 2 and ca["blas_builds"] == 3
+    for k in ("rays", "hits", "shadow", "visible"):
+        assert np.array_equal(ca[k], cb[k]), k
+    assert np.allclose(sa, sb, rtol=1e-12, atol=1e-12)
