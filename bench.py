@@ -181,9 +181,9 @@ def batched_views(integ, args) -> dict:
 # guided line.  The schedule -- recorded paths, steps per batch, a discarded
 # warm-up, the freeze point -- was picked by GPU sweeps over three network
 # initialisations at these fixed hyperparameters.
-GUIDE_SPEC = {"bsdf_fraction": 0.5, "train": 1, "record_paths": 1 << 20, "batch_size": 1 << 16,
-              "learning_rate": 1e-3, "steps_per_batch": 8, "max_steps": 512, "warm_spp": 8,
-              "warm_steps_per_batch": 32}
+GUIDE_SPEC = {"bsdf_fraction": 0.5, "train": 1, "record_paths": 1 << 21, "batch_size": 1 << 16,
+              "learning_rate": 1e-3, "steps_per_batch": 16, "max_steps": 1024, "warm_spp": 8,
+              "warm_steps_per_batch": 64}
 # A faster-learning schedule found by the same sweeps (lr 3e-3, 2^15 samples
 # per step), reported beside the headline.
 GUIDE_TUNED = {"bsdf_fraction": 0.5, "train": 1, "record_paths": 1 << 20, "batch_size": 1 << 15,

@@ -139,6 +139,7 @@ class clhip_guide(C.Structure):
         ("max_steps", C.c_uint32),
         ("precision", C.c_uint32),
         ("steps_per_batch", C.c_uint32),
+        ("train_kernel", C.c_uint32),
     ]
 
 

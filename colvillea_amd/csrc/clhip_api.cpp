@@ -184,7 +184,8 @@ struct clhip_ctx
     clhip_guide guide{};
     bool        guide_ready = false;
     DevBuf      g_params, g_m1, g_m2, g_W, g_WT, g_img, g_gc, g_img8, g_scl;
-    DevBuf      g_X, g_act[4], g_out, g_grad[5], g_loss, g_wpart, g_amaxp, g_partial, g_samples;
+    DevBuf      g_X, g_act[4], g_out, g_grad[5], g_loss, g_wpart, g_spart, g_amaxp, g_partial, g_samples;
+    uint32_t    g_partcap = 0; /* blocks the per-block gradient buffers hold */
     uint32_t    g_N = 0;
     DevBuf      g_V, g_E, g_tag, g_gcount, g_gblock;
     uint32_t    g_slots = 0;
@@ -489,9 +490,15 @@ int guide_train_buffers(clhip_ctx* c, uint32_t N, uint32_t nsteps = 1)
         CLHIP_CHECK(c, c->g_grad[l].alloc((size_t)N * CLPT_G_HID * sizeof(__bf16)));
     CLHIP_CHECK(c, c->g_out.alloc((size_t)N * CLPT_G_HID * sizeof(float)));
     CLHIP_CHECK(c, c->g_loss.alloc((size_t)(N / 256) * sizeof(float)));
-    CLHIP_CHECK(c, c->g_wpart.alloc((size_t)(N / 256) * sizeof(double)));
+    /* per-block gradients: N / 1024 chunks per GEMM, or the fused step's
+     * blocks (at most one per 128 samples and at most the target count, so
+     * this also covers every smaller N) */
+    const uint32_t nblk = std::max(N / 1024u, std::min(N / 128u, std::max(1u, c->num_cus)));
+    CLHIP_CHECK(c, c->g_wpart.alloc((size_t)std::max(N / 256u, nblk) * sizeof(double)));
+    CLHIP_CHECK(c, c->g_spart.alloc((size_t)nblk * 5u * sizeof(float)));
     CLHIP_CHECK(c, c->g_amaxp.alloc((size_t)4 * (N / 64) * sizeof(float)));
-    CLHIP_CHECK(c, c->g_partial.alloc((size_t)(N / 1024) * CLPT_G_PARAMS * sizeof(float)));
+    CLHIP_CHECK(c, c->g_partial.alloc((size_t)nblk * CLPT_G_PARAMS * sizeof(float)));
+    c->g_partcap = nblk;
     CLHIP_CHECK(c, c->g_samples.alloc((size_t)N * std::max(1u, nsteps) * sizeof(GuideSample)));
     c->g_N = N;
     return CLHIP_OK;
@@ -573,6 +580,9 @@ clpt::GuideTrainBufs guide_bufs(clhip_ctx* c)
     b.img       = c->g_img.as<__bf16>();
     b.img8      = c->guide.precision == 1u ? c->g_img8.as<uint8_t>() : nullptr; /* FP8 image only when used */
     b.scl       = c->g_scl.as<float>();
+    b.spart     = c->g_spart.as<float>();
+    b.fused_blocks = c->guide.train_kernel == 0u ? c->num_cus : 0u;
+    b.part_cap     = c->g_partcap;
     return b;
 }
 
@@ -2251,7 +2261,7 @@ int clhip_copy_probe(clhip_ctx* ctx, size_t bytes, uint32_t iters, double* ms_pe
     hipEvent_t e0, e1;
     CLHIP_CHECK(ctx, hipEventCreate(&e0));
     CLHIP_CHECK(ctx, hipEventCreate(&e1));
-    const uint32_t blocks = ctx->num_cus * 8;
+    const uint32_t blocks = ctx->num_cus * 32;
     CLHIP_CHECK(ctx, clpt::launch_copy_probe(ctx->stream, a.as<const float4>(), b.as<float4>(), n, blocks));
     CLHIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
     for (uint32_t i = 0; i < iters; ++i)
@@ -2306,6 +2316,8 @@ int clhip_set_guide(clhip_ctx* ctx, const clhip_guide* g)
         return fail(ctx, CLHIP_E_INVALID, "steps_per_batch %u above 64", v.steps_per_batch);
     if (v.precision > 1u)
         return fail(ctx, CLHIP_E_INVALID, "precision %u: 0 (bfloat16) or 1 (FP8)", v.precision);
+    if (v.train_kernel > 1u)
+        return fail(ctx, CLHIP_E_INVALID, "train_kernel %u: 0 (fused) or 1 (per GEMM)", v.train_kernel);
     if (!(v.learning_rate > 0.0f))
         return fail(ctx, CLHIP_E_INVALID, "learning_rate must be positive");
     const bool fp8_on = v.precision == 1u && ctx->guide.precision != 1u;

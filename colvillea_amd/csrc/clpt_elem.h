@@ -113,11 +113,9 @@ CLPT_FN int clpt_isnan(float x) { return x != x; }
 /* sin and cos of x in one reduction. */
 CLPT_FN void clpt_sincosf(float x, float* s_out, float* c_out)
 {
-    if (clpt_isnan(x) || fabsf(x) == __builtin_huge_valf())
-    {
-        *s_out = *c_out = clpt_u2f(0x7fc00000u);
-        return;
-    }
+    const int bad = clpt_isnan(x) || fabsf(x) == __builtin_huge_valf();
+    if (bad)
+        x = 0.0f;
     const float j = rintf(x * CLPT_2_OVER_PI);
     const int   q = (int)j;
     float       r = x - j * CLPT_PIO2_C1;
@@ -128,13 +126,13 @@ CLPT_FN void clpt_sincosf(float x, float* s_out, float* c_out)
     const float s  = r + (r * z) * sp;
     const float cp = CLPT_COS_C0 + z * (CLPT_COS_C1 + z * (CLPT_COS_C2 + z * CLPT_COS_C3));
     const float c  = (1.0f - 0.5f * z) + (z * z) * cp;
-    switch (q & 3)
-    {
-        case 0: *s_out = s; *c_out = c; break;
-        case 1: *s_out = c; *c_out = -s; break;
-        case 2: *s_out = -s; *c_out = -c; break;
-        default: *s_out = -c; *c_out = s; break;
-    }
+    /* quadrant q & 3: (s, c), (c, -s), (-s, -c), (-c, s), as selects */
+    const int   swap = (q & 1) != 0;
+    const float s1   = swap ? c : s;
+    const float c1   = swap ? s : c;
+    const float nan  = clpt_u2f(0x7fc00000u);
+    *s_out           = bad ? nan : ((q & 2) != 0 ? -s1 : s1);
+    *c_out           = bad ? nan : (((q + 1) & 2) != 0 ? -c1 : c1);
 }
 
 CLPT_FN float clpt_sinf(float x)

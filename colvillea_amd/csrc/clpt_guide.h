@@ -61,6 +61,7 @@ CLPT_HD inline uint32_t guide_perm(uint32_t k) { return (k & ~12u) | ((k & 4u) <
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float  f32x16 __attribute__((ext_vector_type(16)));
+typedef float  f32x8 __attribute__((ext_vector_type(8)));
 
 /* ------------------------------------------------------------ encoding */
 
@@ -117,11 +118,12 @@ CLPT_D void guide_group(const float v[9], float o[8])
 
 CLPT_D bf16x8 guide_pack(const float o[8])
 {
-    bf16x8 b;
+    /* a vector conversion: four packed v_cvt_pk_bf16_f32 */
+    f32x8 v;
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-        b[k] = (__bf16)o[k];
-    return b;
+        v[k] = o[k];
+    return __builtin_convertvector(v, bf16x8);
 }
 
 /* Group 2t+h of query v, h = lane half (0 or 1): one argument reduction per

@@ -63,7 +63,15 @@ struct GuideTrainBufs
     __bf16*        img;
     uint8_t*       img8; /* FP8 image refreshed after every step when not NULL */
     float*         scl;
+    float*         spart;        /* fused step: per-block loss and activation maxima (5 floats) */
+    uint32_t       fused_blocks; /* fused step with about this many blocks; 0 = one launch per GEMM */
+    uint32_t       part_cap;     /* blocks that partial, wpart and spart hold */
 };
+/* Blocks and passes per block of the fused step for N samples and a target
+ * block count; the per-block buffers (partial, wpart, spart) need that many
+ * entries, at most CLPT_GUIDE_FUSED_MAXBLK. */
+#define CLPT_GUIDE_FUSED_MAXBLK 1024u
+uint32_t guide_fused_blocks(uint32_t N, uint32_t target, uint32_t* iters);
 /* bf16 inference, or FP8 when img8 is not NULL (scales scl) */
 /* The allocation-free trace entry point: its fixed grid, the per-device
  * address of its scratch (the stacks' overflow area), and the launch. */

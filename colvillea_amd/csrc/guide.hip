@@ -32,6 +32,83 @@ __device__ __forceinline__ uint32_t guide_count(const GuideCounters* gc, uint32_
     return c > step ? (c - step + nsteps - 1u) / nsteps : 0u;
 }
 
+/* Elementary functions of the training loss: the hardware base-2
+ * exponential and logarithm, reciprocal and reciprocal square root (about
+ * one ulp).  The loss and its gradient need no bit-exact agreement with the
+ * CPU restatement, unlike the mixture the tracer samples. */
+__device__ __forceinline__ float gl_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ float gl_log(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+
+/* The terms of one vMF lobe for direction wi, from its 5 outputs o5
+ * (logit, unnormalised mean, log concentration) and the logit maximum m:
+ * a = e^{logit - m}, the normalised mean, kappa, e = e^{-2 kappa}, mu.wi,
+ * 1/|raw mean| (0 for a vanishing mean, whose direction is +z) and
+ * q = kappa / (2 pi (1 - e)) e^{kappa (mu.wi - 1)}, the lobe's density. */
+struct GuideLobe
+{
+    float a, q, k, e, cmu, inv, mx, my, mz;
+};
+
+__device__ __forceinline__ void guide_lobe(const float* o5, float m, f3 wi, GuideLobe& L)
+{
+    L.a            = gl_exp(o5[0] - m);
+    const float r2 = o5[1] * o5[1] + o5[2] * o5[2] + o5[3] * o5[3];
+    const bool  ok = r2 > 1e-12f;
+    L.inv          = ok ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+    L.mx           = ok ? o5[1] * L.inv : 0.0f;
+    L.my           = ok ? o5[2] * L.inv : 0.0f;
+    L.mz           = ok ? o5[3] * L.inv : 1.0f;
+    L.k            = gl_exp(fminf(fmaxf(o5[4], CLPT_G_KMIN), CLPT_G_KMAX));
+    L.e            = gl_exp(-2.0f * L.k);
+    L.cmu          = L.mx * wi.x + L.my * wi.y + L.mz * wi.z;
+    L.q            = L.k * __builtin_amdgcn_rcpf((2.0f * CLPT_PI_F) * (1.0f - L.e)) * gl_exp(L.k * (L.cmu - 1.0f));
+}
+
+/* Gradient of -wn log p w.r.t. the lobe's 5 outputs, for its mixture
+ * weight pi and responsibility r = pi q / p. */
+__device__ __forceinline__ void guide_lobe_grad(const float* o5, const GuideLobe& L, float pi, float r, float wn, f3 wi,
+                                                float* g5)
+{
+    g5[0]          = -wn * (r - pi);
+    const float s1 = -wn * r * L.k * L.inv;
+    g5[1]          = s1 * (wi.x - L.cmu * L.mx);
+    g5[2]          = s1 * (wi.y - L.cmu * L.my);
+    g5[3]          = s1 * (wi.z - L.cmu * L.mz);
+    const float dl = __builtin_amdgcn_rcpf(L.k) - 2.0f * L.e * __builtin_amdgcn_rcpf(1.0f - L.e) + (L.cmu - 1.0f);
+    g5[4]          = o5[4] > CLPT_G_KMIN && o5[4] < CLPT_G_KMAX ? -wn * r * dl * L.k : 0.0f;
+}
+
+/* The weighted negative log-likelihood -wn log p(wi) of the vMF mixture
+ * with network outputs o, and its gradient w.r.t. the 40 outputs (g). */
+__device__ __forceinline__ float guide_nll_grad(const float o[CLPT_G_NOUT], f3 wi, float wn, float g[CLPT_G_NOUT])
+{
+    float m = o[0];
+#pragma unroll
+    for (int j = 1; j < CLPT_G_LOBES; ++j)
+        m = fmaxf(m, o[5 * j]);
+    GuideLobe L[CLPT_G_LOBES];
+    float     S = 0.0f;
+#pragma unroll
+    for (int j = 0; j < CLPT_G_LOBES; ++j)
+    {
+        guide_lobe(o + 5 * j, m, wi, L[j]);
+        S += L[j].a;
+    }
+    const float iS = __builtin_amdgcn_rcpf(S);
+    float       p  = 0.0f;
+#pragma unroll
+    for (int j = 0; j < CLPT_G_LOBES; ++j)
+        p += (L[j].a * iS) * L[j].q;
+    const float ip = p > 0.0f ? __builtin_amdgcn_rcpf(p) : 0.0f;
+#pragma unroll
+    for (int j = 0; j < CLPT_G_LOBES; ++j)
+    {
+        const float pi = L[j].a * iS;
+        guide_lobe_grad(o + 5 * j, L[j], pi, pi * L[j].q * ip, wn, wi, g + 5 * j);
+    }
+    return -wn * gl_log(fmaxf(p, 1e-30f));
+}
+
 } // namespace
 
 /* ------------------------------------------------------------ inference */
@@ -359,56 +436,24 @@ __global__ __launch_bounds__(256) void k_guide_loss(const float* __restrict__ ou
     const float wsum = (float)wred[0];
     if (n == 0)
         gc->wsum = wsum;
-    float            g[CLPT_G_HID];
+    float g[CLPT_G_HID];
 #pragma unroll
-    for (int k = 0; k < CLPT_G_HID; ++k)
+    for (int k = CLPT_G_NOUT; k < CLPT_G_HID; ++k)
         g[k] = 0.0f;
     float loss = 0.0f;
     if (n < cnt && wsum > 0.0f)
     {
         const GuideSample smp = s[(size_t)n * nsteps];
-        const float       wn  = smp.d.x / wsum;
-        const f3          wi{smp.c.y, smp.c.z, smp.c.w};
         float             o[CLPT_G_NOUT];
 #pragma unroll
         for (int k = 0; k < CLPT_G_NOUT; ++k)
             o[k] = out[(size_t)n * CLPT_G_HID + k];
-        VmfMix m;
-        vmf_mixture(o, m);
-        float v[CLPT_G_LOBES], p = 0.0f;
-#pragma unroll
-        for (int j = 0; j < CLPT_G_LOBES; ++j)
-        {
-            v[j] = m.norm[j] * clpt_expf(m.kappa[j] * (dot3(m.mu[j], wi) - 1.0f));
-            p += v[j];
-        }
-        loss = -wn * clpt_logf(fmaxf(p, 1e-30f));
-        const float ip = p > 0.0f ? 1.0f / p : 0.0f;
-#pragma unroll
-        for (int j = 0; j < CLPT_G_LOBES; ++j)
-        {
-            const float r   = v[j] * ip; /* responsibility of lobe j */
-            const float k   = m.kappa[j];
-            const float cmu = dot3(m.mu[j], wi);
-            g[5 * j]        = -wn * (r - m.w[j]);
-            const f3    raw{o[5 * j + 1], o[5 * j + 2], o[5 * j + 3]};
-            const float r2 = dot3(raw, raw);
-            if (r2 > 1e-12f)
-            {
-                const float s1 = -wn * r * k / sqrtf(r2);
-                g[5 * j + 1]   = s1 * (wi.x - cmu * m.mu[j].x);
-                g[5 * j + 2]   = s1 * (wi.y - cmu * m.mu[j].y);
-                g[5 * j + 3]   = s1 * (wi.z - cmu * m.mu[j].z);
-            }
-            const float sr = o[5 * j + 4];
-            if (sr > CLPT_G_KMIN && sr < CLPT_G_KMAX)
-            {
-                const float e  = clpt_expf(-2.0f * k);
-                const float dl = 1.0f / k - 2.0f * e / (1.0f - e) + (cmu - 1.0f);
-                g[5 * j + 4]   = -wn * r * dl * k;
-            }
-        }
+        loss = guide_nll_grad(o, f3{smp.c.y, smp.c.z, smp.c.w}, smp.d.x / wsum, g);
     }
+    else
+#pragma unroll
+        for (int k = 0; k < CLPT_G_NOUT; ++k)
+            g[k] = 0.0f;
     if (n < N)
     {
         bf16x8* row = reinterpret_cast<bf16x8*>(G5 + (size_t)n * CLPT_G_HID);
@@ -548,6 +593,499 @@ __global__ __launch_bounds__(256) void k_guide_finish(GuideCounters* gc, const f
             gc->samples_total += gc->trained;
         }
         if (last)
+            gc->count = 0u;
+    }
+}
+
+/* ------------------------------------------------------------ fused training step */
+
+/* One Adam step in two launches instead of one per GEMM.
+ *
+ * k_guide_train_fused: CLPT_GT_WAVES waves per block, 32 samples per wave,
+ * CLPT_GT_S samples per pass; a block makes `iters` passes over consecutive
+ * samples.  The forward image (W_l^T rows), the backward image (W_l rows,
+ * layers 1..4, output index in the chained order) and a staging area stay in
+ * LDS.  Per pass and wave, all in the chained register layout of the fused
+ * forward pass (a sample per lane pair, units in registers):
+ *   encode -> 5 forward GEMMs (activations kept as bf16 operand fragments)
+ *   -> mixture loss and its gradient G_4 (the pair exchanges its halves of
+ *   the 40 outputs) -> G_{l-1} = (W_l G_l) . [A_{l-1} > 0] for l = 4..1.
+ * Weight gradients dW_l = A_{l-1} G_l^T contract over samples, so the
+ * operands are staged in LDS as [unit][sample] (all waves' samples), and
+ * wave w accumulates one 32 x 32 tile of every layer (of every 64-feature
+ * chunk of layer 0), adding each pass to the block's gradient.  Sample weights
+ * are normalised by the block's weight sum W_b; the block writes its
+ * gradient, W_b, its loss and its activation maxima.
+ *
+ * k_guide_adam_fused: the gradient sum_b (W_b / W) P_b in block order, Adam,
+ * the bfloat16 copies; its block 0 closes the step (loss, maxima, weight
+ * sum).  The fused kernel advances the step counters.  Both kernels are
+ * deterministic and use no atomics. */
+#define CLPT_GT_WAVES 4
+#define CLPT_GT_S (32 * CLPT_GT_WAVES)
+#define CLPT_GT_SS (CLPT_GT_S + 8)
+#define CLPT_GT_SCAL 5
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+/* One 32 x 32 tile of a weight gradient over the pass's samples: rows
+ * 32 fr.. of staging array A times rows 32 orr.. of staging array B
+ * (both [unit][sample]), added to the block's gradient.  The block's
+ * gradient is reached through a buffer resource (the 32-bit buffer
+ * operations move the float's bits): `lofs` is the lane's byte offset in
+ * the tile, the uniform parts (block, layer, tile corner) are `sofs`, so
+ * the addresses cost no registers per element.  guide_dw_load fetches the
+ * tile's value after the earlier passes (zero on the first) ahead of the
+ * barrier; guide_dw_tile adds the pass and stores, so a tile is summed over
+ * the passes in pass order. */
+__device__ __forceinline__ f32x16 guide_dw_load(__amdgpu_buffer_rsrc_t res, uint32_t lofs, uint32_t sofs, bool add)
+{
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+        acc[r] = add ? __uint_as_float(
+                           __builtin_amdgcn_raw_buffer_load_b32(res, lofs + (8u * (r / 4) + r % 4) * 256u, sofs, 0))
+                     : 0.0f;
+    return acc;
+}
+
+__device__ __forceinline__ void guide_dw_tile(const __bf16 (*A)[CLPT_GT_SS], const __bf16 (*B)[CLPT_GT_SS], uint32_t fr,
+                                              uint32_t orr, uint32_t c, uint32_t h, __amdgpu_buffer_rsrc_t res,
+                                              uint32_t lofs, uint32_t sofs, f32x16 acc)
+{
+#pragma unroll
+    for (int kt = 0; kt < CLPT_GT_S / 16; ++kt)
+    {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&A[32u * fr + c][16 * kt + 8u * h]);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&B[32u * orr + c][16 * kt + 8u * h]);
+        acc            = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(acc[r]), res, lofs + (8u * (r / 4) + r % 4) * 256u,
+                                              sofs, 0);
+}
+
+/* Operand fragment of encoding group 2T + h. */
+template <int T>
+__device__ __forceinline__ bf16x8 guide_feature_frag(const float v[9], bool hb)
+{
+    float o[8];
+    guide_group_sel<T>(v, hb, o);
+    return guide_pack(o);
+}
+
+__global__ __launch_bounds__(256) void k_guide_train_fused(const GuideSample* __restrict__ s, GuideCounters* gc,
+                                                           uint32_t step, uint32_t nsteps, uint32_t N, uint32_t iters,
+                                                           const __bf16* __restrict__ img,
+                                                           const __bf16* __restrict__ Wrm, float* __restrict__ partial,
+                                                           double* __restrict__ wpart, float* __restrict__ spart)
+{
+    __shared__ __align__(16) __bf16 lw[CLPT_G_IMG];
+    __shared__ __align__(16) __bf16 lb[4 * CLPT_G_HID * CLPT_G_S1];
+    __shared__ __align__(16) __bf16 st[2][CLPT_G_HID][CLPT_GT_SS];
+    __shared__ double               dred[CLPT_GT_WAVES];
+    __shared__ float                fred[CLPT_GT_WAVES][CLPT_GT_SCAL];
+
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = __lane_id(), c = lane & 31u, h = lane >> 5;
+    const bool     hb    = h != 0u;
+    const uint32_t cnt   = guide_count(gc, step, nsteps, N);
+    const uint32_t first = blockIdx.x * iters * CLPT_GT_S;
+    const uint32_t stop  = min(first + iters * CLPT_GT_S, cnt);
+
+    {
+        /* every load of a thread issued before its LDS writes */
+        constexpr uint32_t NV = CLPT_G_IMG / 8u, NW = NV / 256u; /* 17 full rounds and a partial one */
+        const uint4*       src = reinterpret_cast<const uint4*>(img);
+        uint4*             dst = reinterpret_cast<uint4*>(lw);
+        uint4              wv[NW];
+#pragma unroll
+        for (uint32_t q = 0; q < NW; ++q)
+            wv[q] = src[tid + 256u * q];
+        const bool  tail = tid + 256u * NW < NV;
+        const uint4 wt   = tail ? src[tid + 256u * NW] : uint4{};
+#pragma unroll
+        for (uint32_t q = 0; q < NW; ++q)
+            dst[tid + 256u * q] = wv[q];
+        if (tail)
+            dst[tid + 256u * NW] = wt;
+        /* lb[l-1][i][16t + 8u + s] = W_l[i][perm(16t + 8u + s)]: the two runs
+         * of 4 consecutive outputs 16t + 4u and 16t + 8 + 4u */
+        bf16x4 lo[8], hi[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8u; ++q)
+        {
+            const uint32_t k = tid + 256u * q, l = k / 512u, i = (k / 8u) % 64u, t = (k % 8u) >> 1, u = k & 1u;
+            const __bf16*  row = Wrm + guide_layer_offset((int)l + 1) + i * CLPT_G_HID;
+            lo[q]              = *reinterpret_cast<const bf16x4*>(row + 16u * t + 4u * u);
+            hi[q]              = *reinterpret_cast<const bf16x4*>(row + 16u * t + 8u + 4u * u);
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 8u; ++q)
+        {
+            const uint32_t k = tid + 256u * q, l = k / 512u, i = (k / 8u) % 64u, t = (k % 8u) >> 1, u = k & 1u;
+            bf16x8         v;
+            v[0] = lo[q][0]; v[1] = lo[q][1]; v[2] = lo[q][2]; v[3] = lo[q][3];
+            v[4] = hi[q][0]; v[5] = hi[q][1]; v[6] = hi[q][2]; v[7] = hi[q][3];
+            *reinterpret_cast<bf16x8*>(lb + (l * CLPT_G_HID + i) * CLPT_G_S1 + 16u * t + 8u * u) = v;
+        }
+    }
+    /* W_b: the block's sample weights, summed in a fixed order */
+    double wl = 0.0;
+    for (uint32_t k = first + tid; k < stop; k += blockDim.x)
+        wl += (double)s[(size_t)k * nsteps].d.x;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        wl += __shfl_xor(wl, off);
+    if (lane == 0u)
+        dred[w] = wl;
+    __syncthreads();
+    const double wb = ((dred[0] + dred[1]) + dred[2]) + dred[3];
+
+    /* this wave's tile (input rows 32 fr.., output columns 32 orr..) of
+     * every layer's gradient and of every 64-feature chunk of layer 0 */
+    const uint32_t fr = w >> 1, orr = w & 1u;
+    const __amdgpu_buffer_rsrc_t pres = __builtin_amdgcn_make_buffer_rsrc(
+        partial + (size_t)blockIdx.x * CLPT_G_PARAMS, (short)0, CLPT_G_PARAMS * 4, 0x00020000);
+    const uint32_t lofs  = ((32u * fr + 4u * h) * CLPT_G_HID + 32u * orr + c) * 4u;
+    float          lsum = 0.0f, amx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (uint32_t it = 0; it < iters; ++it)
+    {
+        const uint32_t k    = first + it * CLPT_GT_S + 32u * w + c;
+        const bool     live = k < cnt;
+        float          v[9];
+        f3             wi{0.0f, 0.0f, 1.0f};
+        float          wn = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 9; ++d)
+            v[d] = 0.0f;
+        if (live)
+        {
+            const GuideSample g = s[(size_t)k * nsteps];
+            v[0] = g.a.x; v[1] = g.a.y; v[2] = g.a.z; v[3] = g.a.w;
+            v[4] = g.b.x; v[5] = g.b.y; v[6] = g.b.z; v[7] = g.b.w;
+            v[8] = g.c.x;
+            wi   = f3{g.c.y, g.c.z, g.c.w};
+            wn   = wb > 0.0 ? (float)((double)g.d.x / wb) : 0.0f;
+        }
+        /* forward, layer 0: features of group 2t + h, kept for the weight
+         * gradient.  An idle sample (zero query) has zero loss gradient, so
+         * it adds nothing to any weight gradient; it is left out of the
+         * activation maxima. */
+        bf16x8 xf[16];
+        f32x16 acc[2] = {guide_zero16(), guide_zero16()};
+#define CLPT_GT_L0(T)                                                                                              \
+    {                                                                                                              \
+        xf[T] = guide_feature_frag<T>(v, hb);                                                                \
+        _Pragma("unroll") for (int rt = 0; rt < 2; ++rt)                                                           \
+        {                                                                                                          \
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(lw + (32u * rt + c) * CLPT_G_S0 + 16u * (T) + 8u * h); \
+            acc[rt]        = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xf[T], acc[rt], 0, 0, 0);                  \
+        }                                                                                                          \
+    }
+        CLPT_GT_L0(0) CLPT_GT_L0(1) CLPT_GT_L0(2) CLPT_GT_L0(3) CLPT_GT_L0(4) CLPT_GT_L0(5) CLPT_GT_L0(6)
+        CLPT_GT_L0(7) CLPT_GT_L0(8) CLPT_GT_L0(9) CLPT_GT_L0(10) CLPT_GT_L0(11) CLPT_GT_L0(12) CLPT_GT_L0(13)
+        CLPT_GT_L0(14) CLPT_GT_L0(15)
+#undef CLPT_GT_L0
+        /* hidden layers; af[l] = relu(Z_l) as the operand fragments of layer l+1 */
+        bf16x8 af[4][4];
+#pragma unroll
+        for (int l = 1; l <= 4; ++l)
+        {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+            {
+                float o[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                {
+                    o[q]       = fmaxf(acc[t / 2][8 * (t % 2) + q], 0.0f);
+                    amx[l - 1] = fmaxf(amx[l - 1], live ? o[q] : 0.0f);
+                }
+                af[l - 1][t] = guide_pack(o);
+            }
+            acc[0] = guide_zero16();
+            acc[1] = guide_zero16();
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt)
+                {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(
+                        lw + guide_img_offset(l) + (32u * rt + c) * CLPT_G_S1 + 16u * t + 8u * h);
+                    acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, af[l - 1][t], acc[rt], 0, 0, 0);
+                }
+        }
+        /* The loss, its 8 lobes split over the lane pair: half h takes lobes
+         * 4h..4h+3, outputs 20h + t.  Units t with bit 2 clear are in this
+         * lane's registers for both halves; the others come from the partner,
+         * which holds them at the register of the other half's unit. */
+        float ol[20];
+#pragma unroll
+        for (int t = 0; t < 20; ++t)
+        {
+            const float v0 = acc[t / 32][4 * ((t % 32) / 8) + t % 4];
+            const float v1 = acc[(20 + t) / 32][4 * (((20 + t) % 32) / 8) + (20 + t) % 4];
+            if (((t >> 2) & 1) == 0)
+                ol[t] = hb ? v1 : v0;
+            else
+                ol[t] = __shfl_xor(hb ? v0 : v1, 32);
+        }
+        GuideLobe L[4];
+        float     m = fmaxf(fmaxf(ol[0], ol[5]), fmaxf(ol[10], ol[15]));
+        m           = fmaxf(m, __shfl_xor(m, 32));
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            guide_lobe(ol + 5 * j, m, wi, L[j]);
+        /* sums over all 8 lobes, lobes 0..3 first in both halves */
+        const float sa = ((L[0].a + L[1].a) + L[2].a) + L[3].a, so = __shfl_xor(sa, 32);
+        const float iS = __builtin_amdgcn_rcpf(hb ? so + sa : sa + so);
+        float       pa = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            pa += (L[j].a * iS) * L[j].q;
+        const float po = __shfl_xor(pa, 32);
+        const float p  = hb ? po + pa : pa + po;
+        const float ip = p > 0.0f ? __builtin_amdgcn_rcpf(p) : 0.0f;
+        float       gl[20];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const float pi = L[j].a * iS;
+            guide_lobe_grad(ol + 5 * j, L[j], pi, pi * L[j].q * ip, wn, wi, gl + 5 * j);
+        }
+        lsum += hb ? 0.0f : -wn * gl_log(fmaxf(p, 1e-30f));
+        /* each half sends the gradients of units 4..7, 12..15 of its range */
+        float gr[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            gr[q] = __shfl_xor(gl[q < 4 ? 4 + q : 8 + q], 32);
+        /* G_4 fragments: element q of fragment t is unit 16t + 8(q/4) + 4h + q%4 */
+        bf16x8 gf[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+            {
+                const int u0 = 16 * t + 8 * (q / 4) + q % 4, u1 = u0 + 4;
+                const float g0 = u0 < 20 ? gl[u0 < 20 ? u0 : 0]
+                               : (u0 >= 24 && u0 <= 27) ? gr[(u0 >= 24 && u0 <= 27) ? u0 - 24 : 0]
+                               : (u0 >= 32 && u0 <= 35) ? gr[(u0 >= 32 && u0 <= 35) ? u0 - 28 : 0] : 0.0f;
+                const float g1 = (u1 >= 20 && u1 < 40) ? gl[(u1 >= 20 && u1 < 40) ? u1 - 20 : 0]
+                               : (u1 >= 4 && u1 <= 7)  ? gr[(u1 >= 4 && u1 <= 7) ? u1 - 4 : 0]
+                               : (u1 >= 12 && u1 <= 15) ? gr[(u1 >= 12 && u1 <= 15) ? u1 - 8 : 0] : 0.0f;
+                gf[t][q] = (__bf16)(hb ? g1 : g0);
+            }
+        /* backward through layers 4..1 */
+#pragma unroll
+        for (int l = 4; l >= 1; --l)
+        {
+            const f32x16 prev = guide_dw_load(pres, lofs, guide_layer_offset(l) * 4u, it > 0u);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                {
+                    const uint32_t u = 16u * t + 8u * (q / 4) + 4u * h + q % 4;
+                    st[0][u][32u * w + c] = af[l - 1][t][q];
+                    st[1][u][32u * w + c] = gf[t][q];
+                }
+            f32x16 bacc[2] = {guide_zero16(), guide_zero16()};
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt)
+                {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(
+                        lb + ((l - 1) * CLPT_G_HID + 32u * rt + c) * CLPT_G_S1 + 16u * t + 8u * h);
+                    bacc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, gf[t], bacc[rt], 0, 0, 0);
+                }
+            __syncthreads();
+            guide_dw_tile(st[0], st[1], fr, orr, c, h, pres, lofs, guide_layer_offset(l) * 4u, prev);
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+            {
+                float o[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    o[q] = (float)af[l - 1][t][q] > 0.0f ? bacc[t / 2][8 * (t % 2) + q] : 0.0f;
+                gf[t] = guide_pack(o);
+            }
+        }
+        /* layer 0: G_0 once, the features chunk by chunk (64 features) */
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                st[1][16u * t + 8u * (q / 4) + 4u * h + q % 4][32u * w + c] = gf[t][q];
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch)
+        {
+            const f32x16 prev = guide_dw_load(pres, lofs, 64u * ch * CLPT_G_HID * 4u, it > 0u);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    st[0][16u * t + 8u * h + q][32u * w + c] = xf[4 * ch + t][q];
+            __syncthreads();
+            guide_dw_tile(st[0], st[1], fr, orr, c, h, pres, lofs, 64u * ch * CLPT_G_HID * 4u, prev);
+            __syncthreads();
+        }
+    }
+
+    /* the block's loss and maxima, in wave order */
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        lsum += __shfl_xor(lsum, off);
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+            amx[l] = fmaxf(amx[l], __shfl_xor(amx[l], off));
+    }
+    if (lane == 0u)
+    {
+        fred[w][0] = lsum;
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+            fred[w][1 + l] = amx[l];
+    }
+    __syncthreads();
+    /* the step's counters (no block of this kernel reads them; the Adam
+     * kernel that follows takes its bias correction from the new count) */
+    if (blockIdx.x == 0u && tid == 0u)
+    {
+        gc->trained = cnt;
+        if (cnt > 0u)
+        {
+            gc->steps += 1u;
+            gc->samples_total += cnt;
+        }
+    }
+    if (tid < CLPT_GT_SCAL)
+    {
+        float a = fred[0][tid];
+#pragma unroll
+        for (int q = 1; q < CLPT_GT_WAVES; ++q)
+            a = tid == 0u ? a + fred[q][tid] : fmaxf(a, fred[q][tid]);
+        spart[(size_t)blockIdx.x * CLPT_GT_SCAL + tid] = a;
+        if (tid == 0u)
+            wpart[blockIdx.x] = wb;
+    }
+}
+
+/* The reduction over the blocks and Adam: CLPT_GA_P parameters per block,
+ * each summed over the fused step's blocks by CLPT_GA_G groups of threads
+ * (group q takes blocks q, q + CLPT_GA_G, ...), the groups then added in
+ * order, so that many loads are in flight and the sum stays in a fixed
+ * order. */
+#define CLPT_GT_MAXBLK CLPT_GUIDE_FUSED_MAXBLK
+#define CLPT_GA_P 32
+#define CLPT_GA_G 8
+__global__ __launch_bounds__(256) void k_guide_adam_fused(float* params, float* m1, float* m2,
+                                                          const float* __restrict__ partial,
+                                                          const double* __restrict__ wpart,
+                                                          const float* __restrict__ spart, uint32_t nblk,
+                                                          GuideCounters* gc, uint32_t step, uint32_t nsteps,
+                                                          float lr, float beta1, float beta2, float eps,
+                                                          __bf16* W, __bf16* WT, __bf16* img, float* grad_out)
+{
+    __shared__ double dred[256];
+    __shared__ float  scale[CLPT_GT_MAXBLK];
+    __shared__ float  gsum[CLPT_GA_G][CLPT_GA_P];
+    __shared__ float  fred[256];
+    const uint32_t    tid = threadIdx.x;
+    double            a   = 0.0;
+    for (uint32_t b = tid; b < nblk; b += blockDim.x)
+        a += wpart[b];
+    dred[tid] = a;
+    __syncthreads();
+    for (uint32_t k = 128; k > 0; k >>= 1)
+    {
+        if (tid < k)
+            dred[tid] += dred[tid + k];
+        __syncthreads();
+    }
+    const double wsum = dred[0];
+    for (uint32_t b = tid; b < nblk; b += blockDim.x)
+        scale[b] = wsum > 0.0 ? (float)(wpart[b] / wsum) : 0.0f;
+    __syncthreads();
+    const uint32_t cnt = gc->trained;
+    const uint32_t j = tid % CLPT_GA_P, grp = tid / CLPT_GA_P;
+    const uint32_t p = blockIdx.x * CLPT_GA_P + j;
+    if (cnt > 0u)
+    {
+        const float* src = partial + p;
+        float        g   = 0.0f;
+        uint32_t     b   = grp;
+        for (; b + 3u * CLPT_GA_G < nblk; b += 4u * CLPT_GA_G)
+        {
+            const float x0 = src[(size_t)b * CLPT_G_PARAMS];
+            const float x1 = src[(size_t)(b + CLPT_GA_G) * CLPT_G_PARAMS];
+            const float x2 = src[(size_t)(b + 2u * CLPT_GA_G) * CLPT_G_PARAMS];
+            const float x3 = src[(size_t)(b + 3u * CLPT_GA_G) * CLPT_G_PARAMS];
+            g              = fmaf(scale[b], x0, g);
+            g              = fmaf(scale[b + CLPT_GA_G], x1, g);
+            g              = fmaf(scale[b + 2u * CLPT_GA_G], x2, g);
+            g              = fmaf(scale[b + 3u * CLPT_GA_G], x3, g);
+        }
+        for (; b < nblk; b += CLPT_GA_G)
+            g = fmaf(scale[b], src[(size_t)b * CLPT_G_PARAMS], g);
+        gsum[grp][j] = g;
+    }
+    __syncthreads();
+    if (cnt > 0u && grp == 0u)
+    {
+        float g = gsum[0][j];
+#pragma unroll
+        for (int q = 1; q < CLPT_GA_G; ++q)
+            g += gsum[q][j];
+        if (grad_out)
+            grad_out[p] = g;
+        const double t  = (double)gc->steps; /* already counts this step */
+        const float  u  = beta1 * m1[p] + (1.0f - beta1) * g;
+        const float  v  = beta2 * m2[p] + (1.0f - beta2) * g * g;
+        m1[p]           = u;
+        m2[p]           = v;
+        const float  uh = (float)((double)u / (1.0 - pow((double)beta1, t)));
+        const float  vh = (float)((double)v / (1.0 - pow((double)beta2, t)));
+        const float  th = params[p] - lr * uh / (sqrtf(vh) + eps);
+        params[p]       = th;
+        guide_store_param(p, th, W, WT, img);
+    }
+    /* block 0 closes the step: the loss (block losses weighted by W_b / W),
+     * the activation maxima, the weight sum; the last step of a batch empties
+     * the sample buffer (no block of this kernel reads the count) */
+    if (blockIdx.x != 0u)
+        return;
+    for (int q = 0; q < CLPT_GT_SCAL; ++q)
+    {
+        float x = 0.0f;
+        for (uint32_t b = tid; b < nblk; b += blockDim.x)
+            x = q == 0 ? fmaf(scale[b], spart[(size_t)b * CLPT_GT_SCAL], x)
+                       : fmaxf(x, spart[(size_t)b * CLPT_GT_SCAL + q]);
+        fred[tid] = x;
+        __syncthreads();
+        for (uint32_t k = 128; k > 0; k >>= 1)
+        {
+            if (tid < k)
+                fred[tid] = q == 0 ? fred[tid] + fred[tid + k] : fmaxf(fred[tid], fred[tid + k]);
+            __syncthreads();
+        }
+        if (tid == 0u && cnt > 0u)
+        {
+            if (q == 0)
+                gc->loss = fred[0];
+            else
+                gc->amax[q - 1] = __float_as_uint(fred[0]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0u)
+    {
+        gc->wsum = (float)wsum;
+        if (step + 1u == nsteps)
             gc->count = 0u;
     }
 }
@@ -733,11 +1271,36 @@ hipError_t launch_guide_gather(hipStream_t s, const GuideView& gv, uint32_t maxd
     return hipGetLastError();
 }
 
+uint32_t guide_fused_blocks(uint32_t N, uint32_t target, uint32_t* iters)
+{
+    const uint32_t passes = N / CLPT_GT_S;
+    const uint32_t it     = std::max(1u, (passes + std::max(1u, target) - 1u) / std::max(1u, target));
+    *iters                = it;
+    return (passes + it - 1u) / it;
+}
+
 hipError_t launch_guide_train(hipStream_t s, const GuideTrainBufs& b, uint32_t N, float lr, float* grad_out,
                               uint32_t step, uint32_t nsteps)
 {
     if (N % 1024u != 0u || N == 0u || step >= nsteps)
         return hipErrorInvalidValue;
+    if (b.fused_blocks)
+    {
+        uint32_t       iters = 1;
+        const uint32_t nblk  = guide_fused_blocks(N, b.fused_blocks, &iters);
+        if (nblk > CLPT_GT_MAXBLK || nblk > b.part_cap)
+            return hipErrorInvalidValue;
+        const GuideSample* smp = b.samples + step;
+        hipLaunchKernelGGL(k_guide_train_fused, dim3(nblk), dim3(64 * CLPT_GT_WAVES), 0, s, smp, b.gc, step, nsteps,
+                           N, iters, (const __bf16*)b.img, (const __bf16*)b.W, b.partial, b.wpart, b.spart);
+        hipLaunchKernelGGL(k_guide_adam_fused, dim3(CLPT_G_PARAMS / CLPT_GA_P), dim3(CLPT_GA_P * CLPT_GA_G), 0, s,
+                           b.params, b.m1, b.m2,
+                           b.partial, b.wpart, b.spart, nblk, b.gc, step, nsteps, lr, 0.9f, 0.999f, 1e-8f, b.W,
+                           b.WT, b.img, grad_out);
+        if (b.img8)
+            hipLaunchKernelGGL(k_guide_pack_fp8, dim3(CLPT_G_LAYERS), dim3(256), 0, s, b.params, b.gc, b.img8, b.scl);
+        return hipGetLastError();
+    }
     const uint32_t     nt   = N / 32u;
     const GuideSample* smp  = b.samples + step; /* rows step + k nsteps */
     hipLaunchKernelGGL(k_guide_features, dim3(N / 256), dim3(256), 0, s, smp, b.gc, step, nsteps, N, b.X, b.wpart);

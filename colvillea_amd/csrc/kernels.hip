@@ -1578,23 +1578,29 @@ __global__ __launch_bounds__(256) void k_trace_batch(SceneView sc, const float4*
     }
 }
 
-/* Streaming copy (16-byte loads and stores, four loads in flight per lane
- * before their stores): the known-traffic kernel for HBM byte counters and
- * the measured copy ceiling. */
-__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n)
+/* Streaming copy (16-byte non-temporal loads and stores, four loads in
+ * flight per lane before their stores): the known-traffic kernel for HBM
+ * byte counters and the measured copy ceiling. */
+typedef float f4v __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src4, float4* __restrict__ dst4, size_t n)
 {
+    const f4v*   src    = reinterpret_cast<const f4v*>(src4);
+    f4v*         dst    = reinterpret_cast<f4v*>(dst4);
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     size_t       i      = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i + 3 * stride < n; i += 4 * stride)
     {
-        const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-        dst[i]              = a;
-        dst[i + stride]     = b;
-        dst[i + 2 * stride] = c;
-        dst[i + 3 * stride] = d;
+        const f4v a = __builtin_nontemporal_load(src + i);
+        const f4v b = __builtin_nontemporal_load(src + i + stride);
+        const f4v c = __builtin_nontemporal_load(src + i + 2 * stride);
+        const f4v d = __builtin_nontemporal_load(src + i + 3 * stride);
+        __builtin_nontemporal_store(a, dst + i);
+        __builtin_nontemporal_store(b, dst + i + stride);
+        __builtin_nontemporal_store(c, dst + i + 2 * stride);
+        __builtin_nontemporal_store(d, dst + i + 3 * stride);
     }
     for (; i < n; i += stride)
-        dst[i] = src[i];
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 /* ------------------------------------------------------------ allocation-free tracing */

@@ -444,7 +444,7 @@ class Integrator:
 
     # -- learned directional proposal (no reference counterpart)
     GUIDE_FIELDS = ("enable", "bsdf_fraction", "train", "record_paths", "batch_size", "learning_rate", "max_steps",
-                    "precision", "steps_per_batch")
+                    "precision", "steps_per_batch", "train_kernel")
 
     def setGuide(self, enable: bool = True, **fields):
         """Turns the learned proposal on or off; other fields as in

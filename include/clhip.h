@@ -374,6 +374,8 @@ typedef struct
     uint32_t max_steps;     /* stop training after this many steps (0 = never) */
     uint32_t precision;     /* inference: 0 = bfloat16, 1 = FP8 e4m3 with per-layer scales (ablation) */
     uint32_t steps_per_batch; /* Adam steps after each batch, each on its own batch_size samples (0 = 1) */
+    uint32_t train_kernel;  /* 0: fused step (one kernel from encoding to weight gradients, one for the
+                               reduction and Adam); 1: one launch per GEMM (the comparison path) */
 } clhip_guide;
 
 typedef struct

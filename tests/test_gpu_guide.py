@@ -140,10 +140,11 @@ def test_fp8_scales_follow_training_activations(integ):
         integ.setGuide(False, precision=0)
 
 
-def test_training_gradient_and_adam_match_oracle(integ):
+@pytest.mark.parametrize("train_kernel", [0, 1], ids=["fused", "per_gemm"])
+def test_training_gradient_and_adam_match_oracle(integ, train_kernel):
     rng = np.random.default_rng(2)
     integ.guideReset(5)
-    integ.setGuide(False, learning_rate=1e-3)
+    integ.setGuide(False, learning_rate=1e-3, train_kernel=train_kernel)
     p0 = integ.guideWeights().astype(np.float64)
     Ws = G.split_params(p0)
     Ws[4][:, :40] *= 10.0
@@ -177,6 +178,57 @@ def test_training_gradient_and_adam_match_oracle(integ):
     assert np.allclose(p1, ref1, rtol=1e-6, atol=1e-9)
     st = integ.guideStats()
     assert st["steps"] == 1 and st["last_count"] == n
+    integ.setGuide(False, learning_rate=1e-3)
+
+
+def _step(integ, samples, kernel):
+    """One step from fresh weights and Adam state (seed 6)."""
+    integ.guideReset(6)
+    integ.setGuide(False, learning_rate=1e-3, train_kernel=kernel, precision=1)
+    loss, g = integ.guideTrain(samples, want_grad=True)
+    return loss, g.astype(np.float64), integ.guideWeights(), integ.guideFp8Scales(), integ.guideStats()
+
+
+@pytest.mark.parametrize("n,tol", [(1, 1e-3), (100, 1e-3), (3000, 2e-2), (70000, 2e-2), (131072, 2e-2)])
+def test_fused_step_equals_per_gemm_step(integ, n, tol):
+    """The fused step (per-block gradients normalised by the block's weight
+    sum, one to several passes per block, idle samples in the last pass)
+    against the per-GEMM path on the same samples: gradient per layer, loss,
+    the Adam update and the activation maxima behind the FP8 scales.  With
+    one block (n <= 128) both round the same scaled loss gradient to
+    bfloat16 and differ only in summation order; with several, the fused
+    step rounds w / W_b instead of w / W, and the random targets make the
+    sample sums cancel, which magnifies that rounding difference."""
+    rng = np.random.default_rng(11 + n)
+    integ.guideReset(6)
+    p0 = integ.guideWeights()
+    q = rand_queries(rng, n)
+    wi = rand_dirs(rng, n)
+    w = rng.uniform(0.05, 3.0, n)
+    samples = np.concatenate([q, wi, w[:, None]], 1)
+    l1, g1, p1, s1, st1 = _step(integ, samples, 1)
+    l0, g0, p0f, s0, st0 = _step(integ, samples, 0)
+    integ.setGuide(False, learning_rate=1e-3, precision=0)
+    assert abs(l0 - l1) <= 1e-5 * abs(l1), (l0, l1)
+    offs = [0, 256 * 64] + [256 * 64 + k * 4096 for k in range(1, 5)]
+    for l in range(5):
+        a, b = offs[l], offs[l + 1]
+        rel = np.linalg.norm(g0[a:b] - g1[a:b]) / np.linalg.norm(g1[a:b])
+        assert rel < tol, (l, rel)
+    # Adam moves every weight by about lr; the update follows the gradient
+    d0, d1 = p0f - p0, p1 - p0
+    assert np.mean(np.sign(d0) == np.sign(d1)) > 0.97
+    assert np.allclose(s0, s1, rtol=2e-3), (s0, s1)
+    assert st0["last_count"] == st1["last_count"] == n and st0["steps"] == st1["steps"] == 1
+
+
+def test_fused_step_without_samples_keeps_weights(integ):
+    integ.guideReset(6)
+    integ.setGuide(False, learning_rate=1e-3, train_kernel=0)
+    p0 = integ.guideWeights()
+    loss, g = integ.guideTrain(np.zeros((0, 13)), want_grad=True)
+    assert np.array_equal(integ.guideWeights(), p0)
+    assert integ.guideStats()["steps"] == 0
 
 
 def test_training_learns_a_known_distribution(integ):
