@@ -183,7 +183,7 @@ def batched_views(integ, args) -> dict:
 # initialisations at these fixed hyperparameters.
 GUIDE_SPEC = {"bsdf_fraction": 0.5, "train": 1, "record_paths": 1 << 21, "batch_size": 1 << 16,
               "learning_rate": 1e-3, "steps_per_batch": 16, "max_steps": 1024, "warm_spp": 8,
-              "warm_steps_per_batch": 64}
+              "warm_steps_per_batch": 128}
 # A faster-learning schedule found by the same sweeps (lr 3e-3, 2^15 samples
 # per step), reported beside the headline.
 GUIDE_TUNED = {"bsdf_fraction": 0.5, "train": 1, "record_paths": 1 << 20, "batch_size": 1 << 15,
@@ -238,6 +238,39 @@ def guided_room(args) -> dict:
     out["tuned"] = line(r2)
     out["curves"] = {"plain": r["plain"], "guided": r["guided"], "guided_tuned": r2["guided"]}
     return out
+
+
+def guide_train_step(args) -> dict:
+    """Device time of one Adam step of the learned proposal (fused step and
+    the per-GEMM comparison path), as the guided tracer runs it: inside a
+    render of the room (480x270), 16 steps after each batch, timed by the
+    library's HIP events around the training launches."""
+    from colvillea_amd import Integrator, IntegratorType, scenes, upload_scene
+
+    desc = scenes.make_c3(width=480, height=270, spp=8)
+    it = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
+    upload_scene(it, desc)
+    res, nsteps = {}, 0
+    for n in (1 << 15, 1 << 16):
+        for kern, name in ((0, "fused"), (1, "per_gemm")):
+            it.guideReset(1)
+            it.setGuide(True, train=1, batch_size=n, record_paths=1 << 20, steps_per_batch=16, learning_rate=1e-3,
+                        train_kernel=kern)
+            it.resetIterationIndex()
+            it.render(2)
+            i0 = it.guideStats()["issued"]
+            it.render(args.train_spp)
+            st = it.guideStats()
+            steps = st["issued"] - i0
+            res.setdefault(name, {})[str(n)] = round(1e3 * st["train_ms"] / max(1, steps), 2)
+            nsteps = steps
+    it.close()
+    # 163840 flop per sample: forward 2 x 32768, backward data 32768, weight gradients 65536
+    flop = {str(n): 163840 * n for n in (1 << 15, 1 << 16)}
+    return {"us_per_step": res, "steps_timed_per_case": nsteps,
+            "tflops_fused": {k: round(flop[k] / (v * 1e-6) / 1e12, 1) for k, v in res["fused"].items()},
+            "gflop_per_step": {k: round(v / 1e9, 2) for k, v in flop.items()}, "peak_bf16_tflops": 2500.0,
+            "note": "device time per Adam step (training launches only) inside a guided render of the room"}
 
 
 def stress_c4(args) -> dict:
@@ -313,12 +346,18 @@ def main():
     ap.add_argument("--c4-ref-spp", type=int, default=256)
     ap.add_argument("--c4-guided-spp", type=int, default=32)
     ap.add_argument("--autotune-log", default=None, help="write the autotuner's trials here (JSON)")
+    ap.add_argument("--train-spp", type=int, default=8, help="spp of the renders that time the Adam step")
+    ap.add_argument("--only-guide-train", action="store_true",
+                    help="print only the Adam-step timing of the learned proposal (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); sharding over GPUs (sample indices "
               "split across GPUs, then one float64 film reduction) is not run.", file=sys.stderr)
         sys.exit(2)
 
+    if args.only_guide_train:
+        print(json.dumps(guide_train_step(args)), flush=True)
+        return
     from colvillea_amd import Integrator, IntegratorType, scenes, upload_scene
 
     t0 = time.perf_counter()
@@ -411,6 +450,9 @@ def main():
     integ.close()
     integ = None
     guided = None if args.no_guided else guided_room(args)
+    if guided is not None:
+        guided["train_step"] = guide_train_step(args)
+        log(f"Adam step: {guided['train_step']['us_per_step']}")
     stress = None if args.no_c4 else stress_c4(args)
     cpu = None
     if not args.no_cpu:

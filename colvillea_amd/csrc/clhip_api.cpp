@@ -2312,8 +2312,8 @@ int clhip_set_guide(clhip_ctx* ctx, const clhip_guide* g)
         v.learning_rate = 1e-3f;
     if (v.steps_per_batch == 0)
         v.steps_per_batch = 1;
-    if (v.steps_per_batch > 64)
-        return fail(ctx, CLHIP_E_INVALID, "steps_per_batch %u above 64", v.steps_per_batch);
+    if (v.steps_per_batch > 256)
+        return fail(ctx, CLHIP_E_INVALID, "steps_per_batch %u above 256", v.steps_per_batch);
     if (v.precision > 1u)
         return fail(ctx, CLHIP_E_INVALID, "precision %u: 0 (bfloat16) or 1 (FP8)", v.precision);
     if (v.train_kernel > 1u)
