@@ -242,35 +242,40 @@ def guided_room(args) -> dict:
 
 def guide_train_step(args) -> dict:
     """Device time of one Adam step of the learned proposal (fused step and
-    the per-GEMM comparison path), as the guided tracer runs it: inside a
-    render of the room (480x270), 16 steps after each batch, timed by the
-    library's HIP events around the training launches."""
+    the per-GEMM comparison path) at its full size, as the guided tracer runs
+    it: inside a render of the open sun-and-sky scene (512x512, 16 spp in one
+    batch, 2^22 recorded paths), whose paths nearly all return light, so
+    that every step holds its full batch of samples; 16 steps after the
+    batch, timed by the library's HIP events around the training launches."""
     from colvillea_amd import Integrator, IntegratorType, scenes, upload_scene
 
-    desc = scenes.make_c3(width=480, height=270, spp=8)
+    desc = scenes.make_c0(width=512, height=512, spp=args.train_spp)
     it = Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height)
     upload_scene(it, desc)
-    res, nsteps = {}, 0
-    for n in (1 << 15, 1 << 16):
+    res, live, nsteps = {}, {}, 0
+    sizes = [int(x) for x in args.train_sizes.split(",")]
+    for n in sizes:
         for kern, name in ((0, "fused"), (1, "per_gemm")):
             it.guideReset(1)
-            it.setGuide(True, train=1, batch_size=n, record_paths=1 << 20, steps_per_batch=16, learning_rate=1e-3,
+            it.setGuide(True, train=1, batch_size=n, record_paths=1 << 22, steps_per_batch=16, learning_rate=1e-3,
                         train_kernel=kern)
             it.resetIterationIndex()
-            it.render(2)
+            it.render(args.train_spp)
             i0 = it.guideStats()["issued"]
             it.render(args.train_spp)
             st = it.guideStats()
-            steps = st["issued"] - i0
-            res.setdefault(name, {})[str(n)] = round(1e3 * st["train_ms"] / max(1, steps), 2)
-            nsteps = steps
+            nsteps = st["issued"] - i0
+            res.setdefault(name, {})[str(n)] = round(1e3 * st["train_ms"] / max(1, nsteps), 2)
+            live.setdefault(name, {})[str(n)] = int(st["last_count"])
+    it.setGuide(False)
     it.close()
     # 163840 flop per sample: forward 2 x 32768, backward data 32768, weight gradients 65536
-    flop = {str(n): 163840 * n for n in (1 << 15, 1 << 16)}
-    return {"us_per_step": res, "steps_timed_per_case": nsteps,
+    flop = {str(n): 163840 * min(n, live["fused"][str(n)]) for n in sizes}
+    return {"us_per_step": res, "live_samples_last_step": live, "steps_timed_per_case": nsteps,
             "tflops_fused": {k: round(flop[k] / (v * 1e-6) / 1e12, 1) for k, v in res["fused"].items()},
             "gflop_per_step": {k: round(v / 1e9, 2) for k, v in flop.items()}, "peak_bf16_tflops": 2500.0,
-            "note": "device time per Adam step (training launches only) inside a guided render of the room"}
+            "note": "device time per Adam step (training launches only) inside a guided render of the sun-and-sky "
+                    "scene, 512x512, full steps"}
 
 
 def stress_c4(args) -> dict:
@@ -346,7 +351,8 @@ def main():
     ap.add_argument("--c4-ref-spp", type=int, default=256)
     ap.add_argument("--c4-guided-spp", type=int, default=32)
     ap.add_argument("--autotune-log", default=None, help="write the autotuner's trials here (JSON)")
-    ap.add_argument("--train-spp", type=int, default=8, help="spp of the renders that time the Adam step")
+    ap.add_argument("--train-spp", type=int, default=16, help="spp of the renders that time the Adam step")
+    ap.add_argument("--train-sizes", default="32768,65536", help="samples per Adam step to time (comma list)")
     ap.add_argument("--only-guide-train", action="store_true",
                     help="print only the Adam-step timing of the learned proposal (for profiling)")
     args = ap.parse_args()

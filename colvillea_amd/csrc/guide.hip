@@ -688,10 +688,35 @@ __global__ __launch_bounds__(256) void k_guide_train_fused(const GuideSample* __
     __shared__ float                fred[CLPT_GT_WAVES][CLPT_GT_SCAL];
 
     const uint32_t tid = threadIdx.x, w = tid >> 6, lane = __lane_id(), c = lane & 31u, h = lane >> 5;
-    const bool     hb    = h != 0u;
-    const uint32_t cnt   = guide_count(gc, step, nsteps, N);
+    const bool     hb  = h != 0u;
+    const uint32_t cnt = guide_count(gc, step, nsteps, N);
+    /* the step's samples (cnt of the N slots) spread over the blocks: the
+     * fewest passes per block that cover them, at most the launch's `iters` */
+    const uint32_t per   = CLPT_GT_S * gridDim.x;
+    iters                = min(iters, max(1u, (cnt + per - 1u) / per));
     const uint32_t first = blockIdx.x * iters * CLPT_GT_S;
     const uint32_t stop  = min(first + iters * CLPT_GT_S, cnt);
+    /* the step's counters (no block of this kernel reads them; the Adam
+     * kernel that follows takes its bias correction from the new count) */
+    if (blockIdx.x == 0u && tid == 0u)
+    {
+        gc->trained = cnt;
+        if (cnt > 0u)
+        {
+            gc->steps += 1u;
+            gc->samples_total += cnt;
+        }
+    }
+    /* a block without samples contributes nothing (W_b = 0, which the
+     * reduction skips) */
+    if (first >= cnt)
+    {
+        if (tid < CLPT_GT_SCAL)
+            spart[(size_t)blockIdx.x * CLPT_GT_SCAL + tid] = 0.0f;
+        if (tid == 0u)
+            wpart[blockIdx.x] = 0.0;
+        return;
+    }
 
     {
         /* every load of a thread issued before its LDS writes */
@@ -952,17 +977,6 @@ __global__ __launch_bounds__(256) void k_guide_train_fused(const GuideSample* __
             fred[w][1 + l] = amx[l];
     }
     __syncthreads();
-    /* the step's counters (no block of this kernel reads them; the Adam
-     * kernel that follows takes its bias correction from the new count) */
-    if (blockIdx.x == 0u && tid == 0u)
-    {
-        gc->trained = cnt;
-        if (cnt > 0u)
-        {
-            gc->steps += 1u;
-            gc->samples_total += cnt;
-        }
-    }
     if (tid < CLPT_GT_SCAL)
     {
         float a = fred[0][tid];
@@ -1019,19 +1033,21 @@ __global__ __launch_bounds__(256) void k_guide_adam_fused(float* params, float* 
         const float* src = partial + p;
         float        g   = 0.0f;
         uint32_t     b   = grp;
-        for (; b + 3u * CLPT_GA_G < nblk; b += 4u * CLPT_GA_G)
+        /* blocks without samples (W_b = 0) wrote no gradient: skipped */
+        for (; b + 3u * CLPT_GA_G < nblk && scale[b + 3u * CLPT_GA_G] != 0.0f; b += 4u * CLPT_GA_G)
         {
             const float x0 = src[(size_t)b * CLPT_G_PARAMS];
             const float x1 = src[(size_t)(b + CLPT_GA_G) * CLPT_G_PARAMS];
             const float x2 = src[(size_t)(b + 2u * CLPT_GA_G) * CLPT_G_PARAMS];
             const float x3 = src[(size_t)(b + 3u * CLPT_GA_G) * CLPT_G_PARAMS];
-            g              = fmaf(scale[b], x0, g);
-            g              = fmaf(scale[b + CLPT_GA_G], x1, g);
-            g              = fmaf(scale[b + 2u * CLPT_GA_G], x2, g);
+            g              = scale[b] != 0.0f ? fmaf(scale[b], x0, g) : g;
+            g              = scale[b + CLPT_GA_G] != 0.0f ? fmaf(scale[b + CLPT_GA_G], x1, g) : g;
+            g              = scale[b + 2u * CLPT_GA_G] != 0.0f ? fmaf(scale[b + 2u * CLPT_GA_G], x2, g) : g;
             g              = fmaf(scale[b + 3u * CLPT_GA_G], x3, g);
         }
         for (; b < nblk; b += CLPT_GA_G)
-            g = fmaf(scale[b], src[(size_t)b * CLPT_G_PARAMS], g);
+            if (scale[b] != 0.0f)
+                g = fmaf(scale[b], src[(size_t)b * CLPT_G_PARAMS], g);
         gsum[grp][j] = g;
     }
     __syncthreads();

@@ -368,6 +368,35 @@ def test_online_training_is_reproducible(room_ref):
     assert np.allclose(f1, f2, rtol=1e-12, atol=1e-12)
 
 
+def test_fused_step_with_few_samples_in_a_large_batch(room_ref):
+    """Online training where a step holds far fewer samples than its batch
+    size (the fused kernel then runs only the blocks and passes those
+    samples need, and the reduction skips the rest): the same samples and
+    step counts as the per-GEMM path, and updates in the same direction."""
+    desc, it, _ = room_ref
+    it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0, samples_per_batch=1)
+    runs = []
+    try:
+        for kern in (1, 0):
+            it.guideReset(29)
+            w0 = it.guideWeights()
+            it.setGuide(True, bsdf_fraction=0.5, train=1, record_paths=2048, batch_size=65536,
+                        learning_rate=1e-3, steps_per_batch=2, max_steps=2, train_kernel=kern)
+            it.resetIterationIndex()
+            it.render(1)
+            st = it.guideStats()
+            runs.append((it.guideWeights() - w0, st["steps"], st["samples"], st["last_loss"]))
+            it.setGuide(False)
+    finally:
+        it.setGuide(False, train_kernel=0)
+        it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=0)
+    (d1, s1, n1, l1), (d0, s0, n0, l0) = runs
+    assert s1 == s0 == 2 and n1 == n0 and 0 < n0 < 2 * 65536 // 8
+    assert abs(l0 - l1) <= 1e-3 * abs(l1), (l0, l1)
+    moved = (d1 != 0) | (d0 != 0)
+    assert np.mean(np.sign(d0[moved]) == np.sign(d1[moved])) > 0.95
+
+
 def test_gather_matches_oracle(room_ref):
     """The training samples of one recorded batch against the oracle's
     restatement of the gather on the same records: same samples, same
