@@ -398,6 +398,13 @@ struct Tracer
             /* else an instance leaf waiting for the held leaf (or, at a
              * ray's first step, for the next pass) */
         }
+        /* a second inner node when the first led to one: the blocks below
+         * (triangle, pops) and above (set-ups) are shared by two visits */
+        if (has_ref && inst_sp >= 0 && (ref & CLPT_LEAF_BIT) == 0u)
+        {
+            ++nnodes;
+            has_ref = visit_node<W>(sc.nodes, r, tmin, best.t, st, ref);
+        }
         /* triangle unit: one triangle of the held leaf */
         if (leaf_left > 0u)
         {
@@ -747,7 +754,7 @@ __global__ __launch_bounds__(256, 6) void k_trace(SceneView sc, RayQueue q, cons
             hb.prim[i] = tr.found ? tr.best.prim : 0xffffffffu;
             fin        = true;
         }
-        sn += (unsigned long long)__popcll(__ballot(vn != 0u));
+        sn += (unsigned long long)__popcll(__ballot(vn != 0u)) + (unsigned long long)__popcll(__ballot(vn > 1u));
         stt += (unsigned long long)__popcll(__ballot(vt != 0u));
         const unsigned long long mask = __ballot(fin);
         if (mask)
@@ -1214,7 +1221,7 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
             }
             fin = true;
         }
-        sn += (unsigned long long)__popcll(__ballot(vn != 0u));
+        sn += (unsigned long long)__popcll(__ballot(vn != 0u)) + (unsigned long long)__popcll(__ballot(vn > 1u));
         stt += (unsigned long long)__popcll(__ballot(vt != 0u));
         sv += (unsigned long long)__popcll(__ballot(vis));
         const unsigned long long mask = __ballot(fin);
