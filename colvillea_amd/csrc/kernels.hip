@@ -727,7 +727,7 @@ __global__ __launch_bounds__(256) void k_camera(BatchTable bt, GuideView gv, Ray
 }
 
 template <int W>
-__global__ __launch_bounds__(256, 6) void k_trace(SceneView sc, RayQueue q, const uint32_t* count, uint32_t depth,
+__global__ __launch_bounds__(256, 5) void k_trace(SceneView sc, RayQueue q, const uint32_t* count, uint32_t depth,
                                                HitBuf hb, uint2* gstack, unsigned long long* ctr)
 {
     CLPT_DECLARE_STACK(st, gstack);
