@@ -315,13 +315,16 @@ def stress_c4(args) -> dict:
                           "t_guided_s": None if r["t_guided"] is None else round(r["t_guided"], 3),
                           "t_plain_s": round(r["t_plain"], 3), "relmse_plain_final": r["eps"],
                           "relmse_guided_final": r["guided"][-1]["relmse"],
+                          "relmse_ratio_equal_spp": round(r["guided"][-1]["relmse"] / r["eps"], 3),
                           "guided_s_final": round(r["guided"][-1]["seconds"], 3),
+                          "train_s_final": round(r["guided"][-1].get("train_seconds", 0.0), 3),
                           "curve": r["guided"]}
     it.close()
     return {"workload": f"C4: {desc.width}x{desc.height}, {desc.spp} spp, depth {desc.max_depth}, "
                         f"{desc.num_triangles()} triangles, 8192x4096 HDRI, seed {desc.info['seed']}",
             "plain_Msamples_per_s": round(plain, 2), "ablation_spp": args.c4_guided_spp,
-            "reference_spp": args.c4_ref_spp, "guide": dict(GUIDE_SPEC), "ablation": ablation}
+            "reference_spp": args.c4_ref_spp, "guide": dict(GUIDE_SPEC), "ablation": ablation,
+            "plain_curve": plain_curve}
 
 
 def main():

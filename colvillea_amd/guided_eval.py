@@ -42,15 +42,18 @@ def _render_curve(it: Integrator, spp_points: List[int], ref, ref_var, warm_spp:
     given, then `guide`), then discard that film; the time is counted.
     With `guide` and `desc`, once training has stopped (max_steps issued)
     the remaining chunks use the library's default batch size, as the plain
-    tracer does."""
+    tracer does.  Guided points also carry the device time of the Adam
+    steps up to them (train_seconds)."""
     it.resetIterationIndex()
     it.sync()
-    done, t_acc, curve = 0, 0.0, []
+    done, t_acc, train_acc, curve = 0, 0.0, 0.0, []
     if warm_spp:
         t = time.perf_counter()
         if warm_guide:
             it.setGuide(True, **warm_guide)
         it.render(warm_spp)
+        if guide is not None:
+            train_acc += it.guideStats()["train_ms"] * 1e-3
         if warm_guide:
             it.setGuide(True, **guide)
         it.resetIterationIndex()
@@ -65,11 +68,16 @@ def _render_curve(it: Integrator, spp_points: List[int], ref, ref_var, warm_spp:
         it.render(s - done)
         it.sync()
         t_acc += time.perf_counter() - t
+        if guide is not None:
+            train_acc += it.guideStats()["train_ms"] * 1e-3
         done = s
         img = it.readbackSums() / s
         den = ref ** 2 + EPS_DEN
         e = float(np.mean((img - ref) ** 2 / den) - np.mean(ref_var / den))
-        curve.append({"spp": s, "seconds": t_acc, "relmse": e})
+        point = {"spp": s, "seconds": t_acc, "relmse": e}
+        if guide is not None:
+            point["train_seconds"] = train_acc  # device time of the Adam steps so far
+        curve.append(point)
     return curve
 
 
