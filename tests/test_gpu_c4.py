@@ -90,3 +90,42 @@ def test_c4_full_frame_one_spp(c4):
     assert cnt["hits"][1] > 0.6 * cnt["rays"][1]  # 71% of the camera rays hit (oracle, 384x216)
     assert np.isfinite(img).all()
     assert img[..., :3].mean() > 0
+
+
+def test_c4_gpu_reference_consistent_with_f64_oracle(c4):
+    """The error reference of the C4 ablation is the plain GPU tracer at high
+    spp.  On the 256 x 256 crop of the full scene its expectation equals the
+    float64 oracle's (libm elementary functions, float64 dome tables of its
+    own): the oracle at 16 chunks of 4 spp against the GPU at 16 chunks of
+    512 spp on independent RNG lanes.  The frame means agree within 4
+    standard errors, and the 64 block means (32 x 32 pixels) have a mean
+    squared t-statistic below 2 (about 1.15 expected for 15 degrees of
+    freedom per side) and none beyond 8."""
+    desc = scenes.crop(c4, *CROP, spp=4)
+    t = time.perf_counter()
+    sc = orc.Scene(desc)
+    ref = np.array([sc.render("f64", samples=(4 * k, 4 * k + 4))[0] / 4 for k in range(16)])
+    sc.close()
+    note(f"float64 oracle: C4 crop x 64 spp in {time.perf_counter() - t:.1f} s")
+    gpu = []
+    with Integrator.createIntegrator(IntegratorType.WavefrontPathTracing, desc.width, desc.height) as it:
+        upload_scene(it, desc)
+        for lane in range(2, 18):
+            it.setParams(max_depth=desc.max_depth, rr_depth=desc.rr_depth, view=lane)
+            it.render(512)
+            gpu.append(it.readbackSums() / 512)
+    gpu = np.array(gpu)
+
+    def stats(ch):
+        frame = ch.sum(-1).mean(axis=(1, 2))
+        blk = ch.sum(-1).reshape(len(ch), 8, 32, 8, 32).mean(axis=(2, 4))
+        return frame.mean(), frame.std(ddof=1) / np.sqrt(len(ch)), blk.mean(0), blk.std(0, ddof=1) / np.sqrt(len(ch))
+
+    fr, fse, br, bse = stats(ref)
+    fg, gse, bg, bgse = stats(gpu)
+    z = (fr - fg) / np.hypot(fse, gse)
+    tb = (br - bg) / np.sqrt(bse ** 2 + bgse ** 2 + 1e-30)
+    note(f"frame mean: oracle {fr:.5g} +- {fse:.2g}, GPU {fg:.5g} +- {gse:.2g} (z = {z:.2f}); "
+         f"blocks: mean t^2 {np.mean(tb ** 2):.2f}, max |t| {np.abs(tb).max():.2f}")
+    assert abs(z) < 4.0
+    assert np.mean(tb ** 2) < 2.0 and np.abs(tb).max() < 8.0
