@@ -685,7 +685,8 @@ CLPT_D f3 sample_emitters(const SceneView& sc, float ux, float uy, f3* dir, floa
 struct RayW
 {
     f3       o, d, inv;
-    uint32_t k; /* the axes kx | ky << 8 | kz << 16 of the watertight test */
+    uint32_t k; /* the axes kx | ky << 8 | kz << 16 of the watertight test, and in bits
+                   24..26 the signs of inv.x, inv.y, inv.z (1 = negative) */
     float    Sx, Sy, Sz;
 };
 
@@ -723,13 +724,16 @@ CLPT_D void ray_setup(RayW& r)
     r.inv = f3{__builtin_amdgcn_rcpf(fabsf(r.d.x) > tiny ? r.d.x : (r.d.x < 0.0f ? -tiny : tiny)),
                __builtin_amdgcn_rcpf(fabsf(r.d.y) > tiny ? r.d.y : (r.d.y < 0.0f ? -tiny : tiny)),
                __builtin_amdgcn_rcpf(fabsf(r.d.z) > tiny ? r.d.z : (r.d.z < 0.0f ? -tiny : tiny))};
+    /* the near plane of each slab: lo where inv >= 0, hi where it is negative */
+    r.k |= ((__float_as_uint(r.inv.x) >> 31) << 24) | ((__float_as_uint(r.inv.y) >> 31) << 25) |
+           ((__float_as_uint(r.inv.z) >> 31) << 26);
 }
 
 /* Watertight two-sided triangle test; hit iff tmin <= t <= tmax. */
 CLPT_D bool tri_test(const RayW& r, float tmin, float tmax, f3 v0, f3 v1, f3 v2, float* t, float* b1, float* b2)
 {
     const f3    A = sub3(v0, r.o), B = sub3(v1, r.o), C = sub3(v2, r.o);
-    const int   kx = (int)(r.k & 0xffu), ky = (int)((r.k >> 8) & 0xffu), kz = (int)(r.k >> 16);
+    const int   kx = (int)(r.k & 0xffu), ky = (int)((r.k >> 8) & 0xffu), kz = (int)((r.k >> 16) & 0xffu);
     const float Az0 = comp3(A, kz), Bz0 = comp3(B, kz), Cz0 = comp3(C, kz);
     const float Ax = comp3(A, kx) - r.Sx * Az0;
     const float Ay = comp3(A, ky) - r.Sy * Az0;

@@ -171,22 +171,33 @@ template <>
 __device__ __forceinline__ bool visit_node<4>(const void* nodes, const RayW& r, float tmin, float tmax,
                                               TraverseStack& st, uint32_t& ref)
 {
-    const float4* np  = reinterpret_cast<const float4*>(static_cast<const Node4*>(nodes) + ref);
-    const float4  lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-    const uint4   ch  = *reinterpret_cast<const uint4*>(np + 6);
-    float         tn[4];
-    uint32_t      cr[4]   = {ch.x, ch.y, ch.z, ch.w};
-    const float   lo_x[4] = {lox.x, lox.y, lox.z, lox.w}, hi_x[4] = {hix.x, hix.y, hix.z, hix.w};
-    const float   lo_y[4] = {loy.x, loy.y, loy.z, loy.w}, hi_y[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-    const float   lo_z[4] = {loz.x, loz.y, loz.z, loz.w}, hi_z[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+    /* Each axis's near plane is lo where the ray's reciprocal is >= 0 and hi
+     * where it is negative, so the loads fetch the near and far plane of
+     * every slab directly (offsets from the ray's sign bits) and no min/max
+     * orders them: t = (plane - o) * inv is the same product as before, and
+     * multiplying by a positive (negative) factor keeps (reverses) order. */
+    const char*    nb = static_cast<const char*>(nodes) + (size_t)ref * sizeof(Node4);
+    const uint32_t sx = ((r.k >> 24) & 1u) * 16u, sy = ((r.k >> 25) & 1u) * 16u, sz = ((r.k >> 26) & 1u) * 16u;
+    const float4   nx4 = *reinterpret_cast<const float4*>(nb + sx);
+    const float4   fx4 = *reinterpret_cast<const float4*>(nb + 16u - sx);
+    const float4   ny4 = *reinterpret_cast<const float4*>(nb + 32u + sy);
+    const float4   fy4 = *reinterpret_cast<const float4*>(nb + 48u - sy);
+    const float4   nz4 = *reinterpret_cast<const float4*>(nb + 64u + sz);
+    const float4   fz4 = *reinterpret_cast<const float4*>(nb + 80u - sz);
+    const uint4    ch  = *reinterpret_cast<const uint4*>(nb + 96u);
+    float          tn[4];
+    uint32_t       cr[4]  = {ch.x, ch.y, ch.z, ch.w};
+    const float    n_x[4] = {nx4.x, nx4.y, nx4.z, nx4.w}, f_x[4] = {fx4.x, fx4.y, fx4.z, fx4.w};
+    const float    n_y[4] = {ny4.x, ny4.y, ny4.z, ny4.w}, f_y[4] = {fy4.x, fy4.y, fy4.z, fy4.w};
+    const float    n_z[4] = {nz4.x, nz4.y, nz4.z, nz4.w}, f_z[4] = {fz4.x, fz4.y, fz4.z, fz4.w};
 #pragma unroll
     for (int c = 0; c < 4; ++c)
     {
-        const float t0x = (lo_x[c] - r.o.x) * r.inv.x, t1x = (hi_x[c] - r.o.x) * r.inv.x;
-        const float t0y = (lo_y[c] - r.o.y) * r.inv.y, t1y = (hi_y[c] - r.o.y) * r.inv.y;
-        const float t0z = (lo_z[c] - r.o.z) * r.inv.z, t1z = (hi_z[c] - r.o.z) * r.inv.z;
-        float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-        float tfar  = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+        const float tnx = (n_x[c] - r.o.x) * r.inv.x, tfx = (f_x[c] - r.o.x) * r.inv.x;
+        const float tny = (n_y[c] - r.o.y) * r.inv.y, tfy = (f_y[c] - r.o.y) * r.inv.y;
+        const float tnz = (n_z[c] - r.o.z) * r.inv.z, tfz = (f_z[c] - r.o.z) * r.inv.z;
+        float tnear = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+        float tfar  = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
         tnear       = tnear * 0.99999f;
         tfar        = tfar * 1.00001f;
         tn[c]       = (cr[c] != CLPT_CHILD_EMPTY && tnear <= tfar) ? tnear : CLPT_FLT_MAX;
@@ -222,26 +233,31 @@ __device__ __forceinline__ bool visit_node<8>(const void* nodes, const RayW& r, 
     const float sx = __uint_as_float((w0.w & 0xffu) << 23);
     const float sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23);
     const float sz = __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
-    /* byte planes: lo x (q0.xy), lo y (q0.zw), lo z (q1.xy), hi x (q1.zw), hi y (q2.xy), hi z (q2.zw) */
-    const uint32_t lxw[2] = {q0.x, q0.y}, lyw[2] = {q0.z, q0.w}, lzw[2] = {q1.x, q1.y};
-    const uint32_t hxw[2] = {q1.z, q1.w}, hyw[2] = {q2.x, q2.y}, hzw[2] = {q2.z, q2.w};
+    /* byte planes: lo x (q0.xy), lo y (q0.zw), lo z (q1.xy), hi x (q1.zw), hi y (q2.xy), hi z (q2.zw);
+     * the near plane of each axis is lo where the ray's reciprocal is >= 0
+     * and hi where it is negative (ray sign bits), so no min/max orders the
+     * two slab distances: the same products, a monotone factor */
+    const bool     nx = ((r.k >> 24) & 1u) != 0u, ny = ((r.k >> 25) & 1u) != 0u, nz = ((r.k >> 26) & 1u) != 0u;
+    const uint32_t axw[2] = {nx ? q1.z : q0.x, nx ? q1.w : q0.y}, bxw[2] = {nx ? q0.x : q1.z, nx ? q0.y : q1.w};
+    const uint32_t ayw[2] = {ny ? q2.x : q0.z, ny ? q2.y : q0.w}, byw[2] = {ny ? q0.z : q2.x, ny ? q0.w : q2.y};
+    const uint32_t azw[2] = {nz ? q2.z : q1.x, nz ? q2.w : q1.y}, bzw[2] = {nz ? q1.x : q2.z, nz ? q1.y : q2.w};
     uint32_t       cr[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
     float          tn[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c)
     {
         const int   wi = c >> 2, sh = 8 * (c & 3);
-        const float lx = px + (float)((lxw[wi] >> sh) & 0xffu) * sx;
-        const float ly = py + (float)((lyw[wi] >> sh) & 0xffu) * sy;
-        const float lz = pz + (float)((lzw[wi] >> sh) & 0xffu) * sz;
-        const float hx = px + (float)((hxw[wi] >> sh) & 0xffu) * sx;
-        const float hy = py + (float)((hyw[wi] >> sh) & 0xffu) * sy;
-        const float hz = pz + (float)((hzw[wi] >> sh) & 0xffu) * sz;
-        const float t0x = (lx - r.o.x) * r.inv.x, t1x = (hx - r.o.x) * r.inv.x;
-        const float t0y = (ly - r.o.y) * r.inv.y, t1y = (hy - r.o.y) * r.inv.y;
-        const float t0z = (lz - r.o.z) * r.inv.z, t1z = (hz - r.o.z) * r.inv.z;
-        float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-        float tfar  = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+        const float ax = px + (float)((axw[wi] >> sh) & 0xffu) * sx;
+        const float ay = py + (float)((ayw[wi] >> sh) & 0xffu) * sy;
+        const float az = pz + (float)((azw[wi] >> sh) & 0xffu) * sz;
+        const float bx = px + (float)((bxw[wi] >> sh) & 0xffu) * sx;
+        const float by = py + (float)((byw[wi] >> sh) & 0xffu) * sy;
+        const float bz = pz + (float)((bzw[wi] >> sh) & 0xffu) * sz;
+        const float tnx = (ax - r.o.x) * r.inv.x, tfx = (bx - r.o.x) * r.inv.x;
+        const float tny = (ay - r.o.y) * r.inv.y, tfy = (by - r.o.y) * r.inv.y;
+        const float tnz = (az - r.o.z) * r.inv.z, tfz = (bz - r.o.z) * r.inv.z;
+        float tnear = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+        float tfar  = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
         tnear       = tnear * 0.99999f;
         tfar        = tfar * 1.00001f;
         tn[c]       = (cr[c] != CLPT_CHILD_EMPTY && tnear <= tfar) ? tnear : CLPT_FLT_MAX;
