@@ -356,7 +356,14 @@ struct Collapser
         {
             nd.child[c] = ref[c];
             if (c >= nk)
+            {
+                /* an empty child's box is inverted and infinite: each slab's
+                 * near distance is +inf and its far distance -inf for every
+                 * ray, so the slab test itself rejects it */
+                nd.lox[c] = nd.loy[c] = nd.loz[c] = INFINITY;
+                nd.hix[c] = nd.hiy[c] = nd.hiz[c] = -INFINITY;
                 continue;
+            }
             nd.lox[c] = clo[c][0];
             nd.loy[c] = clo[c][1];
             nd.loz[c] = clo[c][2];

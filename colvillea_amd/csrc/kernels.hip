@@ -200,7 +200,8 @@ __device__ __forceinline__ bool visit_node<4>(const void* nodes, const RayW& r, 
         float tfar  = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
         tnear       = tnear * 0.99999f;
         tfar        = tfar * 1.00001f;
-        tn[c]       = (cr[c] != CLPT_CHILD_EMPTY && tnear <= tfar) ? tnear : CLPT_FLT_MAX;
+        /* an empty child (inverted infinite box) gives tnear = +inf > tfar */
+        tn[c]       = tnear <= tfar ? tnear : CLPT_FLT_MAX;
     }
     /* nearest first: a 5-comparator sorting network, fixed indices */
     cswap(tn[0], cr[0], tn[1], cr[1]);
