@@ -273,6 +273,7 @@ SceneView scene_view(const clhip_ctx* c)
     s.nodes      = c->d_nodes.p;
     s.bvh_width  = c->bvh_width;
     s.tris       = c->d_tris.as<const TriRec>();
+    s.node_bytes = (uint32_t)std::min<size_t>(c->node_bytes, 0xffffffffu);
     s.inst       = c->d_inst.as<const InstanceRec>();
     s.meshes     = c->d_meshes.as<const MeshRec>();
     s.pos        = c->d_pos.as<const float>();
@@ -1447,6 +1448,10 @@ int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uin
     if (need > 128u)
         return fail(ctx, CLHIP_E_UNSUPPORTED, "tree too deep for the traversal stack (TLAS %u + BLAS %u levels)",
                     r.max_depth, max_blas_depth);
+    /* traversal addresses nodes by 32-bit byte offsets */
+    const size_t nbytes = ctx->bvh_width == 8 ? nodes8.size() * sizeof(Node8) : nodes4.size() * sizeof(Node4);
+    if (nbytes > 0xffffffffull)
+        return fail(ctx, CLHIP_E_UNSUPPORTED, "node array of %zu bytes exceeds 4 GiB", nbytes);
     if (ctx->bvh_width == 8)
         CLHIP_CHECK(ctx, upload(ctx->d_nodes, nodes8));
     else

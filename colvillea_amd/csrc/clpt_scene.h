@@ -153,6 +153,7 @@ struct SceneView
     uint32_t           tlas_root;
     uint32_t           ninst;
     uint32_t           bvh_width; /* 4 (Node4) or 8 (Node8) */
+    uint32_t           node_bytes; /* size of the node array (below 4 GiB) */
 };
 
 /* One camera view: film size, path depth, RNG view lane, the view's offset

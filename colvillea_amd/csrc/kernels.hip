@@ -164,11 +164,13 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32
  * hits farther than the nearest (nearest last) with their entry distance,
  * and continue at the nearest.  Returns false when no child was hit. */
 template <int W>
-__device__ __forceinline__ bool visit_node(const void* nodes, const RayW& r, float tmin, float tmax,
+__device__ __forceinline__ bool visit_node(const SceneView& sc, const RayW& r, float tmin, float tmax,
                                            TraverseStack& st, uint32_t& ref);
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 template <>
-__device__ __forceinline__ bool visit_node<4>(const void* nodes, const RayW& r, float tmin, float tmax,
+__device__ __forceinline__ bool visit_node<4>(const SceneView& sc, const RayW& r, float tmin, float tmax,
                                               TraverseStack& st, uint32_t& ref)
 {
     /* Each axis's near plane is lo where the ray's reciprocal is >= 0 and hi
@@ -176,20 +178,27 @@ __device__ __forceinline__ bool visit_node<4>(const void* nodes, const RayW& r, 
      * every slab directly (offsets from the ray's sign bits) and no min/max
      * orders them: t = (plane - o) * inv is the same product as before, and
      * multiplying by a positive (negative) factor keeps (reverses) order. */
-    const char*    nb = static_cast<const char*>(nodes) + (size_t)ref * sizeof(Node4);
+    /* buffer loads: 32-bit byte offsets into the node array (the host keeps
+     * it under 4 GiB), so each plane's address is one add */
+    const __amdgpu_buffer_rsrc_t res =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(sc.nodes), (short)0, sc.node_bytes, 0x00020000);
+    const uint32_t nb = ref * (uint32_t)sizeof(Node4);
     const uint32_t sx = ((r.k >> 24) & 1u) * 16u, sy = ((r.k >> 25) & 1u) * 16u, sz = ((r.k >> 26) & 1u) * 16u;
-    const float4   nx4 = *reinterpret_cast<const float4*>(nb + sx);
-    const float4   fx4 = *reinterpret_cast<const float4*>(nb + 16u - sx);
-    const float4   ny4 = *reinterpret_cast<const float4*>(nb + 32u + sy);
-    const float4   fy4 = *reinterpret_cast<const float4*>(nb + 48u - sy);
-    const float4   nz4 = *reinterpret_cast<const float4*>(nb + 64u + sz);
-    const float4   fz4 = *reinterpret_cast<const float4*>(nb + 80u - sz);
-    const uint4    ch  = *reinterpret_cast<const uint4*>(nb + 96u);
+    const u32x4    nx4 = __builtin_amdgcn_raw_buffer_load_b128(res, nb + sx, 0, 0);
+    const u32x4    fx4 = __builtin_amdgcn_raw_buffer_load_b128(res, nb - sx + 16u, 0, 0);
+    const u32x4    ny4 = __builtin_amdgcn_raw_buffer_load_b128(res, nb + sy + 32u, 0, 0);
+    const u32x4    fy4 = __builtin_amdgcn_raw_buffer_load_b128(res, nb - sy + 48u, 0, 0);
+    const u32x4    nz4 = __builtin_amdgcn_raw_buffer_load_b128(res, nb + sz + 64u, 0, 0);
+    const u32x4    fz4 = __builtin_amdgcn_raw_buffer_load_b128(res, nb - sz + 80u, 0, 0);
+    const u32x4    ch  = __builtin_amdgcn_raw_buffer_load_b128(res, nb + 96u, 0, 0);
     float          tn[4];
     uint32_t       cr[4]  = {ch.x, ch.y, ch.z, ch.w};
-    const float    n_x[4] = {nx4.x, nx4.y, nx4.z, nx4.w}, f_x[4] = {fx4.x, fx4.y, fx4.z, fx4.w};
-    const float    n_y[4] = {ny4.x, ny4.y, ny4.z, ny4.w}, f_y[4] = {fy4.x, fy4.y, fy4.z, fy4.w};
-    const float    n_z[4] = {nz4.x, nz4.y, nz4.z, nz4.w}, f_z[4] = {fz4.x, fz4.y, fz4.z, fz4.w};
+    const float    n_x[4] = {__uint_as_float(nx4.x), __uint_as_float(nx4.y), __uint_as_float(nx4.z), __uint_as_float(nx4.w)};
+    const float    f_x[4] = {__uint_as_float(fx4.x), __uint_as_float(fx4.y), __uint_as_float(fx4.z), __uint_as_float(fx4.w)};
+    const float    n_y[4] = {__uint_as_float(ny4.x), __uint_as_float(ny4.y), __uint_as_float(ny4.z), __uint_as_float(ny4.w)};
+    const float    f_y[4] = {__uint_as_float(fy4.x), __uint_as_float(fy4.y), __uint_as_float(fy4.z), __uint_as_float(fy4.w)};
+    const float    n_z[4] = {__uint_as_float(nz4.x), __uint_as_float(nz4.y), __uint_as_float(nz4.z), __uint_as_float(nz4.w)};
+    const float    f_z[4] = {__uint_as_float(fz4.x), __uint_as_float(fz4.y), __uint_as_float(fz4.z), __uint_as_float(fz4.w)};
 #pragma unroll
     for (int c = 0; c < 4; ++c)
     {
@@ -224,10 +233,10 @@ __device__ __forceinline__ bool visit_node<4>(const void* nodes, const RayW& r, 
 }
 
 template <>
-__device__ __forceinline__ bool visit_node<8>(const void* nodes, const RayW& r, float tmin, float tmax,
+__device__ __forceinline__ bool visit_node<8>(const SceneView& sc, const RayW& r, float tmin, float tmax,
                                               TraverseStack& st, uint32_t& ref)
 {
-    const uint4* np = reinterpret_cast<const uint4*>(static_cast<const Node8*>(nodes) + ref);
+    const uint4* np = reinterpret_cast<const uint4*>(static_cast<const Node8*>(sc.nodes) + ref);
     const uint4  w0 = np[0], q0 = np[1], q1 = np[2], q2 = np[3], c0 = np[4], c1 = np[5];
     /* node origin and per-axis power-of-two scales */
     const float px = __uint_as_float(w0.x), py = __uint_as_float(w0.y), pz = __uint_as_float(w0.z);
@@ -410,7 +419,7 @@ struct Tracer
             else if ((ref & CLPT_LEAF_BIT) == 0u)
             {
                 ++nnodes;
-                has_ref = visit_node<W>(sc.nodes, r, tmin, best.t, st, ref);
+                has_ref = visit_node<W>(sc, r, tmin, best.t, st, ref);
             }
             /* else an instance leaf waiting for the held leaf (or, at a
              * ray's first step, for the next pass) */
@@ -420,7 +429,7 @@ struct Tracer
         if (has_ref && inst_sp >= 0 && (ref & CLPT_LEAF_BIT) == 0u)
         {
             ++nnodes;
-            has_ref = visit_node<W>(sc.nodes, r, tmin, best.t, st, ref);
+            has_ref = visit_node<W>(sc, r, tmin, best.t, st, ref);
         }
         /* triangle unit: one triangle of the held leaf */
         if (leaf_left > 0u)
