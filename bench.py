@@ -173,7 +173,10 @@ def batched_views(integ, args) -> dict:
                            "p95": round(float(np.percentile(lat, 95)), 2),
                            "p99": round(float(np.percentile(lat, 99)), 2), "max": round(float(lat.max()), 2)},
             "latency_histogram": {"edges_ms": edges[:-1] + ["inf"], "counts": hist},
-            "autotune": None if tune is None else {"trials": len(tune["trials"]), "best_sample_ms": tune["ms"]}}
+            "autotune": None if tune is None else {
+                "trials": len(tune["trials"]), "best_sample_ms": tune["ms"],
+                "searched": [[t["field"], None if t["field"] is None else t["launch"][t["field"]],
+                              None if t["ms"] is None else round(t["ms"], 2)] for t in tune["trials"]]}}
 
 
 # The configuration's learned-proposal hyperparameters (Adam lr 1e-3, 2^16

@@ -125,6 +125,8 @@ class clhip_launch(C.Structure):
         ("paths_per_batch", C.c_uint32),
         ("graph", C.c_uint32),
         ("order", C.c_uint32),
+        ("shade_block", C.c_uint32),
+        ("tile", C.c_uint32),
     ]
 
 

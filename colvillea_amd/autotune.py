@@ -1,9 +1,10 @@
 """Launch-shape autotuner for the batched bounce loop.
 
 The tuned fields are those of clhip_launch (include/clhip.h): persistent
-grid sizes of the closest-hit and any-hit kernels, the grid of the camera
-and shading kernels, the queue chunk (paths per batch) and the order in
-which views fill the batches.  No field changes a rendered value, only the
+grid sizes of the closest-hit and any-hit kernels, the grid and the block
+size of the camera and shading kernels, the queue chunk (paths per batch),
+the pixel tile that orders a sample's paths (rows or square tiles) and the
+order in which views fill the batches.  No field changes a rendered value, only the
 time, so the search may use a bounded sample of the workload.
 
 The search is coordinate descent in a fixed field order starting from the
@@ -23,7 +24,8 @@ from typing import Dict, List, Optional, Sequence
 
 from .integrator import Integrator, View
 
-FIELDS = ("trace_blocks_per_cu", "shadow_blocks_per_cu", "shade_blocks_per_cu", "paths_per_batch", "order")
+FIELDS = ("trace_blocks_per_cu", "shadow_blocks_per_cu", "shade_blocks_per_cu", "shade_block", "paths_per_batch", "tile",
+          "order")
 
 
 def default_space(integ: Integrator) -> Dict[str, List[int]]:
@@ -33,6 +35,8 @@ def default_space(integ: Integrator) -> Dict[str, List[int]]:
         "trace_blocks_per_cu": sorted({max(1, t), max(1, t - 1), max(1, t - 2), max(1, t // 2)}, reverse=True),
         "shadow_blocks_per_cu": sorted({x for x in (max(1, d["shadow_blocks_per_cu"]), 3, 4, 6) if x <= max(1, d["shadow_blocks_per_cu"])}, reverse=True),
         "shade_blocks_per_cu": [d["shade_blocks_per_cu"], 4, 16],
+        "shade_block": [d["shade_block"], 128, 64],
+        "tile": [d["tile"], 8, 16],
         "paths_per_batch": [d["paths_per_batch"], 16 << 20, 32 << 20, 64 << 20],
         "order": [1, 2],
     }

@@ -672,7 +672,7 @@ struct BatchArgs
     TotalCounters* tot;
     uint2*         gst;
     unsigned long long* ctr;
-    uint32_t       max_depth, blocks_trace, blocks_shadow, blocks_shade;
+    uint32_t       max_depth, blocks_trace, blocks_shadow, blocks_shade, shade_threads;
     GuideView      gv;
     uint32_t       guided, blocks_guided;
     GuideSample*   gsamples; /* gather target when recording */
@@ -685,7 +685,8 @@ BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
     BatchArgs a{};
     a.sc = scene_view(c);
     a.bt = BatchTable{c->d_views.as<const ViewRec>(), c->d_batches.as<const BatchRec>(),
-                      c->d_entries.as<const BatchEntry>(), c->d_cursor.as<uint32_t>()};
+                      c->d_entries.as<const BatchEntry>(), c->d_cursor.as<uint32_t>(),
+                      c->launch.tile > 1u ? c->launch.tile : 1u};
     for (int k = 0; k < 2; ++k)
     {
         a.q[k].o     = c->d_q[k][0].as<float4>();
@@ -715,6 +716,7 @@ BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
     a.blocks_trace  = c->blocks_trace;
     a.blocks_shadow = c->blocks_shadow;
     a.blocks_shade  = c->blocks_shade;
+    a.shade_threads = c->launch.shade_block ? c->launch.shade_block : 256u;
     return a;
 }
 
@@ -724,7 +726,7 @@ BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
  * bracket every traversal and shading launch. */
 int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
 {
-    CLHIP_CHECK(c, clpt::launch_camera(c->stream, a.bt, a.gv, a.q[0], a.bc, a.blocks_shade));
+    CLHIP_CHECK(c, clpt::launch_camera(c->stream, a.bt, a.gv, a.q[0], a.bc, a.shade_threads, a.blocks_shade));
     for (uint32_t depth = 1; depth <= a.max_depth; ++depth)
     {
         const RayQueue& qin  = a.q[(depth - 1) & 1];
@@ -749,7 +751,7 @@ int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
                                                          a.accum, a.bc, a.blocks_guided));
             else if (kind == 1)
                 CLHIP_CHECK(c, clpt::launch_shade(c->stream, a.sc, a.bt, depth, qin, a.hb, qout, a.sq, a.accum, a.bc,
-                                                  a.blocks_shade));
+                                                  a.shade_threads, a.blocks_shade));
             else
                 CLHIP_CHECK(c, clpt::launch_shadow(c->stream, a.sc, a.sq, depth, a.accum, a.bc, a.gst, a.ctr,
                                                    a.gv.nslots ? a.gv.E : nullptr, a.blocks_shadow));
@@ -1737,6 +1739,10 @@ int clhip_set_launch(clhip_ctx* ctx, const clhip_launch* l)
                     "persistent grids hold at most %u (closest hit) and %u (any hit) resident blocks per CU", rt, rs);
     if (l->shade_blocks_per_cu > 64 || l->graph > 2 || l->order > 2)
         return fail(ctx, CLHIP_E_INVALID, "clhip_set_launch: field out of range");
+    if (l->shade_block != 0 && l->shade_block != 64 && l->shade_block != 128 && l->shade_block != 256)
+        return fail(ctx, CLHIP_E_INVALID, "shade_block %u: 64, 128 or 256 threads (0 = 256)", l->shade_block);
+    if (l->tile > 32 || (l->tile & (l->tile - 1u)) != 0u)
+        return fail(ctx, CLHIP_E_INVALID, "tile %u: 1 (rows) or a square tile side of 2, 4, 8, 16 or 32", l->tile);
     if (l->paths_per_batch && l->paths_per_batch < 4096)
         return fail(ctx, CLHIP_E_INVALID, "paths_per_batch %u below 4096", l->paths_per_batch);
     ctx->launch = *l;
@@ -1754,6 +1760,8 @@ int clhip_get_launch(const clhip_ctx* ctx, clhip_launch* out)
     out->paths_per_batch      = default_paths_per_batch(ctx);
     out->graph                = use_graph(ctx) ? 1u : 2u;
     out->order                = ctx->launch.order ? ctx->launch.order : 1u;
+    out->shade_block          = ctx->launch.shade_block ? ctx->launch.shade_block : 256u;
+    out->tile                 = ctx->launch.tile ? ctx->launch.tile : 1u;
     return CLHIP_OK;
 }
 

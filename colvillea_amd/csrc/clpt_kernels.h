@@ -6,13 +6,14 @@
 
 namespace clpt
 {
+/* `threads` per block (64, 128 or 256) for the camera and plain shading kernels */
 hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const GuideView& gv, const RayQueue& q, BatchCounters* bc,
-                         uint32_t blocks);
+                         uint32_t threads, uint32_t blocks);
 hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, const uint32_t* count, uint32_t depth,
                         const HitBuf& hb, uint2* gstack, unsigned long long* ctr, uint32_t blocks);
 hipError_t launch_shade(hipStream_t s, const SceneView& sc, const BatchTable& bt, uint32_t depth, const RayQueue& qin,
                         const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, double* accum, BatchCounters* bc,
-                        uint32_t blocks);
+                        uint32_t threads, uint32_t blocks);
 /* Shading with the learned proposal mixed into the continuation (weights in LDS). */
 hipError_t launch_shade_guided(hipStream_t s, const SceneView& sc, const BatchTable& bt, const GuideView& gv,
                                uint32_t depth, const RayQueue& qin, const HitBuf& hb, const RayQueue& qout,

@@ -186,6 +186,7 @@ struct BatchTable
     const BatchRec*   batches;
     const BatchEntry* entries;
     uint32_t*         cursor;
+    uint32_t          tile; /* 1: paths of a sample in pixel rows; T > 1: in T x T tiles */
 };
 
 /* A ray queue: 3 float4 planes + count (device). */

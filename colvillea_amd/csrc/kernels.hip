@@ -667,6 +667,23 @@ __device__ __forceinline__ void camera_ray(const ViewRec& v, uint32_t px, uint32
     *d           = norm3(add3(add3(d00, sclL3(sx, ddu)), sclL3(sy, ddv)));
 }
 
+/* Pixel number q (0 .. w h - 1) of a sample's paths -> the row-major pixel
+ * index, with the paths ordered by square tiles of side T: bands of T pixel
+ * rows (the last may be lower), tiles of T columns within a band (the last
+ * may be narrower), pixels row by row within a tile.  A bijection of
+ * [0, w h), so every pixel gets every sample in any order. */
+__device__ __forceinline__ uint32_t tile_pixel(uint32_t q, uint32_t w, uint32_t h, uint32_t T)
+{
+    const uint32_t band = w * T;
+    const uint32_t r    = q / band;
+    const uint32_t q1   = q - r * band;
+    const uint32_t th   = min(T, h - r * T);
+    const uint32_t c    = q1 / (T * th);
+    const uint32_t s    = q1 - c * T * th;
+    const uint32_t tw   = min(T, w - c * T);
+    return (r * T + s / tw) * w + c * T + s % tw;
+}
+
 /* Path p of batch b -> its view, pixel and sample index. */
 __device__ __forceinline__ void locate_path(const BatchTable& bt, const BatchRec& b, uint32_t p, uint32_t& view,
                                             uint32_t& pixel, uint32_t& sample)
@@ -686,6 +703,11 @@ __device__ __forceinline__ void locate_path(const BatchTable& bt, const BatchRec
     view               = e.view;
     pixel              = l % n;
     sample             = e.sample_base + l / n;
+    if (bt.tile > 1u)
+    {
+        const ViewRec& v = bt.views[e.view];
+        pixel            = tile_pixel(pixel, v.width, v.height, bt.tile);
+    }
 }
 
 /* Paths of a batch that record training vertices: one per block of
@@ -1699,9 +1721,9 @@ static inline uint32_t grid_for(uint64_t items, uint32_t block, uint32_t max_blo
 }
 
 hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const GuideView& gv, const RayQueue& q, BatchCounters* bc,
-                         uint32_t blocks)
+                         uint32_t threads, uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_camera, dim3(blocks), dim3(256), 0, s, bt, gv, q, bc);
+    hipLaunchKernelGGL(k_camera, dim3(blocks), dim3(threads), 0, s, bt, gv, q, bc);
     return hipGetLastError();
 }
 
@@ -1717,9 +1739,9 @@ hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, c
 
 hipError_t launch_shade(hipStream_t s, const SceneView& sc, const BatchTable& bt, uint32_t depth, const RayQueue& qin,
                         const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, double* accum, BatchCounters* bc,
-                        uint32_t blocks)
+                        uint32_t threads, uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, sc, bt, depth, qin, hb, qout, sq, accum, bc);
+    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(threads), 0, s, sc, bt, depth, qin, hb, qout, sq, accum, bc);
     return hipGetLastError();
 }
 

@@ -394,7 +394,7 @@ class Integrator:
 
     # -- launch shape and batched multi-view rendering (no reference counterpart)
     LAUNCH_FIELDS = ("trace_blocks_per_cu", "shadow_blocks_per_cu", "shade_blocks_per_cu", "paths_per_batch",
-                     "graph", "order")
+                     "graph", "order", "shade_block", "tile")
 
     def setLaunch(self, **fields):
         """Sets the bounce loop's launch shape; omitted or zero fields take

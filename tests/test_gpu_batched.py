@@ -151,7 +151,8 @@ def test_launch_shape_does_not_change_results(c1_small):
     desc, integ = c1_small
     views = small_views(6, seed=9)
     shapes = [dict(), dict(trace_blocks_per_cu=1, shadow_blocks_per_cu=1, shade_blocks_per_cu=2),
-              dict(paths_per_batch=5000, order=2, graph=2)]
+              dict(paths_per_batch=5000, order=2, graph=2), dict(shade_block=64, tile=8),
+              dict(shade_block=128, tile=32, paths_per_batch=7000)]
     res = []
     try:
         for sh in shapes:
@@ -191,20 +192,27 @@ def test_launch_and_view_arguments_are_checked(c1_small):
     with pytest.raises(ClhipError):
         integ.setLaunch(paths_per_batch=100)
     with pytest.raises(ClhipError):
+        integ.setLaunch(shade_block=96)
+    with pytest.raises(ClhipError):
+        integ.setLaunch(tile=12)
+    with pytest.raises(ClhipError):
+        integ.setLaunch(tile=64)
+    with pytest.raises(ClhipError):
         integ.renderViews([View(8, 8, 1, 40, desc.camera)])  # depth above 32
     with pytest.raises(ClhipError):
         integ.renderViews([View(8, 8, 1, 4, desc.camera, lane=5000)])
     integ.setLaunch()
     got = integ.getLaunch()
     assert got["graph"] == 1 and got["trace_blocks_per_cu"] >= 1 and got["paths_per_batch"] >= 4096
+    assert got["shade_block"] == 256 and got["tile"] == 1
 
 
 def test_autotuner_logs_and_reapplies(c1_small, tmp_path):
     from colvillea_amd import autotune
     desc, integ = c1_small
     views = small_views(6, seed=21)
-    space = {"trace_blocks_per_cu": [1, 2], "shade_blocks_per_cu": [2, 8], "paths_per_batch": [4096, 1 << 16],
-             "order": [1, 2]}
+    space = {"trace_blocks_per_cu": [1, 2], "shade_blocks_per_cu": [2, 8], "shade_block": [256, 64],
+             "paths_per_batch": [4096, 1 << 16], "tile": [1, 8], "order": [1, 2]}
     try:
         integ.setLaunch()
         base = integ.getLaunch()
@@ -218,7 +226,7 @@ def test_autotuner_logs_and_reapplies(c1_small, tmp_path):
         integ.setLaunch()
         assert autotune.apply(integ, str(path)) == res["choice"]
         got = integ.getLaunch()
-        for k in ("trace_blocks_per_cu", "shade_blocks_per_cu", "paths_per_batch", "order"):
+        for k in ("trace_blocks_per_cu", "shade_blocks_per_cu", "shade_block", "paths_per_batch", "tile", "order"):
             assert got[k] == res["choice"][k], k
     finally:
         restore(integ, desc)
