@@ -36,7 +36,7 @@ def default_space(integ: Integrator) -> Dict[str, List[int]]:
         "shadow_blocks_per_cu": sorted({x for x in (max(1, d["shadow_blocks_per_cu"]), 3, 4, 6) if x <= max(1, d["shadow_blocks_per_cu"])}, reverse=True),
         "shade_blocks_per_cu": [d["shade_blocks_per_cu"], 4, 16],
         "shade_block": [d["shade_block"], 128, 64],
-        "tile": [d["tile"], 8, 16],
+        "tile": [d["tile"], 1, 8, 32],
         "paths_per_batch": [d["paths_per_batch"], 16 << 20, 32 << 20, 64 << 20],
         "order": [1, 2],
     }

@@ -338,8 +338,8 @@ typedef struct
     uint32_t graph;                /* 1: replay a captured hipGraph per batch; 2: plain launches */
     uint32_t order;                /* clhip_render_views: 1 = given order, 2 = least work first */
     uint32_t shade_block;          /* threads per block of the camera and shading kernels: 64, 128 or 256 (0 = 256) */
-    uint32_t tile;                 /* path order within a sample of a view: 1 = pixel rows (0 = 1), or square
-                                      tiles of 2, 4, 8, 16 or 32 pixels on a side, row by row within a tile */
+    uint32_t tile;                 /* path order within a sample of a view: 1 = pixel rows, or square tiles of
+                                      2, 4, 8, 16 or 32 pixels on a side, row by row within a tile (0 = 16) */
 } clhip_launch;
 
 int clhip_set_launch(clhip_ctx* ctx, const clhip_launch* launch);

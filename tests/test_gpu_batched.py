@@ -151,7 +151,7 @@ def test_launch_shape_does_not_change_results(c1_small):
     desc, integ = c1_small
     views = small_views(6, seed=9)
     shapes = [dict(), dict(trace_blocks_per_cu=1, shadow_blocks_per_cu=1, shade_blocks_per_cu=2),
-              dict(paths_per_batch=5000, order=2, graph=2), dict(shade_block=64, tile=8),
+              dict(paths_per_batch=5000, order=2, graph=2, tile=1), dict(shade_block=64, tile=8),
               dict(shade_block=128, tile=32, paths_per_batch=7000)]
     res = []
     try:
@@ -204,7 +204,7 @@ def test_launch_and_view_arguments_are_checked(c1_small):
     integ.setLaunch()
     got = integ.getLaunch()
     assert got["graph"] == 1 and got["trace_blocks_per_cu"] >= 1 and got["paths_per_batch"] >= 4096
-    assert got["shade_block"] == 256 and got["tile"] == 1
+    assert got["shade_block"] == 256 and got["tile"] == 16
 
 
 def test_autotuner_logs_and_reapplies(c1_small, tmp_path):
