@@ -827,6 +827,9 @@ __global__ __launch_bounds__(256, 5) void k_trace(SceneView sc, RayQueue q, cons
     }
 }
 
+/* Rays per thread in one class-grouping window of the shading kernel. */
+#define CLPT_SHADE_GROUP 4u
+
 __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint32_t depth, RayQueue qin, HitBuf hb,
                                                RayQueue qout, ShadowQueue sq, double* accum, BatchCounters* bc)
 {
@@ -835,35 +838,60 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
     const bool     dome   = sc.dome_index >= 0;
     const float    nl     = (float)sc.nemitters;
     const BatchRec b      = bt.batches[*bt.cursor];
-    __shared__ uint32_t order[256]; /* blockDim.x <= 256 (launch bounds) */
+    /* The block takes CLPT_SHADE_GROUP x blockDim.x consecutive rays at a
+     * time and hands them to its threads grouped by class (escaped, or the
+     * hit material's type), blockDim.x at a time, so a wave mostly runs one
+     * BSDF's code instead of every type present among its lanes; the wider
+     * the window, the fewer waves straddle two classes.  Results do not
+     * depend on the order (the RNG is keyed by path and sample); only
+     * float64 film sums and queue slots are taken in another order. */
+    __shared__ uint32_t order[CLPT_SHADE_GROUP * 256]; /* blockDim.x <= 256 (launch bounds) */
     __shared__ uint32_t ncls[8];
-    /* The block takes blockDim.x consecutive rays at a time and hands them to its
-     * threads grouped by class (escaped, or the hit material's type), so a
-     * wave mostly runs one BSDF's code instead of every type present among
-     * its lanes.  Results do not depend on the order (the RNG is keyed by
-     * path and sample); only float64 film sums and queue slots are taken in
-     * another order. */
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base = n - base > stride ? base + stride : n)
+    const uint32_t      win     = CLPT_SHADE_GROUP * blockDim.x;
+    const uint32_t      wstride = gridDim.x * win;
+    /* (window, part): every thread takes the same sequence, so the barriers
+     * of a window's sort are reached by the whole block */
+    for (uint32_t wbase = blockIdx.x * win, part = 0; wbase < n;)
     {
-        const uint32_t i0  = base + threadIdx.x;
-        uint32_t       cls = 7u; /* past the end */
-        if (i0 < n)
+        if (part == 0u)
         {
-            const float4 hh = hb.h[i0];
-            cls             = hh.x >= 0.0f ? 1u + sc.mtl[sc.inst[__float_as_uint(hh.w)].material].type : 0u;
+            uint32_t cls[CLPT_SHADE_GROUP], slot[CLPT_SHADE_GROUP];
+            if (threadIdx.x < 8u)
+                ncls[threadIdx.x] = 0u;
+            __syncthreads();
+#pragma unroll
+            for (uint32_t g = 0; g < CLPT_SHADE_GROUP; ++g)
+            {
+                const uint32_t i0 = wbase + g * blockDim.x + threadIdx.x;
+                cls[g]            = 7u; /* past the end */
+                if (i0 < n)
+                {
+                    const float4 hh = hb.h[i0];
+                    cls[g] = hh.x >= 0.0f ? 1u + sc.mtl[sc.inst[__float_as_uint(hh.w)].material].type : 0u;
+                }
+            }
+#pragma unroll
+            for (uint32_t g = 0; g < CLPT_SHADE_GROUP; ++g)
+                slot[g] = atomicAdd(&ncls[cls[g]], 1u);
+            __syncthreads();
+#pragma unroll
+            for (uint32_t g = 0; g < CLPT_SHADE_GROUP; ++g)
+            {
+                uint32_t first = 0u;
+                for (uint32_t c = 0; c < cls[g]; ++c)
+                    first += ncls[c];
+                order[first + slot[g]] = wbase + g * blockDim.x + threadIdx.x;
+            }
+            __syncthreads();
         }
-        if (threadIdx.x < 8u)
-            ncls[threadIdx.x] = 0u;
-        __syncthreads();
-        const uint32_t slot = atomicAdd(&ncls[cls], 1u);
-        __syncthreads();
-        uint32_t first = 0u;
-        for (uint32_t c = 0; c < cls; ++c)
-            first += ncls[c];
-        order[first + slot] = i0;
-        __syncthreads();
-        const uint32_t i = order[threadIdx.x];
-        __syncthreads();
+        const uint32_t i = order[part * blockDim.x + threadIdx.x];
+        if (++part == CLPT_SHADE_GROUP)
+        {
+            /* the next window's sort starts with a barrier, so no thread
+             * overwrites order while another still reads it */
+            part  = 0u;
+            wbase = n - wbase > wstride ? wbase + wstride : n;
+        }
         if (i >= n)
             continue;
         const float4   d4 = qin.d[i];
