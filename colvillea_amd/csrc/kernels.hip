@@ -1041,40 +1041,59 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
         __syncthreads();
     }
     const uint32_t n      = bc->rays[depth];
-    const uint32_t stride = gridDim.x * blockDim.x;
     const bool     dome   = sc.dome_index >= 0;
     const float    nl     = (float)sc.nemitters;
     const BatchRec b      = bt.batches[*bt.cursor];
     const uint32_t tstr   = guide_tstride(gv, b.npaths);
     const float    cb     = gv.c_bsdf;
-    __shared__ uint32_t order[256]; /* blockDim.x <= 256 (launch bounds) */
+    __shared__ uint32_t order[CLPT_SHADE_GROUP * 256]; /* blockDim.x <= 256 (launch bounds) */
     __shared__ uint32_t ncls[8];
-    /* rays handed to threads grouped by class, as in k_shade: a wave then
-     * mostly holds one material, and waves without a guided vertex skip
-     * the network */
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base = n - base > stride ? base + stride : n)
+    const uint32_t      win     = CLPT_SHADE_GROUP * blockDim.x;
+    const uint32_t      wstride = gridDim.x * win;
+    /* rays handed to threads grouped by class over windows of
+     * CLPT_SHADE_GROUP x blockDim.x rays, as in k_shade: a wave then mostly
+     * holds one material, and waves without a guided vertex skip the
+     * network.  Every thread takes the same (window, part) sequence and
+     * runs the body (the network needs the whole wave). */
+    for (uint32_t wbase = blockIdx.x * win, part = 0; wbase < n;)
     {
+        if (part == 0u)
         {
-            const uint32_t i0  = base + threadIdx.x;
-            uint32_t       cls = 7u; /* past the end */
-            if (i0 < n)
-            {
-                const float4 hh = hb.h[i0];
-                cls             = hh.x >= 0.0f ? 1u + sc.mtl[sc.inst[__float_as_uint(hh.w)].material].type : 0u;
-            }
+            uint32_t cls[CLPT_SHADE_GROUP], slot[CLPT_SHADE_GROUP];
             if (threadIdx.x < 8u)
                 ncls[threadIdx.x] = 0u;
             __syncthreads();
-            const uint32_t slot = atomicAdd(&ncls[cls], 1u);
+#pragma unroll
+            for (uint32_t g = 0; g < CLPT_SHADE_GROUP; ++g)
+            {
+                const uint32_t i0 = wbase + g * blockDim.x + threadIdx.x;
+                cls[g]            = 7u; /* past the end */
+                if (i0 < n)
+                {
+                    const float4 hh = hb.h[i0];
+                    cls[g] = hh.x >= 0.0f ? 1u + sc.mtl[sc.inst[__float_as_uint(hh.w)].material].type : 0u;
+                }
+            }
+#pragma unroll
+            for (uint32_t g = 0; g < CLPT_SHADE_GROUP; ++g)
+                slot[g] = atomicAdd(&ncls[cls[g]], 1u);
             __syncthreads();
-            uint32_t first = 0u;
-            for (uint32_t c = 0; c < cls; ++c)
-                first += ncls[c];
-            order[first + slot] = i0;
+#pragma unroll
+            for (uint32_t g = 0; g < CLPT_SHADE_GROUP; ++g)
+            {
+                uint32_t first = 0u;
+                for (uint32_t c = 0; c < cls[g]; ++c)
+                    first += ncls[c];
+                order[first + slot[g]] = wbase + g * blockDim.x + threadIdx.x;
+            }
             __syncthreads();
         }
-        const uint32_t i = order[threadIdx.x];
-        __syncthreads();
+        const uint32_t i = order[part * blockDim.x + threadIdx.x];
+        if (++part == CLPT_SHADE_GROUP)
+        {
+            part  = 0u;
+            wbase = n - wbase > wstride ? wbase + wstride : n;
+        }
         const bool active = i < n;
         float4         d4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), T4 = d4, h4 = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
         uint32_t       prim = 0u;
