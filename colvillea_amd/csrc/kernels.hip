@@ -256,13 +256,16 @@ __device__ __forceinline__ bool visit_node<8>(const SceneView& sc, const RayW& r
 #pragma unroll
     for (int c = 0; c < 8; ++c)
     {
+        /* plane = p + q s with q a byte and s a normal power of two (or 0):
+         * q s is exact, so the fused multiply-add rounds once, as the add
+         * after an exact product does -- the same bits, one instruction */
         const int   wi = c >> 2, sh = 8 * (c & 3);
-        const float ax = px + (float)((axw[wi] >> sh) & 0xffu) * sx;
-        const float ay = py + (float)((ayw[wi] >> sh) & 0xffu) * sy;
-        const float az = pz + (float)((azw[wi] >> sh) & 0xffu) * sz;
-        const float bx = px + (float)((bxw[wi] >> sh) & 0xffu) * sx;
-        const float by = py + (float)((byw[wi] >> sh) & 0xffu) * sy;
-        const float bz = pz + (float)((bzw[wi] >> sh) & 0xffu) * sz;
+        const float ax = __builtin_fmaf((float)((axw[wi] >> sh) & 0xffu), sx, px);
+        const float ay = __builtin_fmaf((float)((ayw[wi] >> sh) & 0xffu), sy, py);
+        const float az = __builtin_fmaf((float)((azw[wi] >> sh) & 0xffu), sz, pz);
+        const float bx = __builtin_fmaf((float)((bxw[wi] >> sh) & 0xffu), sx, px);
+        const float by = __builtin_fmaf((float)((byw[wi] >> sh) & 0xffu), sy, py);
+        const float bz = __builtin_fmaf((float)((bzw[wi] >> sh) & 0xffu), sz, pz);
         const float tnx = (ax - r.o.x) * r.inv.x, tfx = (bx - r.o.x) * r.inv.x;
         const float tny = (ay - r.o.y) * r.inv.y, tfy = (by - r.o.y) * r.inv.y;
         const float tnz = (az - r.o.z) * r.inv.z, tfz = (bz - r.o.z) * r.inv.z;
