@@ -177,9 +177,9 @@ __device__ __forceinline__ bool visit_node<4>(const SceneView& sc, const RayW& r
      * where it is negative, so the loads fetch the near and far plane of
      * every slab directly (offsets from the ray's sign bits) and no min/max
      * orders them: t = (plane - o) * inv is the same product as before, and
-     * multiplying by a positive (negative) factor keeps (reverses) order. */
-    /* buffer loads: 32-bit byte offsets into the node array (the host keeps
-     * it under 4 GiB), so each plane's address is one add */
+     * multiplying by a positive (negative) factor keeps (reverses) order.
+     * The loads are buffer loads at 32-bit byte offsets into the node array
+     * (the host keeps it under 4 GiB), so each plane's address is one add. */
     const __amdgpu_buffer_rsrc_t res =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(sc.nodes), (short)0, sc.node_bytes, 0x00020000);
     const uint32_t nb = ref * (uint32_t)sizeof(Node4);
