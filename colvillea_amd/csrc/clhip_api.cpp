@@ -157,7 +157,7 @@ struct clhip_ctx
 
     /* render state */
     DevBuf   d_accum, d_out;
-    DevBuf   d_q[2][3], d_hit, d_prim, d_sq[3], d_gstack, d_bc, d_tot, d_ctr;
+    DevBuf   d_q[2][3], d_hit, d_prim, d_sq[3], d_pacc, d_gstack, d_bc, d_tot, d_ctr;
     size_t   qcap = 0;
     uint32_t spp  = 0;
     uint32_t blocks_trace = 0, blocks_shadow = 0, blocks_shade = 0;
@@ -335,7 +335,7 @@ int sync_instances(clhip_ctx* c)
 /* Paths per batch: the launch setting, else 2^27.  A persistent traversal
  * launch finishes with its slowest lane, and the last bounces of a batch
  * hold few rays, so every batch pays a fixed cost whatever its size.  2^27
- * paths (64 spp of a 1080p film, 156 B of queues per path = 21 GB of the
+ * paths (64 spp of a 1080p film, 180 B of queues and path sums per path = 24 GB of the
  * 288 GB of HBM) make it small: 971 Msamples/s on the 1080p scene against
  * 949 at 2^26 and 913 at 2^25.  Buffers are sized by the batch a render
  * actually uses, so small films and few samples allocate less. */
@@ -363,6 +363,7 @@ int ensure_render_buffers(clhip_ctx* c, size_t paths)
         CLHIP_CHECK(c, c->d_sq[0].alloc(paths * sizeof(float4)));
         CLHIP_CHECK(c, c->d_sq[1].alloc(paths * sizeof(float4)));
         CLHIP_CHECK(c, c->d_sq[2].alloc(paths * sizeof(float2)));
+        CLHIP_CHECK(c, c->d_pacc.alloc(paths * 3 * sizeof(double)));
         c->qcap = paths;
     }
     CLHIP_CHECK(c, c->d_gstack.alloc(gthreads * CLPT_GSTACK_ENTRIES * sizeof(uint2)));
@@ -673,6 +674,7 @@ struct BatchArgs
     HitBuf         hb;
     ShadowQueue    sq;
     double*        accum;
+    double*        pacc; /* escaped-ray radiance per path slot, folded into accum at the batch end */
     BatchCounters* bc;
     TotalCounters* tot;
     uint2*         gst;
@@ -702,6 +704,7 @@ BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
     a.hb            = HitBuf{c->d_hit.as<float4>(), c->d_prim.as<uint32_t>()};
     a.sq            = ShadowQueue{c->d_sq[0].as<float4>(), c->d_sq[1].as<float4>(), c->d_sq[2].as<float2>(), nullptr};
     a.accum         = accum;
+    a.pacc          = c->d_pacc.as<double>();
     a.bc            = c->d_bc.as<BatchCounters>();
     a.tot           = c->d_tot.as<TotalCounters>();
     a.gst           = c->d_gstack.as<uint2>();
@@ -753,9 +756,9 @@ int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
                                                   a.blocks_trace));
             else if (kind == 1 && a.guided)
                 CLHIP_CHECK(c, clpt::launch_shade_guided(c->stream, a.sc, a.bt, a.gv, depth, qin, a.hb, qout, a.sq,
-                                                         a.accum, a.bc, a.blocks_guided));
+                                                         a.pacc, a.bc, a.blocks_guided));
             else if (kind == 1)
-                CLHIP_CHECK(c, clpt::launch_shade(c->stream, a.sc, a.bt, depth, qin, a.hb, qout, a.sq, a.accum, a.bc,
+                CLHIP_CHECK(c, clpt::launch_shade(c->stream, a.sc, a.bt, depth, qin, a.hb, qout, a.sq, a.pacc, a.bc,
                                                   a.shade_threads, a.blocks_shade));
             else
                 CLHIP_CHECK(c, clpt::launch_shadow(c->stream, a.sc, a.sq, depth, a.accum, a.bc, a.gst, a.ctr,
@@ -767,6 +770,7 @@ int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
             }
         }
     }
+    CLHIP_CHECK(c, clpt::launch_film_fold(c->stream, a.bt, a.pacc, a.accum, a.blocks_shade));
     if (a.gv.nslots)
         CLHIP_CHECK(c, clpt::launch_guide_gather(c->stream, a.gv, a.max_depth, a.gsamples, a.gcap, a.gc));
     CLHIP_CHECK(c, clpt::launch_batch_end(c->stream, a.bc, a.tot, a.bt.cursor));

@@ -56,6 +56,27 @@ __device__ __forceinline__ void wave_count(uint32_t* counter, bool pred)
         atomicAdd(counter, (uint32_t)__popcll(mask));
 }
 
+/* A path's radiance from escaped rays, gathered by shading in a float64
+ * triple per path slot of the batch and folded into the film at the batch's
+ * end (k_film_fold): depth 1 writes every path's slot (its escape radiance or
+ * zero), later depths add to it.  A path has at most one ray per depth, so no
+ * other lane touches the slot: plain stores instead of atomics on the film. */
+__device__ __forceinline__ void path_set(double* pacc, uint32_t path, f3 c)
+{
+    double* a = pacc + 3 * (size_t)path;
+    a[0]      = (double)c.x;
+    a[1]      = (double)c.y;
+    a[2]      = (double)c.z;
+}
+
+__device__ __forceinline__ void path_add(double* pacc, uint32_t path, f3 c)
+{
+    double* a = pacc + 3 * (size_t)path;
+    a[0] += (double)c.x;
+    a[1] += (double)c.y;
+    a[2] += (double)c.z;
+}
+
 __device__ __forceinline__ void accum_add(double* acc, uint32_t pixel, f3 c)
 {
     unsafeAtomicAdd(acc + 3 * (size_t)pixel + 0, (double)c.x);
@@ -834,7 +855,7 @@ __global__ __launch_bounds__(256, 5) void k_trace(SceneView sc, RayQueue q, cons
 #define CLPT_SHADE_GROUP 4u
 
 __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint32_t depth, RayQueue qin, HitBuf hb,
-                                               RayQueue qout, ShadowQueue sq, double* accum, BatchCounters* bc)
+                                               RayQueue qout, ShadowQueue sq, double* pacc, BatchCounters* bc)
 {
     const uint32_t n      = bc->rays[depth];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -915,23 +936,25 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
         bool           want_shadow = false, want_next = false;
         f3             Li{0.0f, 0.0f, 0.0f}, ldir{0.0f, 0.0f, 0.0f}, ndir{0.0f, 0.0f, 0.0f}, p{0.0f, 0.0f, 0.0f};
         float          npdf = 0.0f;
-        if (!hit)
+        /* escaped ray (raygeneration.cu:105-136, D3 corrected) */
+        f3 esc{0.0f, 0.0f, 0.0f};
+        if (!hit && dome)
         {
-            /* escaped ray (raygeneration.cu:105-136, D3 corrected) */
-            if (dome)
+            const f3 le = dome_le(sc, dir);
+            if (depth == 1)
+                esc = le;
+            else
             {
-                const f3 le = dome_le(sc, dir);
-                if (depth == 1)
-                    accum_add(accum, pixel, le);
-                else
-                {
-                    const float lp = dome_pdf(sc, dir) / nl;
-                    const float w  = T4.w / (T4.w + lp);
-                    accum_add(accum, pixel, mul3(T, scl3(le, w)));
-                }
+                const float lp = dome_pdf(sc, dir) / nl;
+                const float w  = T4.w / (T4.w + lp);
+                esc            = mul3(T, scl3(le, w));
             }
         }
-        else if (depth < fr.max_depth)
+        if (depth == 1u)
+            path_set(pacc, path, esc);
+        else if (!hit && dome)
+            path_add(pacc, path, esc);
+        if (hit && depth < fr.max_depth)
         {
             HitRec hr;
             hr.t    = h4.x;
@@ -1031,7 +1054,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
 template <bool FP8>
 __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTable bt, GuideView gv, uint32_t depth,
                                                          RayQueue qin, HitBuf hb, RayQueue qout, ShadowQueue sq,
-                                                         double* accum, BatchCounters* bc)
+                                                         double* pacc, BatchCounters* bc)
 {
     __shared__ uint4 wlds[CLPT_G_IMG * (FP8 ? 1 : 2) / 16];
     __shared__ float wscl[10];
@@ -1159,26 +1182,27 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
         bool  want_shadow = false, want_next = false;
         f3    Li{0.0f, 0.0f, 0.0f}, ldir{0.0f, 0.0f, 0.0f}, ndir{0.0f, 0.0f, 0.0f};
         float npdf = 0.0f;
-        if (active && !hit)
+        /* escaped ray: the path's slot as in k_shade */
+        f3 esc{0.0f, 0.0f, 0.0f};
+        if (active && !hit && dome)
         {
-            if (dome)
+            const f3 le = dome_le(sc, dir);
+            if (depth == 1)
+                esc = le;
+            else
             {
-                const f3 le = dome_le(sc, dir);
-                f3       cc;
-                if (depth == 1)
-                    cc = le;
-                else
-                {
-                    const float lp = dome_pdf(sc, dir) / nl;
-                    const float w  = T4.w / (T4.w + lp);
-                    cc             = mul3(T, scl3(le, w));
-                }
-                accum_add(accum, pixel, cc);
-                if (train)
-                    gv.E[rec] = make_float4(cc.x, cc.y, cc.z, 0.0f);
+                const float lp = dome_pdf(sc, dir) / nl;
+                const float w  = T4.w / (T4.w + lp);
+                esc            = mul3(T, scl3(le, w));
             }
+            if (train)
+                gv.E[rec] = make_float4(esc.x, esc.y, esc.z, 0.0f);
         }
-        else if (vertex)
+        if (active && depth == 1u)
+            path_set(pacc, path, esc);
+        else if (active && !hit && dome)
+            path_add(pacc, path, esc);
+        if (vertex)
         {
             const f3       woL = to_local(si.f, wo);
             const uint32_t k0  = 2u + 4u * (depth - 1u);
@@ -1344,6 +1368,45 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
         atomicAdd(ctr + 3, stt);
         if (sv)
             atomicAdd(&bc->visible[depth], (uint32_t)sv);
+    }
+}
+
+/* The batch's path sums (path_set/path_add) into the film: the first-sample
+ * path of each pixel of an entry adds the sums of the entry's samples of
+ * that pixel in sample order.  A view has one entry per batch, so each film
+ * pixel has one writer: plain float64 adds, and the film's sums do not
+ * depend on scheduling. */
+__global__ __launch_bounds__(256) void k_film_fold(BatchTable bt, const double* pacc, double* accum)
+{
+    const BatchRec b = bt.batches[*bt.cursor];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < b.npaths; p += gridDim.x * blockDim.x)
+    {
+        uint32_t lo = 0, hi = b.nentries;
+        while (hi - lo > 1)
+        {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (bt.entries[b.entry_off + mid].path_begin <= p)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const BatchEntry e = bt.entries[b.entry_off + lo];
+        const ViewRec&   v = bt.views[e.view];
+        const uint32_t   l = p - e.path_begin;
+        if (l >= v.npix)
+            continue;
+        const uint32_t end = lo + 1u < b.nentries ? bt.entries[b.entry_off + lo + 1u].path_begin : b.npaths;
+        double         s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (uint32_t q = p; q < end; q += v.npix)
+        {
+            s0 += pacc[3 * (size_t)q + 0];
+            s1 += pacc[3 * (size_t)q + 1];
+            s2 += pacc[3 * (size_t)q + 2];
+        }
+        const uint32_t pix = v.film_offset + (bt.tile > 1u ? tile_pixel(l, v.width, v.height, bt.tile) : l);
+        accum[3 * (size_t)pix + 0] += s0;
+        accum[3 * (size_t)pix + 1] += s1;
+        accum[3 * (size_t)pix + 2] += s2;
     }
 }
 
@@ -1788,23 +1851,23 @@ hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, c
 }
 
 hipError_t launch_shade(hipStream_t s, const SceneView& sc, const BatchTable& bt, uint32_t depth, const RayQueue& qin,
-                        const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, double* accum, BatchCounters* bc,
+                        const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, double* pacc, BatchCounters* bc,
                         uint32_t threads, uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(threads), 0, s, sc, bt, depth, qin, hb, qout, sq, accum, bc);
+    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(threads), 0, s, sc, bt, depth, qin, hb, qout, sq, pacc, bc);
     return hipGetLastError();
 }
 
 hipError_t launch_shade_guided(hipStream_t s, const SceneView& sc, const BatchTable& bt, const GuideView& gv,
                                uint32_t depth, const RayQueue& qin, const HitBuf& hb, const RayQueue& qout,
-                               const ShadowQueue& sq, double* accum, BatchCounters* bc, uint32_t blocks)
+                               const ShadowQueue& sq, double* pacc, BatchCounters* bc, uint32_t blocks)
 {
     if (gv.img8)
         hipLaunchKernelGGL(k_shade_guided<true>, dim3(blocks), dim3(256), 0, s, sc, bt, gv, depth, qin, hb, qout, sq,
-                           accum, bc);
+                           pacc, bc);
     else
         hipLaunchKernelGGL(k_shade_guided<false>, dim3(blocks), dim3(256), 0, s, sc, bt, gv, depth, qin, hb, qout, sq,
-                           accum, bc);
+                           pacc, bc);
     return hipGetLastError();
 }
 
@@ -1815,6 +1878,12 @@ hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& 
         hipLaunchKernelGGL(k_shadow<8>, dim3(blocks), dim3(256), 0, s, sc, sq, depth, accum, bc, gstack, ctr, guide_E);
     else
         hipLaunchKernelGGL(k_shadow<4>, dim3(blocks), dim3(256), 0, s, sc, sq, depth, accum, bc, gstack, ctr, guide_E);
+    return hipGetLastError();
+}
+
+hipError_t launch_film_fold(hipStream_t s, const BatchTable& bt, const double* pacc, double* accum, uint32_t blocks)
+{
+    hipLaunchKernelGGL(k_film_fold, dim3(blocks), dim3(256), 0, s, bt, pacc, accum);
     return hipGetLastError();
 }
 
