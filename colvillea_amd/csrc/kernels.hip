@@ -56,25 +56,18 @@ __device__ __forceinline__ void wave_count(uint32_t* counter, bool pred)
         atomicAdd(counter, (uint32_t)__popcll(mask));
 }
 
-/* A path's radiance from escaped rays, gathered by shading in a float64
+/* A path's radiance from its escaped ray, kept by shading in a float64
  * triple per path slot of the batch and folded into the film at the batch's
  * end (k_film_fold): depth 1 writes every path's slot (its escape radiance or
- * zero), later depths add to it.  A path has at most one ray per depth, so no
- * other lane touches the slot: plain stores instead of atomics on the film. */
+ * zero), and a path that escapes later writes its slot then.  An escape ends
+ * the path, so a slot takes at most one nonzero value and is written by one
+ * lane at a time: plain stores, no read, no atomics on the film. */
 __device__ __forceinline__ void path_set(double* pacc, uint32_t path, f3 c)
 {
     double* a = pacc + 3 * (size_t)path;
     a[0]      = (double)c.x;
     a[1]      = (double)c.y;
     a[2]      = (double)c.z;
-}
-
-__device__ __forceinline__ void path_add(double* pacc, uint32_t path, f3 c)
-{
-    double* a = pacc + 3 * (size_t)path;
-    a[0] += (double)c.x;
-    a[1] += (double)c.y;
-    a[2] += (double)c.z;
 }
 
 __device__ __forceinline__ void accum_add(double* acc, uint32_t pixel, f3 c)
@@ -950,10 +943,8 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
                 esc            = mul3(T, scl3(le, w));
             }
         }
-        if (depth == 1u)
+        if (depth == 1u || (!hit && dome))
             path_set(pacc, path, esc);
-        else if (!hit && dome)
-            path_add(pacc, path, esc);
         if (hit && depth < fr.max_depth)
         {
             HitRec hr;
@@ -1198,10 +1189,8 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
             if (train)
                 gv.E[rec] = make_float4(esc.x, esc.y, esc.z, 0.0f);
         }
-        if (active && depth == 1u)
+        if (active && (depth == 1u || (!hit && dome)))
             path_set(pacc, path, esc);
-        else if (active && !hit && dome)
-            path_add(pacc, path, esc);
         if (vertex)
         {
             const f3       woL = to_local(si.f, wo);
