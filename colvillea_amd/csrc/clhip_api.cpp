@@ -335,7 +335,7 @@ int sync_instances(clhip_ctx* c)
 /* Paths per batch: the launch setting, else 2^27.  A persistent traversal
  * launch finishes with its slowest lane, and the last bounces of a batch
  * hold few rays, so every batch pays a fixed cost whatever its size.  2^27
- * paths (64 spp of a 1080p film, 180 B of queues and path sums per path = 24 GB of the
+ * paths (64 spp of a 1080p film, 168 B of queues and path sums per path = 22.5 GB of the
  * 288 GB of HBM) make it small: 971 Msamples/s on the 1080p scene against
  * 949 at 2^26 and 913 at 2^25.  Buffers are sized by the batch a render
  * actually uses, so small films and few samples allocate less. */
@@ -363,7 +363,7 @@ int ensure_render_buffers(clhip_ctx* c, size_t paths)
         CLHIP_CHECK(c, c->d_sq[0].alloc(paths * sizeof(float4)));
         CLHIP_CHECK(c, c->d_sq[1].alloc(paths * sizeof(float4)));
         CLHIP_CHECK(c, c->d_sq[2].alloc(paths * sizeof(float2)));
-        CLHIP_CHECK(c, c->d_pacc.alloc(paths * 3 * sizeof(double)));
+        CLHIP_CHECK(c, c->d_pacc.alloc(paths * 3 * sizeof(float)));
         c->qcap = paths;
     }
     CLHIP_CHECK(c, c->d_gstack.alloc(gthreads * CLPT_GSTACK_ENTRIES * sizeof(uint2)));
@@ -674,7 +674,7 @@ struct BatchArgs
     HitBuf         hb;
     ShadowQueue    sq;
     double*        accum;
-    double*        pacc; /* escaped-ray radiance per path slot, folded into accum at the batch end */
+    float*         pacc; /* escaped-ray radiance per path slot, folded into accum at the batch end */
     BatchCounters* bc;
     TotalCounters* tot;
     uint2*         gst;
@@ -704,7 +704,7 @@ BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
     a.hb            = HitBuf{c->d_hit.as<float4>(), c->d_prim.as<uint32_t>()};
     a.sq            = ShadowQueue{c->d_sq[0].as<float4>(), c->d_sq[1].as<float4>(), c->d_sq[2].as<float2>(), nullptr};
     a.accum         = accum;
-    a.pacc          = c->d_pacc.as<double>();
+    a.pacc          = c->d_pacc.as<float>();
     a.bc            = c->d_bc.as<BatchCounters>();
     a.tot           = c->d_tot.as<TotalCounters>();
     a.gst           = c->d_gstack.as<uint2>();

@@ -12,17 +12,17 @@ hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const GuideView& g
 hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, const uint32_t* count, uint32_t depth,
                         const HitBuf& hb, uint2* gstack, unsigned long long* ctr, uint32_t blocks);
 hipError_t launch_shade(hipStream_t s, const SceneView& sc, const BatchTable& bt, uint32_t depth, const RayQueue& qin,
-                        const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, double* pacc, BatchCounters* bc,
+                        const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, float* pacc, BatchCounters* bc,
                         uint32_t threads, uint32_t blocks);
 /* Shading with the learned proposal mixed into the continuation (weights in LDS). */
 hipError_t launch_shade_guided(hipStream_t s, const SceneView& sc, const BatchTable& bt, const GuideView& gv,
                                uint32_t depth, const RayQueue& qin, const HitBuf& hb, const RayQueue& qout,
-                               const ShadowQueue& sq, double* pacc, BatchCounters* bc, uint32_t blocks);
+                               const ShadowQueue& sq, float* pacc, BatchCounters* bc, uint32_t blocks);
 hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& sq, uint32_t depth, double* accum,
                          BatchCounters* bc, uint2* gstack, unsigned long long* ctr, float4* guide_E, uint32_t blocks);
-/* Escaped-ray radiance of the batch's paths (3 doubles per path slot, written
- * by shading) into the film, one thread per pixel and view. */
-hipError_t launch_film_fold(hipStream_t s, const BatchTable& bt, const double* pacc, double* accum, uint32_t blocks);
+/* Escaped-ray radiance of the batch's paths (3 floats per path slot, written
+ * by shading) into the float64 film, one thread per pixel and view. */
+hipError_t launch_film_fold(hipStream_t s, const BatchTable& bt, const float* pacc, double* accum, uint32_t blocks);
 hipError_t launch_batch_end(hipStream_t s, const BatchCounters* bc, TotalCounters* tot, uint32_t* cursor);
 hipError_t launch_set_cursor(hipStream_t s, uint32_t* cursor, uint32_t value);
 hipError_t launch_resolve(hipStream_t s, const double* accum, uint32_t npix, uint32_t spp, int srgb, float4* out);

@@ -56,18 +56,19 @@ __device__ __forceinline__ void wave_count(uint32_t* counter, bool pred)
         atomicAdd(counter, (uint32_t)__popcll(mask));
 }
 
-/* A path's radiance from its escaped ray, kept by shading in a float64
- * triple per path slot of the batch and folded into the film at the batch's
- * end (k_film_fold): depth 1 writes every path's slot (its escape radiance or
+/* A path's radiance from its escaped ray, kept by shading in a float32
+ * triple per path slot of the batch (the value is a float32 product, so
+ * nothing is rounded) and folded into the float64 film at the batch's end
+ * (k_film_fold): depth 1 writes every path's slot (its escape radiance or
  * zero), and a path that escapes later writes its slot then.  An escape ends
  * the path, so a slot takes at most one nonzero value and is written by one
  * lane at a time: plain stores, no read, no atomics on the film. */
-__device__ __forceinline__ void path_set(double* pacc, uint32_t path, f3 c)
+__device__ __forceinline__ void path_set(float* pacc, uint32_t path, f3 c)
 {
-    double* a = pacc + 3 * (size_t)path;
-    a[0]      = (double)c.x;
-    a[1]      = (double)c.y;
-    a[2]      = (double)c.z;
+    float* a = pacc + 3 * (size_t)path;
+    a[0]     = c.x;
+    a[1]     = c.y;
+    a[2]     = c.z;
 }
 
 __device__ __forceinline__ void accum_add(double* acc, uint32_t pixel, f3 c)
@@ -848,7 +849,7 @@ __global__ __launch_bounds__(256, 5) void k_trace(SceneView sc, RayQueue q, cons
 #define CLPT_SHADE_GROUP 4u
 
 __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint32_t depth, RayQueue qin, HitBuf hb,
-                                               RayQueue qout, ShadowQueue sq, double* pacc, BatchCounters* bc)
+                                               RayQueue qout, ShadowQueue sq, float* pacc, BatchCounters* bc)
 {
     const uint32_t n      = bc->rays[depth];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -1045,7 +1046,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
 template <bool FP8>
 __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTable bt, GuideView gv, uint32_t depth,
                                                          RayQueue qin, HitBuf hb, RayQueue qout, ShadowQueue sq,
-                                                         double* pacc, BatchCounters* bc)
+                                                         float* pacc, BatchCounters* bc)
 {
     __shared__ uint4 wlds[CLPT_G_IMG * (FP8 ? 1 : 2) / 16];
     __shared__ float wscl[10];
@@ -1365,7 +1366,7 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
  * that pixel in sample order.  A view has one entry per batch, so each film
  * pixel has one writer: plain float64 adds, and the film's sums do not
  * depend on scheduling. */
-__global__ __launch_bounds__(256) void k_film_fold(BatchTable bt, const double* pacc, double* accum)
+__global__ __launch_bounds__(256) void k_film_fold(BatchTable bt, const float* pacc, double* accum)
 {
     const BatchRec b = bt.batches[*bt.cursor];
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < b.npaths; p += gridDim.x * blockDim.x)
@@ -1388,9 +1389,9 @@ __global__ __launch_bounds__(256) void k_film_fold(BatchTable bt, const double* 
         double         s0 = 0.0, s1 = 0.0, s2 = 0.0;
         for (uint32_t q = p; q < end; q += v.npix)
         {
-            s0 += pacc[3 * (size_t)q + 0];
-            s1 += pacc[3 * (size_t)q + 1];
-            s2 += pacc[3 * (size_t)q + 2];
+            s0 += (double)pacc[3 * (size_t)q + 0];
+            s1 += (double)pacc[3 * (size_t)q + 1];
+            s2 += (double)pacc[3 * (size_t)q + 2];
         }
         const uint32_t pix = v.film_offset + (bt.tile > 1u ? tile_pixel(l, v.width, v.height, bt.tile) : l);
         accum[3 * (size_t)pix + 0] += s0;
@@ -1840,7 +1841,7 @@ hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, c
 }
 
 hipError_t launch_shade(hipStream_t s, const SceneView& sc, const BatchTable& bt, uint32_t depth, const RayQueue& qin,
-                        const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, double* pacc, BatchCounters* bc,
+                        const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, float* pacc, BatchCounters* bc,
                         uint32_t threads, uint32_t blocks)
 {
     hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(threads), 0, s, sc, bt, depth, qin, hb, qout, sq, pacc, bc);
@@ -1849,7 +1850,7 @@ hipError_t launch_shade(hipStream_t s, const SceneView& sc, const BatchTable& bt
 
 hipError_t launch_shade_guided(hipStream_t s, const SceneView& sc, const BatchTable& bt, const GuideView& gv,
                                uint32_t depth, const RayQueue& qin, const HitBuf& hb, const RayQueue& qout,
-                               const ShadowQueue& sq, double* pacc, BatchCounters* bc, uint32_t blocks)
+                               const ShadowQueue& sq, float* pacc, BatchCounters* bc, uint32_t blocks)
 {
     if (gv.img8)
         hipLaunchKernelGGL(k_shade_guided<true>, dim3(blocks), dim3(256), 0, s, sc, bt, gv, depth, qin, hb, qout, sq,
@@ -1870,7 +1871,7 @@ hipError_t launch_shadow(hipStream_t s, const SceneView& sc, const ShadowQueue& 
     return hipGetLastError();
 }
 
-hipError_t launch_film_fold(hipStream_t s, const BatchTable& bt, const double* pacc, double* accum, uint32_t blocks)
+hipError_t launch_film_fold(hipStream_t s, const BatchTable& bt, const float* pacc, double* accum, uint32_t blocks)
 {
     hipLaunchKernelGGL(k_film_fold, dim3(blocks), dim3(256), 0, s, bt, pacc, accum);
     return hipGetLastError();
