@@ -752,7 +752,7 @@ int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
                 CLHIP_CHECK(c, hipEventRecord(tl.a, c->stream));
             }
             if (kind == 0)
-                CLHIP_CHECK(c, clpt::launch_trace(c->stream, a.sc, qin, &a.bc->rays[depth], depth, a.hb, a.gst, a.ctr,
+                CLHIP_CHECK(c, clpt::launch_trace(c->stream, a.sc, qin, bc_rays(a.bc, depth), depth, a.hb, a.gst, a.ctr,
                                                   a.blocks_trace));
             else if (kind == 1 && a.guided)
                 CLHIP_CHECK(c, clpt::launch_shade_guided(c->stream, a.sc, a.bt, a.gv, depth, qin, a.hb, qout, a.sq,

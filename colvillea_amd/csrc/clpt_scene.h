@@ -262,13 +262,24 @@ struct GuideView
  * work.  Layout of the u32 array: [0..MAXD] ray counts (set by the producer
  * of each queue), [MAXD+1 ..] hits, shadows, visible. */
 #define CLPT_MAXD 32
-struct BatchCounters
+/* Per-batch queue counts.  The rays of depth d + 1 and the shadow rays of
+ * depth d are interleaved (rs[2 d] = shadow rays of depth d, rs[2 d + 1] =
+ * rays of depth d + 1), so that shading reserves both of its output queues
+ * with one 64-bit atomic on &rs[2 d]. */
+struct alignas(8) BatchCounters
 {
-    uint32_t rays[CLPT_MAXD + 2];
+    uint32_t rs[2 * (CLPT_MAXD + 2)];
     uint32_t hits[CLPT_MAXD + 2];
-    uint32_t shadow[CLPT_MAXD + 2];
     uint32_t visible[CLPT_MAXD + 2];
 };
+
+/* rays of depth d >= 1, shadow rays of depth d */
+__host__ __device__ inline uint32_t* bc_rays(BatchCounters* bc, uint32_t d) { return &bc->rs[2 * d - 1]; }
+__host__ __device__ inline uint32_t* bc_shadow(BatchCounters* bc, uint32_t d) { return &bc->rs[2 * d]; }
+__host__ __device__ inline unsigned long long* bc_push(BatchCounters* bc, uint32_t d)
+{
+    return reinterpret_cast<unsigned long long*>(&bc->rs[2 * d]);
+}
 
 struct TotalCounters
 {

@@ -27,19 +27,26 @@
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-/* Reserve `want` slots with one atomic per wave; returns this lane's slot. */
-__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool want)
+/* Reserves slots in two queues with one atomic per wave: the wave's count
+ * for the first queue goes to the low word of a 64-bit counter and for the
+ * second to the high word (a count stays below 2^32, so the low word never
+ * carries); each lane gets its slot in each queue. */
+__device__ __forceinline__ void wave_push2(unsigned long long* counter, bool want_a, bool want_b, uint32_t& slot_a,
+                                           uint32_t& slot_b)
 {
-    const unsigned long long mask = __ballot(want);
-    if (mask == 0ull)
-        return 0u;
-    const uint32_t leader = (uint32_t)(__ffsll((long long)mask) - 1);
-    uint32_t       base   = 0;
+    const unsigned long long ma = __ballot(want_a), mb = __ballot(want_b);
+    slot_a = slot_b = 0u;
+    if ((ma | mb) == 0ull)
+        return;
+    const unsigned long long active = __ballot(true);
+    const uint32_t           leader = (uint32_t)(__ffsll((long long)active) - 1);
+    unsigned long long       base   = 0ull;
     if (lane_id() == leader)
-        base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
-    const unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
-    return base + (uint32_t)__popcll(below);
+        base = atomicAdd(counter, (unsigned long long)__popcll(ma) | ((unsigned long long)__popcll(mb) << 32));
+    const uint32_t lo = __shfl((uint32_t)base, (int)leader), hi = __shfl((uint32_t)(base >> 32), (int)leader);
+    const unsigned long long below = (1ull << lane_id()) - 1ull;
+    slot_a                         = lo + (uint32_t)__popcll(ma & below);
+    slot_b                         = hi + (uint32_t)__popcll(mb & below);
 }
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
@@ -759,12 +766,12 @@ __global__ __launch_bounds__(256) void k_camera(BatchTable bt, GuideView gv, Ray
     {
         for (int k = 0; k < CLPT_MAXD + 2; ++k)
         {
-            bc->rays[k]    = 0u;
-            bc->hits[k]    = 0u;
-            bc->shadow[k]  = 0u;
-            bc->visible[k] = 0u;
+            bc->rs[2 * k]     = 0u;
+            bc->rs[2 * k + 1] = 0u;
+            bc->hits[k]       = 0u;
+            bc->visible[k]    = 0u;
         }
-        bc->rays[1] = npaths;
+        *bc_rays(bc, 1) = npaths;
     }
     for (uint32_t i = gid; i < npaths; i += gridDim.x * blockDim.x)
     {
@@ -851,7 +858,7 @@ __global__ __launch_bounds__(256, 5) void k_trace(SceneView sc, RayQueue q, cons
 __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint32_t depth, RayQueue qin, HitBuf hb,
                                                RayQueue qout, ShadowQueue sq, float* pacc, BatchCounters* bc)
 {
-    const uint32_t n      = bc->rays[depth];
+    const uint32_t n      = *bc_rays(bc, depth);
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool     dome   = sc.dome_index >= 0;
     const float    nl     = (float)sc.nemitters;
@@ -1014,14 +1021,14 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
         }
         /* converged: wave-aggregated queue pushes */
         wave_count(&bc->hits[depth], hit);
-        const uint32_t ss = wave_push(&bc->shadow[depth], want_shadow);
+        uint32_t ss, ns;
+        wave_push2(bc_push(bc, depth), want_shadow, want_next, ss, ns);
         if (want_shadow)
         {
             sq.o[ss] = make_float4(p.x, p.y, p.z, Li.x);
             sq.d[ss] = make_float4(ldir.x, ldir.y, ldir.z, Li.y);
             sq.x[ss] = make_float2(Li.z, __uint_as_float(pixel));
         }
-        const uint32_t ns = wave_push(&bc->rays[depth + 1], want_next);
         if (want_next)
         {
             qout.o[ns] = make_float4(p.x, p.y, p.z, 0.0f);
@@ -1058,7 +1065,7 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
             wscl[threadIdx.x] = FP8 ? gv.scl[threadIdx.x] : 1.0f;
         __syncthreads();
     }
-    const uint32_t n      = bc->rays[depth];
+    const uint32_t n      = *bc_rays(bc, depth);
     const bool     dome   = sc.dome_index >= 0;
     const float    nl     = (float)sc.nemitters;
     const BatchRec b      = bt.batches[*bt.cursor];
@@ -1277,7 +1284,8 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
             }
         }
         wave_count(&bc->hits[depth], hit);
-        const uint32_t ss = wave_push(&bc->shadow[depth], want_shadow);
+        uint32_t ss, ns;
+        wave_push2(bc_push(bc, depth), want_shadow, want_next, ss, ns);
         if (want_shadow)
         {
             sq.o[ss]   = make_float4(si.p.x, si.p.y, si.p.z, Li.x);
@@ -1285,7 +1293,6 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
             sq.x[ss]   = make_float2(Li.z, __uint_as_float(pixel));
             sq.tag[ss] = train ? (uint32_t)rec : 0xffffffffu;
         }
-        const uint32_t ns = wave_push(&bc->rays[depth + 1], want_next);
         if (want_next)
         {
             qout.o[ns] = make_float4(si.p.x, si.p.y, si.p.z, 0.0f);
@@ -1302,7 +1309,7 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
 {
     CLPT_DECLARE_STACK(st, gstack);
     unsigned long long sn = 0, stt = 0, sv = 0; /* the wave's node visits, triangle tests, unoccluded rays */
-    const uint32_t     n      = bc->shadow[depth];
+    const uint32_t     n      = *bc_shadow(bc, depth);
     WavePool       pool{st.gid - lane_id(), st.gstride, 64u};
     uint32_t       i      = st.gid;
     bool           active = i < n;
@@ -1405,9 +1412,9 @@ __global__ void k_batch_end(const BatchCounters* bc, TotalCounters* tot, uint32_
     const int b = threadIdx.x;
     if (b < CLPT_MAXD + 2)
     {
-        tot->rays[b] += bc->rays[b];
+        tot->rays[b] += b >= 1 ? bc->rs[2 * b - 1] : 0u;
         tot->hits[b] += bc->hits[b];
-        tot->shadow[b] += bc->shadow[b];
+        tot->shadow[b] += bc->rs[2 * b];
         tot->visible[b] += bc->visible[b];
     }
     if (b == 0)
