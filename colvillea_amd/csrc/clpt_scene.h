@@ -122,7 +122,7 @@ struct alignas(16) EmitterRec
 };
 
 /* Buckets of the HDRI CDF search cut points (SceneView::cutv, cutcond). */
-#define CLPT_CUTS 256u
+#define CLPT_CUTS 1024u
 
 /* Everything a kernel needs to find the scene; passed by value. */
 struct SceneView
@@ -148,8 +148,8 @@ struct SceneView
     const float*       cdfcond;
     const float*       pv;
     const float*       cdfv;
-    const uint32_t*    cutv;    /* 257 search cut points of cdfv (CLPT_CUTS buckets) */
-    const uint32_t*    cutcond; /* 257 per row of cdfcond */
+    const uint32_t*    cutv;    /* CLPT_CUTS + 1 search cut points of cdfv */
+    const uint32_t*    cutcond; /* CLPT_CUTS + 1 per row of cdfcond */
     uint32_t           tlas_root;
     uint32_t           ninst;
     uint32_t           bvh_width; /* 4 (Node4) or 8 (Node8) */

@@ -1607,7 +1607,7 @@ __global__ __launch_bounds__(256) void k_dome_marginal(uint32_t h, float* pv, fl
 }
 
 /* Search cut points of the row CDFs (blocks 0..h-1) and of the marginal
- * (block h): cut[k] = upper_bound over the CDF's entries of k / 256. */
+ * (block h): cut[k] = upper_bound over the CDF's entries of k / CLPT_CUTS. */
 __global__ __launch_bounds__(256) void k_dome_cuts(uint32_t w, uint32_t h, const float* cdfcond, const float* cdfv,
                                                     uint32_t* cutv, uint32_t* cutcond)
 {
