@@ -1368,42 +1368,33 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
     }
 }
 
-/* The batch's path sums (path_set/path_add) into the film: the first-sample
- * path of each pixel of an entry adds the sums of the entry's samples of
- * that pixel in sample order.  A view has one entry per batch, so each film
- * pixel has one writer: plain float64 adds, and the film's sums do not
- * depend on scheduling. */
+/* The batch's path sums (path_set) into the film: for each entry, one thread
+ * per pixel adds the sums of the entry's samples of that pixel, in sample
+ * order.  A view has one entry per batch, so each film pixel has one writer:
+ * plain float64 adds, and the film's sums do not depend on scheduling. */
 __global__ __launch_bounds__(256) void k_film_fold(BatchTable bt, const float* pacc, double* accum)
 {
-    const BatchRec b = bt.batches[*bt.cursor];
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < b.npaths; p += gridDim.x * blockDim.x)
+    const BatchRec b      = bt.batches[*bt.cursor];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t en = 0; en < b.nentries; ++en)
     {
-        uint32_t lo = 0, hi = b.nentries;
-        while (hi - lo > 1)
+        const BatchEntry e   = bt.entries[b.entry_off + en];
+        const ViewRec&   v   = bt.views[e.view];
+        const uint32_t   end = en + 1u < b.nentries ? bt.entries[b.entry_off + en + 1u].path_begin : b.npaths;
+        for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < v.npix; l += stride)
         {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (bt.entries[b.entry_off + mid].path_begin <= p)
-                lo = mid;
-            else
-                hi = mid;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (uint32_t q = e.path_begin + l; q < end; q += v.npix)
+            {
+                s0 += (double)pacc[3 * (size_t)q + 0];
+                s1 += (double)pacc[3 * (size_t)q + 1];
+                s2 += (double)pacc[3 * (size_t)q + 2];
+            }
+            const uint32_t pix = v.film_offset + (bt.tile > 1u ? tile_pixel(l, v.width, v.height, bt.tile) : l);
+            accum[3 * (size_t)pix + 0] += s0;
+            accum[3 * (size_t)pix + 1] += s1;
+            accum[3 * (size_t)pix + 2] += s2;
         }
-        const BatchEntry e = bt.entries[b.entry_off + lo];
-        const ViewRec&   v = bt.views[e.view];
-        const uint32_t   l = p - e.path_begin;
-        if (l >= v.npix)
-            continue;
-        const uint32_t end = lo + 1u < b.nentries ? bt.entries[b.entry_off + lo + 1u].path_begin : b.npaths;
-        double         s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (uint32_t q = p; q < end; q += v.npix)
-        {
-            s0 += (double)pacc[3 * (size_t)q + 0];
-            s1 += (double)pacc[3 * (size_t)q + 1];
-            s2 += (double)pacc[3 * (size_t)q + 2];
-        }
-        const uint32_t pix = v.film_offset + (bt.tile > 1u ? tile_pixel(l, v.width, v.height, bt.tile) : l);
-        accum[3 * (size_t)pix + 0] += s0;
-        accum[3 * (size_t)pix + 1] += s1;
-        accum[3 * (size_t)pix + 2] += s2;
     }
 }
 
