@@ -680,6 +680,7 @@ struct BatchArgs
     uint2*         gst;
     unsigned long long* ctr;
     uint32_t       max_depth, blocks_trace, blocks_shadow, blocks_shade, shade_threads;
+    const float*   cam_o; /* the single view's camera position (device), or null */
     GuideView      gv;
     uint32_t       guided, blocks_guided;
     GuideSample*   gsamples; /* gather target when recording */
@@ -687,7 +688,9 @@ struct BatchArgs
     uint32_t       gcap;
 };
 
-BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
+/* single_view: every batch holds one view (view 0), so camera rays share its
+ * position and the origin plane of depth 1 is not written */
+BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth, bool single_view)
 {
     BatchArgs a{};
     a.sc = scene_view(c);
@@ -725,6 +728,9 @@ BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
     a.blocks_shadow = c->blocks_shadow;
     a.blocks_shade  = c->blocks_shade;
     a.shade_threads = c->launch.shade_block ? c->launch.shade_block : 256u;
+    a.cam_o         = single_view ? reinterpret_cast<const float*>(reinterpret_cast<const char*>(c->d_views.p) +
+                                                                   offsetof(ViewRec, pos))
+                                  : nullptr;
     return a;
 }
 
@@ -734,7 +740,8 @@ BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth)
  * bracket every traversal and shading launch. */
 int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
 {
-    CLHIP_CHECK(c, clpt::launch_camera(c->stream, a.bt, a.gv, a.q[0], a.bc, a.shade_threads, a.blocks_shade));
+    CLHIP_CHECK(c, clpt::launch_camera(c->stream, a.bt, a.gv, a.q[0], a.bc, a.cam_o ? 0u : 1u, a.shade_threads,
+                                       a.blocks_shade));
     for (uint32_t depth = 1; depth <= a.max_depth; ++depth)
     {
         const RayQueue& qin  = a.q[(depth - 1) & 1];
@@ -753,7 +760,7 @@ int issue_batch(clhip_ctx* c, const BatchArgs& a, bool timed)
             }
             if (kind == 0)
                 CLHIP_CHECK(c, clpt::launch_trace(c->stream, a.sc, qin, bc_rays(a.bc, depth), depth, a.hb, a.gst, a.ctr,
-                                                  a.blocks_trace));
+                                                  depth == 1 ? a.cam_o : nullptr, a.blocks_trace));
             else if (kind == 1 && a.guided)
                 CLHIP_CHECK(c, clpt::launch_shade_guided(c->stream, a.sc, a.bt, a.gv, depth, qin, a.hb, qout, a.sq,
                                                          a.pacc, a.bc, a.blocks_guided));
@@ -859,7 +866,8 @@ int batch_graph(clhip_ctx* c, const BatchArgs& a, hipGraphExec_t* out)
 /* Runs batches 0..nbatches-1 of the uploaded tables into `accum`.  Counters
  * and the render events cover this call; with `batch_events` an event is
  * recorded after every batch. */
-int run_batches(clhip_ctx* c, uint32_t nbatches, uint32_t max_depth, double* accum, bool batch_events)
+int run_batches(clhip_ctx* c, uint32_t nbatches, uint32_t max_depth, double* accum, bool batch_events,
+                bool single_view)
 {
     CLHIP_CHECK(c, hipMemsetAsync(c->d_tot.p, 0, sizeof(TotalCounters), c->stream));
     CLHIP_CHECK(c, hipMemsetAsync(c->d_ctr.p, 0, 4 * sizeof(unsigned long long), c->stream));
@@ -887,7 +895,7 @@ int run_batches(clhip_ctx* c, uint32_t nbatches, uint32_t max_depth, double* acc
             return rc;
     }
     CLHIP_CHECK(c, clpt::launch_set_cursor(c->stream, c->d_cursor.as<uint32_t>(), 0u));
-    BatchArgs      a = batch_args(c, accum, max_depth);
+    BatchArgs      a = batch_args(c, accum, max_depth, single_view);
     hipGraphExec_t x = nullptr;
     if (use_graph(c))
     {
@@ -902,7 +910,7 @@ int run_batches(clhip_ctx* c, uint32_t nbatches, uint32_t max_depth, double* acc
          * arguments (and graph) per batch while guiding */
         if (guiding(c) && b > 0)
         {
-            const BatchArgs a2 = batch_args(c, accum, max_depth);
+            const BatchArgs a2 = batch_args(c, accum, max_depth, single_view);
             if (std::memcmp(&a2, &a, sizeof(a)) != 0)
             {
                 a = a2;
@@ -1731,7 +1739,7 @@ int clhip_render(clhip_ctx* ctx, uint32_t spp)
     rc = upload_tables(ctx, views, batches, entries);
     if (rc)
         return rc;
-    rc = run_batches(ctx, (uint32_t)batches.size(), v.max_depth, ctx->d_accum.as<double>(), false);
+    rc = run_batches(ctx, (uint32_t)batches.size(), v.max_depth, ctx->d_accum.as<double>(), false, true);
     if (rc)
         return rc;
     ctx->spp += spp;
@@ -1884,7 +1892,7 @@ int clhip_render_views(clhip_ctx* ctx, const clhip_view* views, uint32_t nviews)
     rc = upload_tables(ctx, ctx->vviews, batches, entries);
     if (rc)
         return rc;
-    return run_batches(ctx, (uint32_t)batches.size(), maxd, ctx->d_vaccum.as<double>(), true);
+    return run_batches(ctx, (uint32_t)batches.size(), maxd, ctx->d_vaccum.as<double>(), true, false);
 }
 
 int clhip_view_sums(clhip_ctx* ctx, uint32_t view, double* rgb, size_t bytes)

@@ -6,11 +6,14 @@
 
 namespace clpt
 {
-/* `threads` per block (64, 128 or 256) for the camera and plain shading kernels */
+/* `threads` per block (64, 128 or 256) for the camera and plain shading
+ * kernels; write_o = 0 leaves the origin plane unwritten for a batch of one
+ * view, whose closest-hit launch at depth 1 then takes cam_o (the view's
+ * camera position in device memory) as every ray's origin */
 hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const GuideView& gv, const RayQueue& q, BatchCounters* bc,
-                         uint32_t threads, uint32_t blocks);
+                         uint32_t write_o, uint32_t threads, uint32_t blocks);
 hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, const uint32_t* count, uint32_t depth,
-                        const HitBuf& hb, uint2* gstack, unsigned long long* ctr, uint32_t blocks);
+                        const HitBuf& hb, uint2* gstack, unsigned long long* ctr, const float* cam_o, uint32_t blocks);
 hipError_t launch_shade(hipStream_t s, const SceneView& sc, const BatchTable& bt, uint32_t depth, const RayQueue& qin,
                         const HitBuf& hb, const RayQueue& qout, const ShadowQueue& sq, float* pacc, BatchCounters* bc,
                         uint32_t threads, uint32_t blocks);

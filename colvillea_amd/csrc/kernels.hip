@@ -756,7 +756,10 @@ __device__ __forceinline__ bool guide_records(uint32_t path, uint32_t tstr, uint
     return path % tstr == h % tstr;
 }
 
-__global__ __launch_bounds__(256) void k_camera(BatchTable bt, GuideView gv, RayQueue q, BatchCounters* bc)
+/* write_o = 0: the batch's paths all start at one camera position, which
+ * traversal takes from the view (the origin plane is not written) */
+__global__ __launch_bounds__(256) void k_camera(BatchTable bt, GuideView gv, RayQueue q, BatchCounters* bc,
+                                                uint32_t write_o)
 {
     const uint32_t gid    = blockIdx.x * blockDim.x + threadIdx.x;
     const BatchRec b      = bt.batches[*bt.cursor];
@@ -780,7 +783,8 @@ __global__ __launch_bounds__(256) void k_camera(BatchTable bt, GuideView gv, Ray
         const ViewRec v = bt.views[vi];
         f3            o, d;
         camera_ray(v, pixel % v.width, pixel / v.width, sample, &o, &d);
-        q.o[i] = make_float4(o.x, o.y, o.z, 0.0f);
+        if (write_o)
+            q.o[i] = make_float4(o.x, o.y, o.z, 0.0f);
         q.d[i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));
         /* the throughput plane is not written: a camera path's throughput is
          * 1, and shading at depth 1 takes it as such */
@@ -801,19 +805,24 @@ __global__ __launch_bounds__(256) void k_camera(BatchTable bt, GuideView gv, Ray
 
 template <int W>
 __global__ __launch_bounds__(256, 5) void k_trace(SceneView sc, RayQueue q, const uint32_t* count, uint32_t depth,
-                                               HitBuf hb, uint2* gstack, unsigned long long* ctr)
+                                               HitBuf hb, uint2* gstack, unsigned long long* ctr,
+                                               const float* cam_o)
 {
     CLPT_DECLARE_STACK(st, gstack);
     unsigned long long sn = 0, stt = 0; /* the wave's node visits and triangle tests */
     const uint32_t     n    = *count;
     const float        tmin = depth == 1 ? 0.0f : CLPT_RAY_EPS;
+    /* camera rays of a single view: every origin is the camera position
+     * (cam_o, the view's pos), not read from the origin plane */
+    const f3       co     = cam_o ? f3{cam_o[0], cam_o[1], cam_o[2]} : f3{0.0f, 0.0f, 0.0f};
     WavePool       pool{st.gid - lane_id(), st.gstride, 64u};
     uint32_t       i      = st.gid;
     bool           active = i < n;
     Tracer<false, W> tr;
     if (active)
     {
-        const float4 o = q.o[i], d = q.d[i];
+        const float4 d = q.d[i];
+        const float4 o = cam_o ? make_float4(co.x, co.y, co.z, 0.0f) : q.o[i];
         tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_FLT_MAX);
     }
     while (__any(active))
@@ -838,7 +847,8 @@ __global__ __launch_bounds__(256, 5) void k_trace(SceneView sc, RayQueue q, cons
                 active = i < n;
                 if (active)
                 {
-                    const float4 o = q.o[i], d = q.d[i];
+                    const float4 d = q.d[i];
+                    const float4 o = cam_o ? make_float4(co.x, co.y, co.z, 0.0f) : q.o[i];
                     tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_FLT_MAX);
                 }
             }
@@ -1822,19 +1832,19 @@ static inline uint32_t grid_for(uint64_t items, uint32_t block, uint32_t max_blo
 }
 
 hipError_t launch_camera(hipStream_t s, const BatchTable& bt, const GuideView& gv, const RayQueue& q, BatchCounters* bc,
-                         uint32_t threads, uint32_t blocks)
+                         uint32_t write_o, uint32_t threads, uint32_t blocks)
 {
-    hipLaunchKernelGGL(k_camera, dim3(blocks), dim3(threads), 0, s, bt, gv, q, bc);
+    hipLaunchKernelGGL(k_camera, dim3(blocks), dim3(threads), 0, s, bt, gv, q, bc, write_o);
     return hipGetLastError();
 }
 
 hipError_t launch_trace(hipStream_t s, const SceneView& sc, const RayQueue& q, const uint32_t* count, uint32_t depth,
-                        const HitBuf& hb, uint2* gstack, unsigned long long* ctr, uint32_t blocks)
+                        const HitBuf& hb, uint2* gstack, unsigned long long* ctr, const float* cam_o, uint32_t blocks)
 {
     if (sc.bvh_width == 8)
-        hipLaunchKernelGGL(k_trace<8>, dim3(blocks), dim3(256), 0, s, sc, q, count, depth, hb, gstack, ctr);
+        hipLaunchKernelGGL(k_trace<8>, dim3(blocks), dim3(256), 0, s, sc, q, count, depth, hb, gstack, ctr, cam_o);
     else
-        hipLaunchKernelGGL(k_trace<4>, dim3(blocks), dim3(256), 0, s, sc, q, count, depth, hb, gstack, ctr);
+        hipLaunchKernelGGL(k_trace<4>, dim3(blocks), dim3(256), 0, s, sc, q, count, depth, hb, gstack, ctr, cam_o);
     return hipGetLastError();
 }
 
