@@ -882,6 +882,10 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
      * float64 film sums and queue slots are taken in another order. */
     __shared__ uint32_t order[CLPT_SHADE_GROUP * 256]; /* blockDim.x <= 256 (launch bounds) */
     __shared__ uint32_t ncls[8];
+    /* the window's hit records and directions, read once by the class pass
+     * (four rays per thread in flight) and taken from LDS by the shading */
+    __shared__ float4   sh_h[CLPT_SHADE_GROUP * 256];
+    __shared__ float4   sh_d[CLPT_SHADE_GROUP * 256];
     const uint32_t      win     = CLPT_SHADE_GROUP * blockDim.x;
     const uint32_t      wstride = gridDim.x * win;
     /* (window, part): every thread takes the same sequence, so the barriers
@@ -897,11 +901,14 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
 #pragma unroll
             for (uint32_t g = 0; g < CLPT_SHADE_GROUP; ++g)
             {
-                const uint32_t i0 = wbase + g * blockDim.x + threadIdx.x;
+                const uint32_t w  = g * blockDim.x + threadIdx.x;
+                const uint32_t i0 = wbase + w;
                 cls[g]            = 7u; /* past the end */
                 if (i0 < n)
                 {
                     const float4 hh = hb.h[i0];
+                    sh_h[w]         = hh;
+                    sh_d[w]         = qin.d[i0];
                     cls[g] = hh.x >= 0.0f ? 1u + sc.mtl[sc.inst[__float_as_uint(hh.w)].material].type : 0u;
                 }
             }
@@ -919,7 +926,8 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
             }
             __syncthreads();
         }
-        const uint32_t i = order[part * blockDim.x + threadIdx.x];
+        const uint32_t i  = order[part * blockDim.x + threadIdx.x];
+        const uint32_t wi = i - wbase; /* the ray's place in the window */
         if (++part == CLPT_SHADE_GROUP)
         {
             /* the next window's sort starts with a barrier, so no thread
@@ -929,11 +937,11 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
         }
         if (i >= n)
             continue;
-        const float4   d4 = qin.d[i];
+        const float4   d4 = sh_d[wi];
         /* throughput and, in w, the previous BSDF pdf; the origin plane is
          * traversal's only */
         const float4   T4 = depth == 1u ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : qin.T[i];
-        const float4   h4 = hb.h[i];
+        const float4   h4 = sh_h[wi];
         const uint32_t path = __float_as_uint(d4.w);
         uint32_t       vi, vpix, sample;
         locate_path(bt, b, path, vi, vpix, sample);
