@@ -1332,9 +1332,16 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
     uint32_t       i      = st.gid;
     bool           active = i < n;
     Tracer<true, W> tr;
+    /* the ray's radiance and pixel, read with the ray so that a visible ray's
+     * film update needs no load when it finishes */
+    f3       lo{0.0f, 0.0f, 0.0f};
+    uint32_t lpix = 0;
     if (active)
     {
         const float4 o = sq.o[i], d = sq.d[i];
+        const float2 x = sq.x[i];
+        lo             = f3{o.w, d.w, x.x};
+        lpix           = __float_as_uint(x.y);
         tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_FLT_MAX);
     }
     while (__any(active))
@@ -1345,15 +1352,13 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
         {
             if (!tr.found)
             {
-                const float4 o = sq.o[i], d = sq.d[i];
-                const float2 x = sq.x[i];
-                accum_add(accum, __float_as_uint(x.y), f3{o.w, d.w, x.x});
+                accum_add(accum, lpix, lo);
                 vis = true;
                 if (sq.tag)
                 {
                     const uint32_t tg = sq.tag[i];
                     if (tg != 0xffffffffu)
-                        guide_E[tg] = make_float4(o.w, d.w, x.x, 0.0f);
+                        guide_E[tg] = make_float4(lo.x, lo.y, lo.z, 0.0f);
                 }
             }
             fin = true;
@@ -1371,6 +1376,9 @@ __global__ __launch_bounds__(256, 6) void k_shadow(SceneView sc, ShadowQueue sq,
                 if (active)
                 {
                     const float4 o = sq.o[i], d = sq.d[i];
+                    const float2 x = sq.x[i];
+                    lo             = f3{o.w, d.w, x.x};
+                    lpix           = __float_as_uint(x.y);
                     tr.start(sc, st, f3{o.x, o.y, o.z}, f3{d.x, d.y, d.z}, CLPT_FLT_MAX);
                 }
             }
