@@ -20,7 +20,8 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 # MI355X L2: 8 XCDs x 16 channels x 128 B per clock at 2.4 GHz
 L2_PEAK_GBS = 8 * 16 * 128 * 2.4
-QUEUE_BYTES_PER_RAY = 32 + 16 + 4  # ray o/d float4 x2 read, hit float4 + prim write
+QUEUE_BYTES_PER_RAY = 32 + 16  # ray o/d float4 x2 read, hit float4 (b1, b2, instance, primitive) written
+ORIGIN_BYTES = 16  # a camera ray of a one-view render reads no origin: it is the camera position
 NODE_BYTES = {4: 128, 8: 96}  # Node4: float child boxes; Node8: byte-quantised child boxes
 TRI_BYTES = 48
 
@@ -423,11 +424,12 @@ def main():
     # -- every ray read and its hit written once, every node and triangle
     # record of the tree fetched once -- over its event-timed launches
     per_launch_ms = trace_ms / max(1, launches)
-    compulsory = rays_traced * QUEUE_BYTES_PER_RAY + launches * c["scene_bytes"]
+    queue_bytes = rays_traced * QUEUE_BYTES_PER_RAY - int(c["rays"][1]) * ORIGIN_BYTES
+    compulsory = queue_bytes + launches * c["scene_bytes"]
     achieved = compulsory / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
     # the bytes the kernel requests from the cache hierarchy: each node visit
     # and triangle test fetches its whole record (mostly L2/MALL hits)
-    requested = rays_traced * QUEUE_BYTES_PER_RAY + nodes * NODE_BYTES[args.bvh_width] + tris * TRI_BYTES
+    requested = queue_bytes + nodes * NODE_BYTES[args.bvh_width] + tris * TRI_BYTES
     req_gbs = requested / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
     # measured per-launch counters of the same kernel and workload (rocprofv3
     # --pmc runs, summarised with their commands under profiles/)
@@ -440,7 +442,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": meas["hbm_bytes_per_launch"] if meas else None,
                 "kernel": "k_trace", "bvh_width": args.bvh_width, "avg_launch_ms": round(per_launch_ms, 4),
                 "algorithmic_bytes_per_launch": int(compulsory / max(1, launches)),
-                "algorithmic_bytes": f"{QUEUE_BYTES_PER_RAY} B per closest-hit ray (ray read, hit written) + the "
+                "algorithmic_bytes": f"{QUEUE_BYTES_PER_RAY} B per closest-hit ray (ray read, hit written; camera rays "
+                                     f"{QUEUE_BYTES_PER_RAY - ORIGIN_BYTES} B, their origin is the camera's) + the "
                                      f"tree's node and triangle records once per launch ({c['scene_bytes']} B)",
                 "nodes_per_ray": round(nodes / max(1, rays_traced), 2),
                 "tris_per_ray": round(tris / max(1, rays_traced), 2),

@@ -157,7 +157,7 @@ struct clhip_ctx
 
     /* render state */
     DevBuf   d_accum, d_out;
-    DevBuf   d_q[2][3], d_hit, d_prim, d_sq[3], d_pacc, d_gstack, d_bc, d_tot, d_ctr;
+    DevBuf   d_q[2][3], d_hit, d_sq[3], d_pacc, d_gstack, d_bc, d_tot, d_ctr;
     size_t   qcap = 0;
     uint32_t spp  = 0;
     uint32_t blocks_trace = 0, blocks_shadow = 0, blocks_shade = 0;
@@ -335,7 +335,7 @@ int sync_instances(clhip_ctx* c)
 /* Paths per batch: the launch setting, else 2^27.  A persistent traversal
  * launch finishes with its slowest lane, and the last bounces of a batch
  * hold few rays, so every batch pays a fixed cost whatever its size.  2^27
- * paths (64 spp of a 1080p film, 168 B of queues and path sums per path = 22.5 GB of the
+ * paths (64 spp of a 1080p film, 164 B of queues and path sums per path = 22.0 GB of the
  * 288 GB of HBM) make it small: 971 Msamples/s on the 1080p scene against
  * 949 at 2^26 and 913 at 2^25.  Buffers are sized by the batch a render
  * actually uses, so small films and few samples allocate less. */
@@ -359,7 +359,6 @@ int ensure_render_buffers(clhip_ctx* c, size_t paths)
             for (int k = 0; k < 3; ++k)
                 CLHIP_CHECK(c, c->d_q[q][k].alloc(paths * sizeof(float4)));
         CLHIP_CHECK(c, c->d_hit.alloc(paths * sizeof(float4)));
-        CLHIP_CHECK(c, c->d_prim.alloc(paths * sizeof(uint32_t)));
         CLHIP_CHECK(c, c->d_sq[0].alloc(paths * sizeof(float4)));
         CLHIP_CHECK(c, c->d_sq[1].alloc(paths * sizeof(float4)));
         CLHIP_CHECK(c, c->d_sq[2].alloc(paths * sizeof(float2)));
@@ -704,7 +703,7 @@ BatchArgs batch_args(clhip_ctx* c, double* accum, uint32_t max_depth, bool singl
         a.q[k].T     = c->d_q[k][2].as<float4>();
         a.q[k].count = nullptr;
     }
-    a.hb            = HitBuf{c->d_hit.as<float4>(), c->d_prim.as<uint32_t>()};
+    a.hb            = HitBuf{c->d_hit.as<float4>()};
     a.sq            = ShadowQueue{c->d_sq[0].as<float4>(), c->d_sq[1].as<float4>(), c->d_sq[2].as<float2>(), nullptr};
     a.accum         = accum;
     a.pacc          = c->d_pacc.as<float>();

@@ -199,10 +199,12 @@ struct RayQueue
     uint32_t* count;
 };
 
+/* Closest hit of each ray: b1, b2, instance (bits; CLPT_NO_HIT for a miss),
+ * original primitive index (bits).  Shading needs no hit distance. */
+#define CLPT_NO_HIT 0xffffffffu
 struct HitBuf
 {
-    float4*   h;    /* t, b1, b2, instance (bits) */
-    uint32_t* prim; /* original primitive index */
+    float4* h;
 };
 
 struct ShadowQueue

@@ -831,9 +831,8 @@ __global__ __launch_bounds__(256, 5) void k_trace(SceneView sc, RayQueue q, cons
         uint32_t vn = 0, vt = 0;
         if (active && tr.step(sc, st, tmin, vn, vt))
         {
-            hb.h[i]    = make_float4(tr.found ? tr.best.t : -1.0f, tr.best.b1, tr.best.b2,
-                                     __uint_as_float(tr.found ? tr.best.inst : 0xffffffffu));
-            hb.prim[i] = tr.found ? tr.best.prim : 0xffffffffu;
+            hb.h[i] = make_float4(tr.best.b1, tr.best.b2, __uint_as_float(tr.found ? tr.best.inst : CLPT_NO_HIT),
+                                  __uint_as_float(tr.best.prim));
             fin        = true;
         }
         sn += (unsigned long long)__popcll(__ballot(vn != 0u)) + (unsigned long long)__popcll(__ballot(vn > 1u));
@@ -909,7 +908,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
                     const float4 hh = hb.h[i0];
                     sh_h[w]         = hh;
                     sh_d[w]         = qin.d[i0];
-                    cls[g] = hh.x >= 0.0f ? 1u + sc.mtl[sc.inst[__float_as_uint(hh.w)].material].type : 0u;
+                    cls[g] = __float_as_uint(hh.z) != CLPT_NO_HIT ? 1u + sc.mtl[sc.inst[__float_as_uint(hh.z)].material].type : 0u;
                 }
             }
 #pragma unroll
@@ -951,7 +950,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
         const uint32_t z  = (fr.lane << 20) | sample;
         const f3       dir{d4.x, d4.y, d4.z};
         f3             T{T4.x, T4.y, T4.z};
-        const bool     hit = h4.x >= 0.0f;
+        const bool     hit = __float_as_uint(h4.z) != CLPT_NO_HIT;
         bool           want_shadow = false, want_next = false;
         f3             Li{0.0f, 0.0f, 0.0f}, ldir{0.0f, 0.0f, 0.0f}, ndir{0.0f, 0.0f, 0.0f}, p{0.0f, 0.0f, 0.0f};
         float          npdf = 0.0f;
@@ -974,11 +973,11 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, BatchTable bt, uint
         if (hit && depth < fr.max_depth)
         {
             HitRec hr;
-            hr.t    = h4.x;
-            hr.b1   = h4.y;
-            hr.b2   = h4.z;
-            hr.inst = __float_as_uint(h4.w);
-            hr.prim = hb.prim[i];
+            hr.t    = 0.0f; /* shading needs no distance */
+            hr.b1   = h4.x;
+            hr.b2   = h4.y;
+            hr.inst = __float_as_uint(h4.z);
+            hr.prim = __float_as_uint(h4.w);
             Surface si;
             surface_interaction(sc, hr, si);
             p                    = si.p;
@@ -1114,7 +1113,7 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
                 if (i0 < n)
                 {
                     const float4 hh = hb.h[i0];
-                    cls[g] = hh.x >= 0.0f ? 1u + sc.mtl[sc.inst[__float_as_uint(hh.w)].material].type : 0u;
+                    cls[g] = __float_as_uint(hh.z) != CLPT_NO_HIT ? 1u + sc.mtl[sc.inst[__float_as_uint(hh.z)].material].type : 0u;
                 }
             }
 #pragma unroll
@@ -1138,14 +1137,12 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
             wbase = n - wbase > wstride ? wbase + wstride : n;
         }
         const bool active = i < n;
-        float4         d4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), T4 = d4, h4 = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
-        uint32_t       prim = 0u;
+        float4 d4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), T4 = d4, h4 = make_float4(0.0f, 0.0f, __uint_as_float(CLPT_NO_HIT), 0.0f);
         if (active)
         {
-            d4   = qin.d[i];
-            T4   = depth == 1u ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : qin.T[i];
-            h4   = hb.h[i];
-            prim = hb.prim[i];
+            d4 = qin.d[i];
+            T4 = depth == 1u ? make_float4(1.0f, 1.0f, 1.0f, 0.0f) : qin.T[i];
+            h4 = hb.h[i];
         }
         const uint32_t path = __float_as_uint(d4.w);
         uint32_t       vi = 0, vpix = 0, sample = 0;
@@ -1157,7 +1154,7 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
         const uint32_t z      = (fr.lane << 20) | sample;
         const f3       dir{d4.x, d4.y, d4.z};
         f3             T{T4.x, T4.y, T4.z};
-        const bool     hit    = active && h4.x >= 0.0f;
+        const bool     hit    = active && __float_as_uint(h4.z) != CLPT_NO_HIT;
         const bool     vertex = hit && depth < fr.max_depth;
         const bool     train  = active && guide_records(path, tstr, *bt.cursor);
         /* record (depth, slot) of a training path; planes of V are (CLPT_MAXD+1) * nslots apart */
@@ -1168,11 +1165,11 @@ __global__ __launch_bounds__(256, 2) void k_shade_guided(SceneView sc, BatchTabl
         if (vertex)
         {
             HitRec hr;
-            hr.t    = h4.x;
-            hr.b1   = h4.y;
-            hr.b2   = h4.z;
-            hr.inst = __float_as_uint(h4.w);
-            hr.prim = prim;
+            hr.t    = 0.0f; /* shading needs no distance */
+            hr.b1   = h4.x;
+            hr.b2   = h4.y;
+            hr.inst = __float_as_uint(h4.z);
+            hr.prim = __float_as_uint(h4.w);
             surface_interaction(sc, hr, si);
             mt = sc.mtl[si.material];
             material_at(sc, mt, si, neg3(dir));
