@@ -2,7 +2,7 @@
  * bvh_build.cpp -- host BVH builder replacing OptiX's BLAS/TLAS build
  * (OptiXDevice::buildOptiXAccelBLASes/TLAS, optixdevice.cpp:136-233).
  *
- * Binned SAH (16 bins per axis) produces a binary tree, which is collapsed
+ * Binned SAH (32 bins per axis) produces a binary tree, which is collapsed
  * to a 4- or 8-wide tree by repeatedly opening the child with the largest
  * surface area.  BVH4 nodes are 128-byte Node4 records with float boxes
  * (eight 16-byte loads); BVH8 nodes are 96-byte Node8 records with child
@@ -50,7 +50,7 @@ struct Box
     }
 };
 
-constexpr int      kBins         = 16;
+constexpr int      kBins         = 32;
 constexpr uint32_t kMaxSahDepth  = 48; /* deeper subtrees split at the median */
 constexpr double   kTraverseCost = 1.0;
 
