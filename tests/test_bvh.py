@@ -1,5 +1,6 @@
-"""BVH8 builder on the host (no GPU): quantised child boxes are conservative
-and every primitive lands in exactly one leaf (tests/bvh_check.cpp)."""
+"""BVH4 and BVH8 builders on the host (no GPU): child boxes (quantised, for
+BVH8) contain their subtrees, leaves respect the leaf cap, and every
+primitive lands in exactly one leaf (tests/bvh_check.cpp)."""
 import os
 import subprocess
 
@@ -8,7 +9,7 @@ from colvillea_amd import build as hipbuild
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_bvh8_quantised_boxes_are_conservative(tmp_path):
+def test_bvh_boxes_are_conservative_and_leaves_capped(tmp_path):
     exe = str(tmp_path / "bvh_check")
     src = os.path.join(REPO, "tests", "bvh_check.cpp")
     lib = os.path.join(REPO, "colvillea_amd", "csrc", "bvh_build.cpp")
