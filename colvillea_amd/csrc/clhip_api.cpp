@@ -136,6 +136,7 @@ struct clhip_ctx
     std::vector<InstanceRec> inst;
     std::vector<uint32_t>    inst_entity;
     uint32_t                 tlas_root = 0, tlas_depth = 0;
+    uint32_t                 world_inst = 0xffffffffu; /* instance whose mesh subtrees the top tree holds */
     bool                     inst_dirty = true;
 
     /* materials / entities */
@@ -299,8 +300,9 @@ SceneView scene_view(const clhip_ctx* c)
     s.cdfv      = c->d_cdfv.as<const float>();
     s.cutv      = c->d_cutv.as<const uint32_t>();
     s.cutcond   = c->d_cutcond.as<const uint32_t>();
-    s.tlas_root = c->tlas_root;
-    s.ninst     = (uint32_t)c->inst.size();
+    s.tlas_root  = c->tlas_root;
+    s.world_inst = c->world_inst;
+    s.ninst      = (uint32_t)c->inst.size();
     return s;
 }
 
@@ -981,6 +983,86 @@ struct Tmp
         return hipMemcpy(h, b.p, count * sizeof(T), hipMemcpyDeviceToHost);
     }
 };
+/* Partial re-braiding (Benthin, Woop, Wald and Afra, "Improved Two-Level
+ * BVHs using Partial Re-Braiding", HPG 2017) for one instance: the top
+ * tree is built over the largest subtrees of that instance's mesh tree in
+ * place of the instance's one box, so that its parts and the other
+ * instances interleave in one hierarchy.  Only an instance whose transform
+ * is the identity qualifies (its subtrees are valid in world space as they
+ * are); traversal tests their triangles with the world ray and records
+ * their hits under SceneView::world_inst.  The parts are found by opening
+ * inner nodes from the mesh tree's root, largest surface area first, until
+ * kRebraidParts are held; each keeps the box its parent stores for it (for
+ * BVH8 the decoded box, which contains the subtree). */
+constexpr size_t kRebraidParts = 64;
+
+struct TopPrim
+{
+    uint32_t ref;
+    float    lo[3], hi[3];
+};
+
+void child_box(const Node4& n, int c, float lo[3], float hi[3])
+{
+    lo[0] = n.lox[c];
+    lo[1] = n.loy[c];
+    lo[2] = n.loz[c];
+    hi[0] = n.hix[c];
+    hi[1] = n.hiy[c];
+    hi[2] = n.hiz[c];
+}
+
+void child_box(const Node8& n, int c, float lo[3], float hi[3])
+{
+    for (int a = 0; a < 3; ++a)
+    {
+        lo[a] = clpt::node8_lo(n, a, c);
+        hi[a] = clpt::node8_hi(n, a, c);
+    }
+}
+
+template <typename NodeT, int W>
+void rebraid_parts(const std::vector<NodeT>& nodes, uint32_t root, std::vector<TopPrim>& out)
+{
+    std::vector<TopPrim> parts;
+    const auto           open = [&](uint32_t node) {
+        const NodeT& n = nodes[node];
+        for (int c = 0; c < W; ++c)
+            if (n.child[c] != CLPT_CHILD_EMPTY)
+            {
+                TopPrim p;
+                p.ref = n.child[c];
+                child_box(n, c, p.lo, p.hi);
+                parts.push_back(p);
+            }
+    };
+    open(root);
+    while (parts.size() < kRebraidParts)
+    {
+        size_t pick = parts.size();
+        double best = -1.0;
+        for (size_t k = 0; k < parts.size(); ++k)
+        {
+            if (parts[k].ref & CLPT_LEAF_BIT)
+                continue;
+            const double dx = (double)parts[k].hi[0] - parts[k].lo[0], dy = (double)parts[k].hi[1] - parts[k].lo[1],
+                         dz = (double)parts[k].hi[2] - parts[k].lo[2];
+            const double area = dx * dy + dy * dz + dz * dx;
+            if (area > best)
+            {
+                best = area;
+                pick = k;
+            }
+        }
+        if (pick == parts.size())
+            break; /* only leaves left */
+        const uint32_t node = parts[pick].ref;
+        parts[pick]         = parts.back();
+        parts.pop_back();
+        open(node);
+    }
+    out.insert(out.end(), parts.begin(), parts.end());
+}
 } // namespace
 
 extern "C" {
@@ -1401,13 +1483,17 @@ int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uin
     std::vector<uint32_t>    ent(num_instances);
     std::vector<float>       lo(3 * (size_t)num_instances), hi(3 * (size_t)num_instances);
     uint32_t                 max_blas_depth = 0;
+    static const float       id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    /* the re-braided instance: the most triangles among those whose
+     * transform and inverse are the identity (by value: the computed
+     * inverse holds -0 translations) */
+    uint32_t                 flat = 0xffffffffu;
     for (uint32_t i = 0; i < num_instances; ++i)
     {
         const uint32_t m = mesh_of_instance[i];
         if (m >= ctx->meshes.size())
             return fail(ctx, CLHIP_E_INVALID, "instance %u: mesh %u of %zu", i, m, ctx->meshes.size());
         InstanceRec& r = inst[i];
-        static const float id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
         std::memcpy(r.xf, xforms ? xforms + 12 * (size_t)i : id, sizeof(r.xf));
         inverse3x4(r.xf, r.inv);
         for (int k = 0; k < 12; ++k)
@@ -1420,6 +1506,11 @@ int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uin
         ent[i]      = r.entity;
         max_blas_depth = std::max(max_blas_depth, ctx->meshes[m].depth);
         const MeshInfo& mi = ctx->meshes[m];
+        bool            identity = true;
+        for (int k = 0; k < 12; ++k)
+            identity = identity && r.xf[k] == id[k] && r.inv[k] == id[k];
+        if (mi.ntris > 0 && identity && (flat == 0xffffffffu || mi.ntris > ctx->meshes[inst[flat].mesh].ntris))
+            flat = i;
         for (int k = 0; k < 3; ++k)
         {
             lo[3 * (size_t)i + k] = INFINITY;
@@ -1445,20 +1536,51 @@ int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uin
             }
         }
     }
+    /* the top tree's primitives: every other instance's box, and the
+     * re-braided instance's parts */
+    std::vector<TopPrim> prims;
+    prims.reserve(num_instances + kRebraidParts + 8);
+    for (uint32_t i = 0; i < num_instances; ++i)
+        if (i != flat)
+        {
+            TopPrim p;
+            p.ref = CLPT_INST_LEAF(i);
+            for (int k = 0; k < 3; ++k)
+            {
+                p.lo[k] = lo[3 * (size_t)i + k];
+                p.hi[k] = hi[3 * (size_t)i + k];
+            }
+            prims.push_back(p);
+        }
+    if (flat != 0xffffffffu)
+    {
+        if (ctx->bvh_width == 8)
+            rebraid_parts<Node8, 8>(ctx->blas_nodes8, inst[flat].blas_root, prims);
+        else
+            rebraid_parts<Node4, 4>(ctx->blas_nodes4, inst[flat].blas_root, prims);
+    }
+    const uint32_t     np = (uint32_t)prims.size();
+    std::vector<float> plo(3 * (size_t)np), phi(3 * (size_t)np);
+    for (uint32_t p = 0; p < np; ++p)
+        for (int k = 0; k < 3; ++k)
+        {
+            plo[3 * (size_t)p + k] = prims[p].lo[k];
+            phi[3 * (size_t)p + k] = prims[p].hi[k];
+        }
     std::vector<uint32_t> order;
-    const auto            leaf = [&order](uint32_t first, uint32_t) { return CLPT_INST_LEAF(order[first]); };
+    const auto            leaf = [&order, &prims](uint32_t first, uint32_t) { return prims[order[first]].ref; };
     std::vector<Node4>    nodes4;
     std::vector<Node8>    nodes8;
     clpt::BvhResult       r;
     if (ctx->bvh_width == 8)
     {
         nodes8 = ctx->blas_nodes8;
-        r      = clpt::build_bvh8(lo.data(), hi.data(), num_instances, 1u, nodes8, order, leaf);
+        r      = clpt::build_bvh8(plo.data(), phi.data(), np, 1u, nodes8, order, leaf);
     }
     else
     {
         nodes4 = ctx->blas_nodes4;
-        r      = clpt::build_bvh4(lo.data(), hi.data(), num_instances, 1u, nodes4, order, leaf);
+        r      = clpt::build_bvh4(plo.data(), phi.data(), np, 1u, nodes4, order, leaf);
     }
     /* stack bound: each W-wide level leaves at most W - 1 entries behind,
      * plus the popped entry and a margin */
@@ -1482,6 +1604,7 @@ int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uin
     ctx->inst_entity = std::move(ent);
     ctx->tlas_root   = r.root;
     ctx->tlas_depth  = r.max_depth;
+    ctx->world_inst  = flat;
     for (int k = 0; k < 3; ++k)
     {
         ctx->world_lo[k] = num_instances ? r.lo[k] : 0.0f;

@@ -154,6 +154,8 @@ struct SceneView
     const uint32_t*    cutv;    /* CLPT_CUTS + 1 search cut points of cdfv */
     const uint32_t*    cutcond; /* CLPT_CUTS + 1 per row of cdfcond */
     uint32_t           tlas_root;
+    uint32_t           world_inst; /* the instance whose mesh subtrees the top tree holds (its
+                                      transform is the identity), or 0xffffffff */
     uint32_t           ninst;
     uint32_t           bvh_width; /* 4 (Node4) or 8 (Node8) */
     uint32_t           node_bytes; /* size of the node array (below 4 GiB) */

@@ -340,7 +340,13 @@ __device__ __forceinline__ bool visit_node<8>(const SceneView& sc, const RayW& r
  * the visiting order (ties are broken by (t, instance, primitive)).
  * ANY = occlusion: stop at the first hit (D15: optix.cu:106 traces shadow
  * rays with closest-hit flags).  Inside an instance the object-space ray
- * replaces the world ray.  The stack depth at instance entry marks where the
+ * replaces the world ray.  The top tree may also hold subtrees of the mesh
+ * tree of one instance whose transform is the identity
+ * (SceneView::world_inst), whose triangles are tested with the world ray
+ * itself: the world ray is taken with every -0 component made +0, and for
+ * such a ray the instance's stored inverse (ones on the diagonal, zeros of
+ * either sign elsewhere) reproduces every component exactly, so those
+ * triangles see the bits an instance entry would give them.  The stack depth at instance entry marks where the
  * instance's entries begin: popping an entry below it leaves the instance,
  * and the world ray is restored as soon as no leaf of the instance is held
  * (a held leaf's triangles still need the object-space ray).
@@ -367,9 +373,10 @@ struct Tracer
 
     __device__ __forceinline__ void start(const SceneView& sc, TraverseStack& st, f3 o, f3 d, float t1)
     {
-        rw.o      = o;
-        rw.d      = d;
-        cur_inst  = 0xffffffffu;
+        /* x + 0 turns -0 into +0 and keeps every other value */
+        rw.o      = f3{o.x + 0.0f, o.y + 0.0f, o.z + 0.0f};
+        rw.d      = f3{d.x + 0.0f, d.y + 0.0f, d.z + 0.0f};
+        cur_inst  = sc.world_inst;
         best.t    = t1;
         best.b1   = 0.0f;
         best.b2   = 0.0f;
@@ -398,7 +405,7 @@ struct Tracer
             {
                 /* back in the top tree: the world ray again */
                 r        = rw;
-                cur_inst = 0xffffffffu;
+                cur_inst = sc.world_inst;
                 inst_sp  = 0;
             }
             if (enter || inst_sp == -2)
