@@ -1605,10 +1605,18 @@ int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uin
     ctx->tlas_root   = r.root;
     ctx->tlas_depth  = r.max_depth;
     ctx->world_inst  = flat;
+    /* the scene box (it normalises the learned proposal's positions) is the
+     * union of the instance boxes, whatever parts the top tree holds */
     for (int k = 0; k < 3; ++k)
     {
-        ctx->world_lo[k] = num_instances ? r.lo[k] : 0.0f;
-        ctx->world_hi[k] = num_instances ? r.hi[k] : 1.0f;
+        float wl = INFINITY, wh = -INFINITY;
+        for (uint32_t i = 0; i < num_instances; ++i)
+        {
+            wl = std::min(wl, lo[3 * (size_t)i + k]);
+            wh = std::max(wh, hi[3 * (size_t)i + k]);
+        }
+        ctx->world_lo[k] = num_instances ? wl : 0.0f;
+        ctx->world_hi[k] = num_instances ? wh : 1.0f;
     }
     ctx->have_tlas   = true;
     ctx->inst_dirty  = true;
