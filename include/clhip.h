@@ -193,7 +193,10 @@ int clhip_build_blas_subset(clhip_ctx* ctx, const clhip_mesh* meshes, const uint
  * instance per entity.  instance_ids[i] is the entity index of instance i
  * (the reference passes entity indices, scene.cpp:232-236).  xforms is NULL
  * (identity, as in the reference) or num_instances row-major 3x4
- * object-to-world matrices (an extension). */
+ * object-to-world matrices (an extension).  The top tree holds the 64
+ * largest subtrees of the identity-transformed instance with the most
+ * triangles in place of that instance's one box (partial re-braiding);
+ * results do not depend on it. */
 int clhip_build_tlas(clhip_ctx* ctx, const uint32_t* mesh_of_instance, const uint32_t* instance_ids,
                      const float* xforms, uint32_t num_instances);
 
